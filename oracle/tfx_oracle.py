@@ -1,0 +1,816 @@
+"""
+FP64 NumPy restatement of Tensor Fox's dense CPD path (test infrastructure only).
+
+This module is the ORACLE: a from-scratch CPU restatement, in float64 NumPy, of
+the algorithm behind the reference's ``cpd(T, R, options)`` for dense tensors
+(MLSVD compression -> random init -> damped Gauss-Newton with preconditioned CG
+-> uncompression -> optional refinement -> final error).  Only ``tests/``, the
+``smoke()`` check and the CPU-baseline leg of ``bench.py`` may import it, and
+only as the checker / CPU baseline.  The product (the HIP path under
+``tensor-fox_amd/``) never imports it.
+
+Every function cites the reference function it follows (paths relative to the
+reference checkout).  Behaviour that parity depends on is reproduced on
+purpose, including these quirks of the reference:
+
+* ``regularization`` scales by max|P1| of the LAST mode, not the running max
+  (GaussNewton.py:594-600).
+* ``x`` is never re-synchronised with the equalized factors
+  (GaussNewton.py:285-289, Conversion.py:39-47).
+* CG solves the split-preconditioned system with damping ``damp*Gamma`` and
+  returns ``M*y`` (GaussNewton.py:349-359, 379).
+* In ``cg`` the first search direction aliases the residual array
+  (``P = residual_cg``, GaussNewton.py:349-350), so the in-place residual
+  update makes the second direction ``(1+beta)*r1`` (GaussNewton.py:388-392).
+* dGN returns the BEST factors; with refinement the reported ``rel_error`` is
+  the LAST refinement error (GaussNewton.py:178-180, Auxiliar.py:148-149).
+* Main-stage errors are relative to ||S|| (the core), GaussNewton.py:106.
+
+Randomness uses NumPy's global legacy RNG in the reference's order:
+range-finder Gaussians per mode (Compression.py:379-382), ``randn`` init per
+mode (Initialization.py:62), one ``randint`` per dGN iteration for the CG cap
+(GaussNewton.py:267).  The zero-cleaning draw of ``clean_mode_l`` runs under
+Numba's private RNG in the reference (Initialization.py:375-386); here it uses
+a private ``RandomState`` so that it never disturbs the global stream either.
+
+Deviation: ``consistency`` does not sleep for one second
+(Auxiliar.py:76).
+
+Parity status per function is recorded in the tests that pin it
+(tests/test_oracle_*.py); every function here has at least one pin that is not
+a retyping of its own formula (closed forms, brute-force einsum, an explicit
+Jacobian, finite differences, textbook CG on tiny systems, the reference's
+doctest thresholds and notebook outputs).
+"""
+
+import sys
+from copy import deepcopy
+
+import numpy as np
+from numpy.linalg import norm
+from scipy.linalg import qr, svd
+
+_CLEAN_RNG = np.random.RandomState(12345)
+
+
+# ----------------------------------------------------------------------------------------------
+# Options / consistency / output (Auxiliar.py)
+# ----------------------------------------------------------------------------------------------
+
+_DEFAULTS = dict(
+    maxiter=200, tol=1e-16, tol_step=1e-6, tol_improv=1e-6, tol_grad=1e-6, tol_jump=10,
+    method='dGN', inner_method='cg', cg_maxiter=100, cg_factor=1, cg_tol=1e-16,
+    bi_method_parameters=None, initialization='random', trunc_dims=0, mlsvd_method='seq',
+    tol_mlsvd=1e-16, init_damp=1, refine=False, symm=False, factors_norm=0, trials=3,
+    display=0, epochs=1, gpu=False, mkl_dot=True, svd_n_components=None)
+
+
+class _Options:
+    pass
+
+
+def make_options(options):
+    """Defaults + attribute-presence overrides.  Follows Auxiliar.py:242-362."""
+    out = _Options()
+    for k, v in _DEFAULTS.items():
+        setattr(out, k, v)
+    out.bi_method_parameters = ['als', 500, 1e-6]
+    names = dir(options)
+    for k in _DEFAULTS:
+        if k != 'bi_method_parameters' and k in names:
+            setattr(out, k, getattr(options, k))
+    if 'bi_method' in names:
+        out.bi_method_parameters[0] = options.bi_method
+        out.bi_method_parameters[1] = {'cg': 1, 'cg_static': 100, 'als': 500}.get(
+            options.bi_method, out.bi_method_parameters[1])
+    if 'bi_method_maxiter' in names:
+        out.bi_method_parameters[1] = options.bi_method_maxiter
+    if 'bi_method_tol' in names:
+        out.bi_method_parameters[2] = options.bi_method_tol
+    if out.gpu and 'mlsvd_method' not in names:
+        out.mlsvd_method = 'gpu'
+    return out
+
+
+def consistency(R, dims, options):
+    """Input validation with SystemExit messages.  Follows Auxiliar.py:24-78 (no sleep)."""
+    L = len(dims)
+    if L > 12:
+        sys.exit('Tensor Fox does not work with tensors of order higher than 12.')
+    for l in range(L):
+        if dims[l] == 1:
+            sys.exit('At least one dimension is equal to 1. This situation not supported by Tensor Fox.')
+    if (type(R) != int) or (R < 1):
+        sys.exit('Rank must be a positive integer.')
+    if L == 3:
+        m, n, p = int(dims[0]), int(dims[1]), int(dims[2])
+        if R > min(m * n, m * p, n * p):
+            sys.exit('Rank must satisfy 1 <= rank <= min(m*n, m*p, n*p) = ' + str(min(m * n, m * p, n * p)) + '.')
+    if L > 3 and R == 1:
+        sys.exit('Rank must be greater than 1 for tensor with order greater than 3.')
+    if options.symm:
+        for i in range(L):
+            for j in range(L):
+                if dims[i] != dims[j]:
+                    sys.exit('Symmetric tensors must have equal dimensions.')
+    if options.method not in ('dGN', 'als', 'ttcpd'):
+        sys.exit("Wrong method name. Must be 'dGN', 'als' or 'ttcpd'.")
+
+
+def sort_dims(T):
+    """Descending-dimension permutation as a view.  Follows Auxiliar.py:93-115 (dense branch)."""
+    dims = np.array(T.shape)
+    L = len(dims)
+    ordering = np.argsort(-dims)
+    return np.moveaxis(T, ordering, np.arange(-L, 0)), ordering
+
+
+def unsort_dims(factors, ordering):
+    """Inverse of sort_dims on the factor list.  Follows Auxiliar.py:118-128."""
+    new = [None] * len(factors)
+    for l in range(len(factors)):
+        new[ordering[l]] = factors[l]
+    return new
+
+
+class Output:
+    """Fields of the reference output class.  Follows Auxiliar.py:131-220."""
+
+    def __init__(self, num_steps, rel_error, step_sizes, errors, improv, gradients, stop, options):
+        self.num_steps = num_steps
+        self.rel_error = rel_error
+        self.accuracy = max(0, 100 * (1 - rel_error))
+        self.step_sizes = step_sizes
+        self.errors = errors
+        self.improv = improv
+        self.gradients = gradients
+        self.stop = stop
+        self.options = options
+
+
+# ----------------------------------------------------------------------------------------------
+# Multilinear algebra and conversions (Conversion.py, Critical.py, MultilinearAlgebra.py)
+# ----------------------------------------------------------------------------------------------
+
+def unfold(T, mode):
+    """Kolda mode-`mode` unfolding (1-based): column index = remaining indices, first remaining
+    fastest.  Follows Critical.py:123-150 (order 3) and the generated orders up to 12."""
+    n = mode - 1
+    return np.reshape(np.moveaxis(T, n, 0), (T.shape[n], -1), order='F')
+
+
+def foldback(Tl, mode, dims):
+    """Inverse of unfold.  Follows Conversion.py:305-315 / Critical.py:1293-1365."""
+    n = mode - 1
+    rest = [dims[m] for m in range(len(dims)) if m != n]
+    X = np.reshape(Tl, [dims[n]] + rest, order='F')
+    return np.ascontiguousarray(np.moveaxis(X, 0, n))
+
+
+def khatri_rao(A, B):
+    """M[i*b1 + k, r] = A[i, r] * B[k, r].  Follows MultilinearAlgebra.py:248-273."""
+    a1, R = A.shape
+    b1 = B.shape[0]
+    return (A[:, None, :] * B[None, :, :]).reshape(a1 * b1, R)
+
+
+def cpd2unfold1(factors):
+    """First unfolding of [[F_1..F_L]].  Follows Conversion.py:117-141."""
+    M = factors[1]
+    for l in range(2, len(factors)):
+        M = khatri_rao(factors[l], M)
+    return factors[0] @ M.T
+
+
+def cpd2tens(factors):
+    """Dense tensor of [[F_1..F_L]].  Follows Conversion.py:52-79."""
+    dims = [f.shape[0] for f in factors]
+    return foldback(cpd2unfold1(factors), 1, dims)
+
+
+def fastnorm(A, B):
+    """Frobenius norm of A - B.  Follows Critical.py:14-22."""
+    return float(np.sqrt(np.sum((A - B) ** 2)))
+
+
+def equalize(factors, R):
+    """Per rank-one term, give every mode the norm (prod ||f_l^r||)^(1/L).
+    Follows Conversion.py:361-387 (in place)."""
+    L = len(factors)
+    for r in range(R):
+        norm_r = np.array([norm(factors[l][:, r]) for l in range(L)])
+        if np.prod(norm_r) != 0.0:
+            num = np.prod(norm_r) ** (1 / L)
+            for l in range(L):
+                factors[l][:, r] = (num / norm_r[l]) * factors[l][:, r]
+    return factors
+
+
+def transform(factors, symm, factors_norm):
+    """Symmetric averaging and fixed factor norm.  Follows Conversion.py:412-444, including the
+    in-place accumulation into factors[0]."""
+    L = len(factors)
+    if symm:
+        s = factors[0]
+        for l in range(1, L):
+            s += factors[l]
+        factors[0] = s / L
+        for l in range(1, L):
+            factors[l] = factors[0]
+    if factors_norm > 0:
+        for l in range(L):
+            factors[l] = factors_norm * (1 / norm(factors[l])) * factors[l]
+    return factors
+
+
+def x2cpd(x, factors, eq=True):
+    """Write the flat point x (column-major per factor) into the factors, then equalize.
+    Follows Conversion.py:21-49 (in place)."""
+    R = factors[0].shape[1]
+    s = 0
+    for l in range(len(factors)):
+        dim = factors[l].shape[0]
+        for r in range(R):
+            factors[l][:, r] = x[s:s + dim]
+            s += dim
+    if eq:
+        factors = equalize(factors, R)
+    return factors
+
+
+def multilin_mult(UT, T1, dims):
+    """(U_1^T, ..., U_L^T) * T from the first unfolding.  Follows MultilinearAlgebra.py:72-105."""
+    L = len(dims)
+    S_dims = list(dims)
+    unf = T1
+    for l in range(L):
+        unf2 = UT[l] @ unf
+        S_dims[l] = UT[l].shape[0]
+        S = foldback(unf2, l + 1, S_dims)
+        if l < L - 1:
+            unf = unfold(S, l + 2)
+        else:
+            return S
+
+
+# ----------------------------------------------------------------------------------------------
+# MLSVD (Compression.py)
+# ----------------------------------------------------------------------------------------------
+
+def randomized_range_finder(A, size, n_iter):
+    """Gaussian sketch, n_iter un-normalised power iterations, economic QR.
+    Follows Compression.py:337-397 (draws from the global legacy RNG)."""
+    Q = np.random.mtrand._rand.normal(size=(A.shape[1], size))
+    for _ in range(n_iter):
+        Q = A @ Q
+        Q = A.T @ Q
+    Q, _ = qr(A @ Q, mode='economic')
+    return Q
+
+
+def svd_flip(u, v, u_based_decision=True):
+    """Sign convention: largest-|.| loading positive.  Follows Compression.py:521-564."""
+    if u_based_decision:
+        idx = np.argmax(np.abs(u), axis=0)
+        signs = np.sign(u[idx, range(u.shape[1])])
+    else:
+        idx = np.argmax(np.abs(v), axis=1)
+        signs = np.sign(v[range(v.shape[0]), idx])
+    u *= signs
+    v *= signs[:, np.newaxis]
+    return u, v
+
+
+def randomized_svd(M, n_components, n_oversamples=10, n_iter=2):
+    """Halko et al. Alg. 4.3/4.4 with transpose='auto' and flip_sign=True.
+    Follows Compression.py:400-518."""
+    n_random = n_components + n_oversamples
+    n_samples, n_features = M.shape
+    transpose = n_samples < n_features
+    if transpose:
+        M = M.T
+    Q = randomized_range_finder(M, n_random, n_iter)
+    B = Q.T @ M
+    Uhat, s, V = svd(B, full_matrices=False)
+    U = Q @ Uhat
+    if not transpose:
+        U, V = svd_flip(U, V)
+    else:
+        U, V = svd_flip(U, V, u_based_decision=False)
+    if transpose:
+        return V[:n_components, :].T, s[:n_components], U[:, :n_components].T
+    return U[:, :n_components], s[:n_components], V[:n_components, :]
+
+
+def clean_compression(U, sigma, Vt, tol_mlsvd, L):
+    """Smallest i with sum(sigma[i:]^2)/sum(sigma^2) < tol_mlsvd/L.  Follows Compression.py:261-301."""
+    eps = tol_mlsvd / L
+    for i in range(1, len(sigma)):
+        if np.sum(sigma[i:] ** 2) / np.sum(sigma ** 2) < eps:
+            sigma = sigma[:i]
+            U = U[:, :i]
+            Vt = Vt[:i, :]
+            break
+    return U, sigma, Vt, sigma.size
+
+
+def compute_svd(Tl, dims, R, tol_mlsvd, L, l, svd_n_components):
+    """Truncated rSVD of one unfolding.  Follows Compression.py:192-237 (dense CPU branch);
+    note n_components uses the ORIGINAL dims (Compression.py:197)."""
+    low_rank = min(R, dims[l])
+    n_components = low_rank if svd_n_components is None else svd_n_components
+    Ul, sigma_l, Vlt = randomized_svd(Tl, n_components, n_oversamples=10, n_iter=2)
+    return clean_compression(Ul, sigma_l, Vlt, tol_mlsvd, L)
+
+
+def _pick_tol(tol_mlsvd, L):
+    if type(tol_mlsvd) == list:
+        return tol_mlsvd[0] if L > 3 else tol_mlsvd[1]
+    return tol_mlsvd
+
+
+def mlsvd(T, Tsize, R, options):
+    """Truncated MLSVD: 'seq' (ST-HOSVD) or 'classic'.  Follows Compression.py:48-189 (dense)."""
+    options = make_options(options)
+    dims = T.shape
+    L = len(dims)
+    tol_mlsvd = _pick_tol(options.tol_mlsvd, L)
+    trunc_dims = options.trunc_dims
+    U, sigmas = [], []
+    if tol_mlsvd == -1:
+        return np.array(T), [np.identity(dims[l]) for l in range(L)], unfold(T, 1), \
+            [np.ones(dims[l]) for l in range(L)]
+    if options.mlsvd_method == 'seq':
+        S_dims = list(dims)
+        S = np.asarray(T)
+        T1 = None
+        for l in range(L):
+            Sl = unfold(S, l + 1)
+            if l == 0:
+                T1 = np.ascontiguousarray(Sl)
+            Ul, sl, Vlt, dim = compute_svd(Sl, dims, R, tol_mlsvd, L, l, options.svd_n_components)
+            U.append(Ul)
+            sigmas.append(sl)
+            Sl = (Vlt.T * sl).T
+            S_dims[l] = dim
+            S = foldback(Sl, l + 1, S_dims)
+    elif options.mlsvd_method == 'classic':
+        T1 = None
+        for l in range(L):
+            Tl = unfold(T, l + 1)
+            if l == 0:
+                T1 = np.ascontiguousarray(Tl)
+            Ul, sl, Vlt, dim = compute_svd(Tl, dims, R, tol_mlsvd, L, l, options.svd_n_components)
+            U.append(Ul)
+            sigmas.append(sl)
+        S = multilin_mult([u.T for u in U], T1, dims)
+    else:
+        raise ValueError("mlsvd_method must be 'seq' or 'classic' on the dense path")
+    if type(trunc_dims) == list:
+        slices = []
+        for l in range(L):
+            slices.append(slice(0, trunc_dims[l]))
+            U[l] = U[l][:, :min(trunc_dims[l], U[l].shape[1])]
+        S = S[tuple(slices)]
+    return S, U, T1, sigmas
+
+
+# ----------------------------------------------------------------------------------------------
+# Initialization (Initialization.py)
+# ----------------------------------------------------------------------------------------------
+
+def clean_zeros(init_factors, dims, R):
+    """Replace exact zeros by 1e-4*randn*||F||.  Follows Initialization.py:361-386 (private RNG)."""
+    for l in range(len(dims)):
+        F = init_factors[l]
+        for i in range(dims[l]):
+            for r in range(R):
+                if F[i, r] == 0.0:
+                    F[i, r] = 1e-4 * _CLEAN_RNG.randn() * norm(F)
+    return init_factors
+
+
+def starting_point(S, U, R, ordering, options):
+    """'random' or user-list initialization.  Follows Initialization.py:20-94."""
+    init = options.initialization
+    dims = S.shape
+    L = len(dims)
+    if type(init) == list:
+        return [U[l].T @ init[ordering[l]] for l in range(L)]
+    if init == 'random':
+        F = [np.random.randn(dims[l], R) for l in range(L)]
+    else:
+        sys.exit('Error with init parameter initialization.')
+    F = clean_zeros(F, dims, R)
+    F = equalize(F, R)
+    return transform(F, options.symm, options.factors_norm)
+
+
+# ----------------------------------------------------------------------------------------------
+# Damped Gauss-Newton (GaussNewton.py)
+# ----------------------------------------------------------------------------------------------
+
+def gramians(factors):
+    """Gr_l = F_l^T F_l, P2[l][m] = prod_{k != l,m} Gr_k, P1[l] = prod_{k != l} Gr_k.
+    Follows GaussNewton.py:520-548."""
+    L = len(factors)
+    R = factors[0].shape[1]
+    Gr = np.array([f.T @ f for f in factors])
+    P2 = np.ones((L, L, R, R))
+    for l in range(L):
+        for m in range(L):
+            if l != m:
+                for k in range(L):
+                    if k != l and k != m:
+                        P2[l, m] *= Gr[k]
+    P1 = np.ones((L, R, R))
+    for l in range(L):
+        for k in range(L):
+            if k != l:
+                P1[l] *= Gr[k]
+    return Gr, P1, P2
+
+
+def _split(v, dims, R):
+    out, s = [], 0
+    for d in dims:
+        out.append(v[s:s + R * d].reshape(R, d).T)   # I_l x R view of a column-major block
+        s += R * d
+    return out
+
+
+def matvec(factors, v):
+    """(J^T J) v for the CPD Jacobian: result_l = V_l P1_l + F_l sum_{m != l} P2[l][m] o (V_m^T F_m).
+    Follows GaussNewton.py:342-346, 551-581 (A[m] = V_m^T F_m, B[l] = V_l P1[l])."""
+    _, P1, P2 = gramians(factors)
+    return _matvec_P(factors, P1, P2, v)
+
+
+def regularization(P1, dims):
+    """gamma[l, r] = max|P1_{L-1}| * sqrt|P1_l[r, r]| (the last mode's max, a reference quirk);
+    Gamma repeats gamma[l, r] over the I_l rows of column r.  Follows GaussNewton.py:584-606."""
+    L, R = P1.shape[0], P1.shape[1]
+    tmp = np.max(np.abs(P1[L - 1]))
+    gamma = tmp * np.sqrt(np.abs(np.array([np.diag(P1[l]) for l in range(L)])))
+    Gamma = np.concatenate([np.repeat(gamma[l], dims[l]) for l in range(L)])
+    return Gamma, gamma
+
+
+def precond(Gamma, gamma, damp, dims):
+    """M = 1/sqrt(gamma^2 + damp^2 Gamma^2).  Follows GaussNewton.py:609-627."""
+    g = np.concatenate([np.repeat(gamma[l], dims[l]) for l in range(len(dims))])
+    return 1 / np.sqrt(g ** 2 + damp ** 2 * Gamma ** 2)
+
+
+def compute_grad(T, factors, P1):
+    """grad f = vec(F_l P1_l - T_(l) KR_l) with KR_l the Khatri-Rao product of the other factors in
+    reversed mode order.  Follows GaussNewton.py:401-427."""
+    L = len(factors)
+    g = []
+    for l in range(L):
+        itr = [m for m in reversed(range(L)) if m != l]
+        M = factors[itr[0]]
+        for ll in range(L - 2):
+            M = khatri_rao(M, factors[itr[ll + 1]])
+        N = unfold(T, l + 1) @ M
+        g.append((factors[l] @ P1[l] - N).T.ravel())
+    return np.concatenate(g)
+
+
+def mttkrp(T, factors, l):
+    """T_(l) times the Khatri-Rao product of the other factors (reversed order), the MTTKRP
+    inside compute_grad (GaussNewton.py:411-423)."""
+    L = len(factors)
+    itr = [m for m in reversed(range(L)) if m != l]
+    M = factors[itr[0]]
+    for ll in range(L - 2):
+        M = khatri_rao(M, factors[itr[ll + 1]])
+    return unfold(T, l + 1) @ M
+
+
+def cg(T, factors, damp, maxiter, tol):
+    """Preconditioned CG on (J^T J + damp Gamma) y = -grad f, returning M*y.
+    Follows GaussNewton.py:315-359 and cg_iterations GaussNewton.py:362-398, including the
+    aliasing of P and residual_cg in the first iteration."""
+    L = len(factors)
+    R = factors[0].shape[1]
+    dims = [f.shape[0] for f in factors]
+    n = R * sum(dims)
+    if n > 100:
+        maxiter = min(maxiter, n)
+    else:
+        tol = 0
+    Gr, P1, P2 = gramians(factors)
+    Gamma, gamma = regularization(P1, dims)
+    M = precond(Gamma, gamma, damp, dims)
+    y = np.zeros(n)
+    grad = -compute_grad(T, factors, P1)
+    JT_J_grad = _matvec_P(factors, P1, P2, grad)
+    r = M * grad
+    P = r                      # alias, as in the reference
+    rn = float(r @ r)
+    if rn == 0.0:
+        rn = 1e-6
+    itn = 0
+    for itn in range(maxiter):
+        Q = M * P
+        z = _matvec_P(factors, P1, P2, Q) + damp * Gamma * Q
+        z = M * z
+        den = float(P @ z)
+        if den == 0.0:
+            den = 1e-6
+        alpha = rn / den
+        y += alpha * P
+        r -= alpha * z         # also updates P while it still aliases r
+        rn_new = float(r @ r)
+        beta = rn_new / rn
+        rn = rn_new
+        P = r + beta * P
+        if rn <= tol:
+            break
+    return M * y, grad, JT_J_grad, itn + 1, rn
+
+
+def _matvec_P(factors, P1, P2, v):
+    L = len(factors)
+    R = factors[0].shape[1]
+    dims = [f.shape[0] for f in factors]
+    V = _split(v, dims, R)
+    A = [V[m].T @ factors[m] for m in range(L)]
+    z = []
+    for l in range(L):
+        tmp = np.zeros((R, R))
+        for m in range(L):
+            if m != l:
+                tmp += P2[l, m] * A[m]
+        z.append((factors[l] @ tmp + V[l] @ P1[l]).T.ravel())
+    return np.concatenate(z)
+
+
+def hessian(factors):
+    """Explicit Gauss-Newton matrix J^T J built from an explicit Jacobian (test helper; plays the
+    role of GaussNewton.py:666-695 for tiny sizes)."""
+    J = jacobian(factors)
+    return J.T @ J
+
+
+def jacobian(factors):
+    """Explicit Jacobian of vec_C([[F]]) w.r.t. x = concat_l vec_F(F_l), by brute force."""
+    L = len(factors)
+    R = factors[0].shape[1]
+    dims = [f.shape[0] for f in factors]
+    N = int(np.prod(dims))
+    cols = []
+    for l in range(L):
+        for r in range(R):
+            for i in range(dims[l]):
+                vecs = [f[:, r] for f in factors]
+                e = np.zeros(dims[l])
+                e[i] = 1.0
+                vecs[l] = e
+                t = vecs[0]
+                for m in range(1, L):
+                    t = np.multiply.outer(t, vecs[m])
+                cols.append(t.reshape(N))
+    return np.array(cols).T
+
+
+def update_gain_ratio(damp, old_error, error, Tsize, old_x, x, grad):
+    """Follows GaussNewton.py:472-487."""
+    num = (Tsize ** 2) * (old_error ** 2 - error ** 2)
+    den = float((x - old_x) @ (grad + damp * (x - old_x)))
+    if den != 0.0:
+        return num / den
+    if num != 0.0:
+        return np.sign(num) * np.inf
+    return -np.inf
+
+
+def update_damp(damp, init_damp, gain_ratio, it):
+    """Follows GaussNewton.py:490-503."""
+    if type(init_damp) == list:
+        return init_damp[it]
+    if gain_ratio < 0.25:
+        return 3 * damp
+    if gain_ratio > 0.75:
+        return damp / 3
+    return damp
+
+
+def update_delta(delta, gain_ratio, step_size):
+    """Follows GaussNewton.py:506-517."""
+    if gain_ratio > 0.75:
+        return max(delta, 2 * step_size)
+    sigma = 0.25 / (1 + np.exp(-14 * (gain_ratio - 0.75))) + 0.75
+    return min(sigma * delta, delta)
+
+
+def dogleg(y, grad, JT_J_grad, delta):
+    """Powell dogleg step.  Follows GaussNewton.py:766-795."""
+    alpha = norm(grad) ** 2 / float(grad @ JT_J_grad)
+    y_sd = alpha * grad
+    c1 = norm(y_sd)
+    c2 = float((y - y_sd) @ y_sd)
+    c3 = norm(y - y_sd) ** 2
+    c4 = delta ** 2 - c1 ** 2
+    m = c2 ** 2 + c3 * c4
+    if norm(y) < delta:
+        return y
+    if c1 >= delta:
+        return (delta / c1) * y_sd
+    if m < 0:
+        return y
+    c5 = np.sqrt(m)
+    beta = (-c2 + c5) / c3 if c2 <= 0 else c4 / (c2 + c5)
+    return y_sd + beta * (y - y_sd)
+
+
+def residual_error(T, factors, Tsize):
+    """||T - [[F]]|| / Tsize through the first unfolding.  Follows GaussNewton.py:298-299."""
+    return fastnorm(unfold(T, 1), cpd2unfold1(factors)) / Tsize
+
+
+def compute_dogleg_steps(T, Tsize, factors, grad, JT_J_grad, x, y, error, damp, symm, factors_norm):
+    """Up to 11 dogleg trials after an error jump.  Follows GaussNewton.py:713-763."""
+    count = 0
+    best_x, best_y, best_error = x.copy(), y.copy(), error
+    best_factors = deepcopy(factors)
+    gain_ratio = 1
+    delta = 1
+    while gain_ratio > 0:
+        old_x, old_error = x, error
+        y = dogleg(y, grad, JT_J_grad, delta)
+        x = x + y
+        factors = x2cpd(x, factors, eq=False)
+        factors = transform(factors, symm, factors_norm)
+        error = residual_error(T, factors, Tsize)
+        gain_ratio = update_gain_ratio(damp, old_error, error, Tsize, old_x, x, grad)
+        if error < old_error:
+            best_x, best_y, best_error = x.copy(), y.copy(), error
+            best_factors = deepcopy(factors)
+        delta = update_delta(delta, gain_ratio, norm(x - old_x))
+        count += 1
+        if count > 10:
+            break
+    return best_factors, best_x, best_y, best_error
+
+
+def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
+    """One dGN step.  Follows GaussNewton.py:254-312 for inner methods 'cg' and 'cg_static'."""
+    damp, inner_method, cg_maxiter, cg_factor, cg_tol, tol_jump, symm, factors_norm = inner
+    if type(inner_method) == list:
+        inner_method = inner_method[it]
+    if inner_method == 'cg':
+        cg_maxiter = 1 + (L - 2) * int(cg_factor * np.random.randint(1 + it ** 0.4, 2 + it ** 0.9))
+    elif inner_method != 'cg_static':
+        sys.exit("Wrong inner method name. Must be 'cg', 'cg_static', 'als' or 'direct'.")
+    y, grad, JT_J_grad, itn, rn = cg(T, factors, damp, cg_maxiter, cg_tol)
+    x = x + y
+    factors = x2cpd(x, factors)
+    factors = transform(factors, symm, factors_norm)
+    error = residual_error(T, factors, Tsize)
+    if it > 3 and error > tol_jump * old_error:
+        x = x - y
+        factors, x, y, error = compute_dogleg_steps(T, Tsize, factors, grad, JT_J_grad, x, y, error,
+                                                    damp, symm, factors_norm)
+    return factors, x, y, grad, itn, rn, error
+
+
+def dGN(T, factors, R, options, trace=None):
+    """Damped Gauss-Newton.  Follows GaussNewton.py:29-251 (no fixed mode).  If `trace` is a
+    list, per-iteration (error, damp, gain_ratio, itn) tuples are appended to it."""
+    init_damp = options.init_damp
+    maxiter = options.maxiter
+    tol, tol_step, tol_improv, tol_grad = options.tol, options.tol_step, options.tol_improv, options.tol_grad
+    L = len(factors)
+    dims = T.shape
+    Tsize = norm(T)
+    error = 1
+    best_error = np.inf
+    stop = 5
+    damp = init_damp[0] if type(init_damp) == list else init_damp * np.mean(np.abs(T))
+    const = 1 + int(maxiter / 10)
+    if R * sum(dims) <= 100:
+        tol = tol_step = tol_improv = tol_grad = 0
+    x = np.concatenate([f.flatten('F') for f in factors])
+    step_sizes = np.zeros(maxiter)
+    errors = np.zeros(maxiter)
+    improv = np.zeros(maxiter)
+    gradients = np.zeros(maxiter)
+    best_factors = deepcopy(factors)
+    it = 0
+    for it in range(maxiter):
+        old_x, old_error = x, error
+        inner = (damp, options.inner_method, options.cg_maxiter, options.cg_factor, options.cg_tol,
+                 options.tol_jump, options.symm, options.factors_norm)
+        factors, x, y, grad, itn, rn, error = compute_step(T, Tsize, factors, x, inner, it, old_error, L)
+        gain_ratio = update_gain_ratio(damp, old_error, error, Tsize, old_x, x, grad)
+        if trace is not None:
+            trace.append((error, damp, gain_ratio, itn))
+        damp = update_damp(damp, init_damp, gain_ratio, it)
+        if error < best_error:
+            best_error = error
+            best_factors = deepcopy(factors)
+        errors[it] = error
+        step_sizes[it] = norm(x - old_x) / norm(old_x)
+        gradients[it] = norm(grad, np.inf)
+        improv[it] = errors[it] if it == 0 else abs(errors[it] - errors[it - 1])
+        if it > 1:
+            if errors[it] < tol:
+                stop = 0
+                break
+            if step_sizes[it] < tol_step:
+                stop = 1
+                break
+            if improv[it] < tol_improv:
+                stop = 2
+                break
+            if gradients[it] < tol_grad:
+                stop = 3
+                break
+            if it > 2 * const and it % const == 0:
+                mean1 = np.mean(errors[it - 2 * const: it - const])
+                mean2 = np.mean(errors[it - const: it])
+                if mean1 - mean2 <= tol_improv:
+                    stop = 4
+                    break
+                mean3 = np.mean(improv[it - const: it])
+                if mean3 < 1e-3 * mean2:
+                    stop = 7
+                    break
+            if error > max(1, Tsize ** 2) / (1e-16 + tol):
+                stop = 6
+                break
+    n = it + 1
+    return best_factors, step_sizes[:n], errors[:n], improv[:n], gradients[:n], stop
+
+
+# ----------------------------------------------------------------------------------------------
+# Drivers (TensorFox.py)
+# ----------------------------------------------------------------------------------------------
+
+def tricpd(T, R, options, stages=None):
+    """sort -> mlsvd -> init -> dGN -> U*W -> optional refinement -> final error -> unsort.
+    Follows TensorFox.py:424-655 (dense, method 'dGN').  `stages`, if a dict, receives
+    the intermediate results (core, U, init factors) for tests."""
+    L = T.ndim
+    T, ordering = sort_dims(T)
+    Tsize = norm(T)
+    S, U, T1, sigmas = mlsvd(T, Tsize, R, options)
+    if options.symm:
+        R_min = min(S.shape)
+        S = S[tuple(slice(R_min) for _ in range(L))]
+        U = [u[:, :R_min] for u in U]
+    init = starting_point(S, U, R, ordering, options)
+    if stages is not None:
+        stages.update(S=S, U=U, init=deepcopy(init), sigmas=sigmas, ordering=ordering)
+    factors, ss_m, e_m, im_m, g_m, stop_m = dGN(S, init, R, options)
+    factors = [U[l] @ factors[l] for l in range(L)]
+    if stages is not None:
+        stages.update(main_factors=deepcopy(factors))
+    if options.refine:
+        factors, ss_r, e_r, im_r, g_r, stop_r = dGN(np.asarray(T), factors, R, options)
+    else:
+        ss_r, e_r, im_r, g_r, stop_r = np.array([0]), np.array([0]), np.array([0]), np.array([0]), 8
+    T1_approx = cpd2unfold1(factors)
+    factors = unsort_dims(factors, ordering)
+    if options.refine:
+        num_steps = ss_m.size + ss_r.size
+        rel_error = e_r[-1]
+    else:
+        num_steps = ss_m.size
+        rel_error = fastnorm(T1, T1_approx) / Tsize
+    out = Output(num_steps, rel_error, [ss_m, ss_r], [e_m, e_r], [im_m, im_r], [g_m, g_r],
+                 [stop_m, stop_r], options)
+    return factors, out
+
+
+def cpd(T, R, options=False, stages=None):
+    """Drop-in dense CPD.  Follows TensorFox.py:69-365 (dGN branch, dense input)."""
+    dims = T.shape
+    options = make_options(options)
+    consistency(R, dims, options)
+    if options.method != 'dGN':
+        raise NotImplementedError("the oracle covers method='dGN' only")
+    return tricpd(np.asarray(T, dtype=np.float64), R, options, stages=stages)
+
+
+# ----------------------------------------------------------------------------------------------
+# Generators
+# ----------------------------------------------------------------------------------------------
+
+def gen_rand_tensor(dims, R, noise=0):
+    """Random rank-R tensor with N(0,1) factors and ABSOLUTE noise.  Follows Auxiliar.py:530-572."""
+    F = [np.random.randn(d, R) for d in dims]
+    T = cpd2tens(F)
+    if noise != 0:
+        return T, T + noise * np.random.randn(*dims), F
+    return T, F
+
+
+def cpd_tensor(factors):
+    """Dense [[F]] in C order via einsum (brute force, used to pin cpd2tens)."""
+    L = len(factors)
+    letters = 'abcdefghijkl'[:L]
+    spec = ','.join(ch + 'z' for ch in letters) + '->' + letters
+    return np.einsum(spec, *factors)
