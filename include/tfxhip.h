@@ -1,0 +1,154 @@
+/*
+ * tfxhip.h -- C ABI of libtfxhip.so, the MI355X (gfx950) kernels behind the dense CPD path of
+ * Tensor Fox (reference: modules/TensorFox/TensorFox.py:69 `cpd` -> :424 `tricpd`).
+ *
+ * The reference has no native boundary: its `cpd` is Python calling NumPy/SciPy/Numba
+ * (modules/TensorFox/__init__.py:6 re-exports it).  Each entry point below replaces the CPU
+ * function cited next to it; the Python package TensorFox (tensor-fox_amd/TensorFox) keeps the
+ * reference's `cpd(T, R, options)` signature and binds these symbols with ctypes.
+ *
+ * Conventions
+ *  - every call returns 0 on success and a negative TFX_E* code on failure; the message is
+ *    available from tfx_last_error(ctx); nothing throws across the ABI;
+ *  - all array arguments are device pointers owned by the caller; the context owns only
+ *    workspace, grown on demand with hipMalloc;
+ *  - every call is asynchronous on `stream`; scalar results are written to device memory;
+ *  - one context per host thread / stream;
+ *  - tensors are dense C-order (last index fastest) with dims[0..L-1], 2 <= L <= TFX_MAXL;
+ *  - factor matrices use the reference's flat point layout x = concat_l vec_F(F_l)
+ *    (GaussNewton.py:125, Conversion.py:21-49): F_l[i, r] sits at off_l + r*I_l + i with
+ *    off_l = R * sum_{m<l} I_m.  "flat" below means a buffer of n = R*sum(I_l) scalars in that
+ *    layout;
+ *  - dtype: TFX_F32 or TFX_F64 selects the storage / working type of every floating array of a
+ *    call, unless the argument says double.
+ */
+#ifndef TFXHIP_H
+#define TFXHIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFX_MAXL 8
+
+enum { TFX_F32 = 0, TFX_F64 = 1 };
+
+/* arithmetic used for the tensor-touching products (MTTKRP, Gram, TTM) */
+enum { TFX_PREC_NATIVE = 0,     /* FMA in the storage type */
+       TFX_PREC_SPLIT16 = 1 };  /* FP32 storage split into 16-bit hi+lo MFMA operands, FP32 acc */
+
+enum { TFX_OK = 0, TFX_E_ARG = -1, TFX_E_HIP = -2, TFX_E_NOMEM = -3, TFX_E_UNSUPPORTED = -4 };
+
+typedef struct tfx_ctx tfx_ctx;
+
+/* Context and errors (no reference counterpart: the reference reports errors with sys.exit,
+ * Auxiliar.py:33-71, which the Python layer reproduces). */
+int tfx_ctx_create(int device, tfx_ctx** out);
+int tfx_ctx_destroy(tfx_ctx* ctx);
+const char* tfx_last_error(const tfx_ctx* ctx);
+int tfx_device_info(tfx_ctx* ctx, int* cu_count, int* arch_major, int* arch_minor);
+
+/* MTTKRP for the modes set in mode_mask, in one pass over T.
+ * Replaces compute_grad's Khatri-Rao build + dot(Tl[l], M) (GaussNewton.py:411-423).
+ * out_flat receives N_l = T_(l) * KR_l in the flat layout (modes not in the mask are untouched).
+ * prec: TFX_PREC_NATIVE, or TFX_PREC_SPLIT16 with dtype TFX_F32. */
+int tfx_mttkrp(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
+               const void* F_flat, int R, unsigned mode_mask, void* out_flat, int prec,
+               hipStream_t stream);
+
+/* ||T - [[F]]||_F^2, accumulated in FP64, written to *out_sqnorm (device).
+ * Replaces cpd2unfold1 + fastnorm (Conversion.py:117-141, Critical.py:14-22). */
+int tfx_cpd_residual(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
+                     const void* F_flat, int R, double* out_sqnorm, hipStream_t stream);
+
+/* As tfx_cpd_residual with the factors in f_dtype and the tensor in t_dtype (FP32 tensor with
+ * FP64 factors gives the final FP64 error evaluation of the mixed-precision path). */
+int tfx_cpd_residual_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L, const int64_t* dims,
+                           const void* F_flat, int f_dtype, int R, double* out_sqnorm,
+                           hipStream_t stream);
+
+/* out[0] = sum |x|, out[1] = sum x^2 over n entries (FP64): ||T|| (TensorFox.py:503) and
+ * mean|T| for the initial damping (GaussNewton.py:113). */
+int tfx_tensor_stats(tfx_ctx* ctx, const void* X, int dtype, int64_t n, double* out,
+                     hipStream_t stream);
+
+/* Gr_l = F_l^T F_l, P1_l = o_{k!=l} Gr_k, P2_{l,m} = o_{k!=l,m} Gr_k (L x L x R x R, row-major),
+ * and gamma[l, r] = max|P1_{L-1}| * sqrt|P1_l[r, r]| (L x R).
+ * Replaces gramians + regularization (GaussNewton.py:520-548, 584-606). */
+int tfx_gramians(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, const void* F_flat,
+                 void* Gr, void* P1, void* P2, void* gamma, hipStream_t stream);
+
+/* grad_flat = N_flat - F_l P1_l per mode, i.e. minus the gradient of 1/2||T - [[F]]||^2.
+ * Replaces the tail of compute_grad and the sign flip in cg (GaussNewton.py:423-425, 339). */
+int tfx_neg_grad(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, const void* F_flat,
+                 const void* P1, const void* N_flat, void* grad_flat, hipStream_t stream);
+
+/* Split-preconditioned CG on (J^T J + damp Gamma) y = grad, one persistent workgroup.
+ * Replaces cg + cg_iterations + matvec + precond (GaussNewton.py:315-398, 551-581, 609-627),
+ * including the aliasing of P and residual_cg in the first iteration (GaussNewton.py:349-392).
+ * Writes y_flat = M*y, jtjg_flat = J^T J grad, and scal[0] = iterations done, scal[1] = final
+ * residual norm^2 (double).  Inner products and norms are accumulated in FP64. */
+int tfx_cg(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, const void* F_flat,
+           const void* P1, const void* P2, const void* gamma, double damp, const void* grad_flat,
+           int maxiter, double tol, void* y_flat, void* jtjg_flat, double* scal,
+           hipStream_t stream);
+
+/* x_new = x + alpha*y; F = x2cpd(x_new) with equalize when eq != 0, then transform(symm,
+ * factors_norm).  Replaces x = x + y / x2cpd / equalize / transform (GaussNewton.py:285-289,
+ * Conversion.py:21-49, 361-387, 412-444).  scal (double, 4 entries) receives
+ * [ (alpha y).(g + damp alpha y), ||alpha y||^2, ||x||^2, max|g| ] for the gain ratio, step size
+ * and gradient norm (GaussNewton.py:174, 184-185, 472-487). */
+int tfx_dgn_update(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R,
+                   const void* x_flat, const void* y_flat, double alpha, const void* grad_flat,
+                   double damp, int eq, int symm, double factors_norm, void* x_new_flat,
+                   void* F_flat, double* scal, hipStream_t stream);
+
+/* dots[0..3] = [y.y, y.g, g.g, g.Jg] in FP64 (the inner products of dogleg, GaussNewton.py:766-795). */
+int tfx_dogleg_dots(tfx_ctx* ctx, int dtype, int64_t n, const void* y, const void* g,
+                    const void* jg, double* dots, hipStream_t stream);
+
+/* out = a*y + b*g (the dogleg step y_dl, GaussNewton.py:781-795). */
+int tfx_axpby(tfx_ctx* ctx, int dtype, int64_t n, double a, const void* y, double b,
+              const void* g, void* out, hipStream_t stream);
+
+/* G = X_(mode) X_(mode)^T (I_mode x I_mode, FP64 output).
+ * Replaces the range finder's Gram-type products on unfold(S, l) (Compression.py:147-150,
+ * 389-395); the eigenpairs of G give the left singular pairs of the unfolding. */
+int tfx_gram_mode(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims, int mode,
+                  double* G, int prec, hipStream_t stream);
+
+/* Top-k eigenpairs of a symmetric n x n FP64 matrix by block subspace iteration from the
+ * n x p start block Q0 (column-major), then Rayleigh-Ritz with a Jacobi eigensolver, all FP64.
+ * U (n x k, column-major), lam (k, descending).  Sign: the largest-|.| entry of each column of U
+ * is made positive (svd_flip with the V-based decision, Compression.py:510-512, 559-563).
+ * Replaces randomized_svd's power iterations, QR and small SVD (Compression.py:337-518). */
+int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q0, int p, int k, int iters,
+                double* U, double* lam, hipStream_t stream);
+
+/* Y = X x_mode U^T: Y[.., s, ..] = sum_a U[a, s] X[.., a, ..]; U is I_mode x k column-major
+ * FP64; Y has dims with dims[mode] replaced by k, same dtype as X.  sig2 (k, FP64, optional)
+ * receives the squared norms of the rows of Y_(mode).
+ * Replaces Sl = diag(sigma) Vlt + foldback (Compression.py:153-156) and multilin_mult
+ * (MultilinearAlgebra.py:72-105). */
+int tfx_ttm(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims, int mode,
+            const double* U, int k, void* Y, double* sig2, int prec, hipStream_t stream);
+
+/* C (m x n, column-major, dtype) = op(A) * B with A (FP64, column-major, m x kk or kk x m when
+ * transA) and B (dtype, column-major kk x n).  Used for the uncompression U_l W_l
+ * (TensorFox.py:573-574) and the projection of a user initialization U_l^T F_l
+ * (Initialization.py:58-59). */
+int tfx_gemm_small(tfx_ctx* ctx, int dtype, int transA, int m, int n, int kk, const double* A,
+                   const void* B, void* C, hipStream_t stream);
+
+/* Element-type conversion between flat buffers (FP64 <-> FP32), used when a stage hands its
+ * factors to a stage of another precision. */
+int tfx_convert(tfx_ctx* ctx, int64_t n, const void* src, int src_dtype, void* dst, int dst_dtype,
+                hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,17 @@
+"""TensorFox for MI355X: the dense CPD path of Tensor Fox (felipebottega/Tensor-Fox v1.2.1,
+modules/TensorFox/__init__.py) on hand-written gfx950 HIP kernels.
+
+    import TensorFox as tfx
+    factors, output = tfx.cpd(T, R, options)
+"""
+
+__version__ = '1.2.1+mi355x'
+
+
+def version():
+    return (1, 2, 1)
+
+
+from .auxiliar import (Output, consistency, gen_rand_tensor, make_lowrank, make_options,  # noqa: E402
+                       sort_dims, unsort_dims, cpd2tens)
+from .tensorfox import cpd, tricpd  # noqa: E402

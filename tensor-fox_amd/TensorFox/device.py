@@ -1,0 +1,133 @@
+"""Thin typed wrappers over libtfxhip entry points.
+
+PyTorch-ROCm tensors are used only as device containers and for the current HIP stream; every
+arithmetic kernel is a libtfxhip kernel.  All wrappers are asynchronous on the current stream.
+"""
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+_DT = {torch.float32: _lib.TFX_F32, torch.float64: _lib.TFX_F64}
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def dtype_code(t):
+    try:
+        return _DT[t.dtype]
+    except KeyError:
+        raise TypeError('unsupported dtype %s (float32 or float64 expected)' % t.dtype)
+
+
+class Device:
+    """A GPU and its libtfxhip context."""
+
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError('the TensorFox MI355X path needs a ROCm GPU (torch.cuda.is_available() '
+                               'is False)')
+        self.index = int(device)
+        self.dev = torch.device('cuda', self.index)
+        self.lib = _lib.load()
+        with torch.cuda.device(self.dev):
+            self.ctx = _lib.context(self.index)
+        cu, ma, mi = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._call('tfx_device_info', ctypes.byref(cu), ctypes.byref(ma), ctypes.byref(mi))
+        self.cu_count = cu.value
+
+    # -- plumbing ------------------------------------------------------------------------------
+    def stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(self.ctx, *args)
+        _lib.check(rc, self.ctx)
+
+    def empty(self, shape, dtype):
+        return torch.empty(shape, dtype=dtype, device=self.dev)
+
+    def zeros(self, shape, dtype):
+        return torch.zeros(shape, dtype=dtype, device=self.dev)
+
+    def upload(self, a, dtype):
+        return torch.as_tensor(a).to(device=self.dev, dtype=dtype).contiguous()
+
+    # -- kernels -------------------------------------------------------------------------------
+    def mttkrp(self, T, dims, F, R, out, mask=None, prec=_lib.TFX_PREC_NATIVE):
+        L = len(dims)
+        if mask is None:
+            mask = (1 << L) - 1
+        self._call('tfx_mttkrp', _p(T), dtype_code(T), L, _lib.dims_arg(dims), _p(F), int(R),
+                   ctypes.c_uint(mask), _p(out), int(prec), self.stream())
+
+    def residual_sq(self, T, dims, F, R, out):
+        """out (float64 device scalar) = ||T - [[F]]||^2."""
+        self._call('tfx_cpd_residual_mixed', _p(T), dtype_code(T), len(dims), _lib.dims_arg(dims),
+                   _p(F), dtype_code(F), int(R), _p(out), self.stream())
+
+    def tensor_stats(self, X, out):
+        """out[0] = sum|X|, out[1] = sum X^2 (float64 device)."""
+        self._call('tfx_tensor_stats', _p(X), dtype_code(X), ctypes.c_int64(X.numel()), _p(out),
+                   self.stream())
+
+    def gramians(self, dims, R, F, Gr, P1, P2, gamma):
+        self._call('tfx_gramians', dtype_code(F), len(dims), _lib.dims_arg(dims), int(R), _p(F),
+                   _p(Gr), _p(P1), _p(P2), _p(gamma), self.stream())
+
+    def neg_grad(self, dims, R, F, P1, N, grad):
+        self._call('tfx_neg_grad', dtype_code(F), len(dims), _lib.dims_arg(dims), int(R), _p(F),
+                   _p(P1), _p(N), _p(grad), self.stream())
+
+    def cg(self, dims, R, F, P1, P2, gamma, damp, grad, maxiter, tol, y, jtjg, scal):
+        self._call('tfx_cg', dtype_code(F), len(dims), _lib.dims_arg(dims), int(R), _p(F), _p(P1),
+                   _p(P2), _p(gamma), float(damp), _p(grad), int(maxiter), float(tol), _p(y),
+                   _p(jtjg), _p(scal), self.stream())
+
+    def dgn_update(self, dims, R, x, y, alpha, grad, damp, eq, symm, factors_norm, x_new, F, scal):
+        self._call('tfx_dgn_update', dtype_code(x), len(dims), _lib.dims_arg(dims), int(R), _p(x),
+                   _p(y), float(alpha), _p(grad), float(damp), int(bool(eq)), int(bool(symm)),
+                   float(factors_norm), _p(x_new), _p(F), _p(scal), self.stream())
+
+    def dogleg_dots(self, y, g, jg, out):
+        self._call('tfx_dogleg_dots', dtype_code(y), ctypes.c_int64(y.numel()), _p(y), _p(g), _p(jg),
+                   _p(out), self.stream())
+
+    def axpby(self, a, y, b, g, out):
+        self._call('tfx_axpby', dtype_code(y), ctypes.c_int64(y.numel()), float(a), _p(y), float(b),
+                   _p(g), _p(out), self.stream())
+
+    def gram_mode(self, X, dims, mode, G, prec=_lib.TFX_PREC_NATIVE):
+        self._call('tfx_gram_mode', _p(X), dtype_code(X), len(dims), _lib.dims_arg(dims), int(mode),
+                   _p(G), int(prec), self.stream())
+
+    def eig_top(self, G, n, Q0, p, k, iters, U, lam):
+        self._call('tfx_eig_top', _p(G), int(n), _p(Q0), int(p), int(k), int(iters), _p(U), _p(lam),
+                   self.stream())
+
+    def ttm(self, X, dims, mode, U, k, Y, sig2=None, prec=_lib.TFX_PREC_NATIVE):
+        self._call('tfx_ttm', _p(X), dtype_code(X), len(dims), _lib.dims_arg(dims), int(mode), _p(U),
+                   int(k), _p(Y), _p(sig2), int(prec), self.stream())
+
+    def gemm_small(self, transA, m, n, kk, A, B, C):
+        self._call('tfx_gemm_small', dtype_code(B), int(bool(transA)), int(m), int(n), int(kk), _p(A),
+                   _p(B), _p(C), self.stream())
+
+    def convert(self, src, dst):
+        self._call('tfx_convert', ctypes.c_int64(src.numel()), _p(src), dtype_code(src), _p(dst),
+                   dtype_code(dst), self.stream())
+
+
+_devices = {}
+
+
+def get_device(index=0):
+    d = _devices.get(index)
+    if d is None:
+        d = Device(index)
+        _devices[index] = d
+    return d

@@ -1,0 +1,247 @@
+"""Damped Gauss-Newton on the GPU (GaussNewton.py:29-251 and compute_step :254-312).
+
+Per iteration the device runs: Gram/Hadamard products and regularization, the fused MTTKRP,
+minus the gradient, the persistent preconditioned CG, the point update with equalization, and
+the residual.  The host keeps only the scalar control flow of the reference (CG cap draw from the
+global NumPy RNG, gain ratio, damping, best point, stopping codes), reading one small vector of
+FP64 scalars per iteration.
+"""
+
+import sys
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_S_CG, _S_UPD, _S_ERR = 0, 2, 6     # offsets in the per-iteration scalar vector
+
+
+class Problem:
+    """Device buffers of one dGN run on a tensor with dims `dims`, rank R, working dtype `wdt`."""
+
+    def __init__(self, dev, T, dims, R, wdt, prec):
+        self.dev, self.T, self.dims, self.R, self.wdt, self.prec = dev, T, tuple(dims), R, wdt, prec
+        L = len(dims)
+        n = R * sum(dims)
+        self.n = n
+        e = dev.empty
+        self.Gr = e((L, R, R), wdt)
+        self.P1 = e((L, R, R), wdt)
+        self.P2 = e((L, L, R, R), wdt)
+        self.gamma = e((L, R), wdt)
+        self.N = e(n, wdt)
+        self.grad = e(n, wdt)
+        self.y = e(n, wdt)
+        self.jtjg = e(n, wdt)
+        self.xa = e(n, wdt)
+        self.xb = e(n, wdt)
+        self.F = e(n, wdt)
+        self.best_F = e(n, wdt)
+        self.scal = dev.zeros(16, torch.float64)
+        self.tmp_scal = dev.zeros(16, torch.float64)
+
+    def error_sq(self, F, out):
+        self.dev.residual_sq(self.T, self.dims, F, self.R, out)
+
+
+def _mttkrp_prec(prob):
+    return prob.prec
+
+
+def _gain_ratio(damp, old_error, error, Tsize, gain_den):
+    """update_gain_ratio (GaussNewton.py:472-487) with the denominator computed on the device."""
+    num = (Tsize ** 2) * (old_error ** 2 - error ** 2)
+    if gain_den != 0.0:
+        return num / gain_den
+    if num != 0.0:
+        return np.sign(num) * np.inf
+    return -np.inf
+
+
+def _update_damp(damp, init_damp, gain_ratio, it):
+    """GaussNewton.py:490-503."""
+    if type(init_damp) == list:
+        return init_damp[it]
+    if gain_ratio < 0.25:
+        return 3 * damp
+    if gain_ratio > 0.75:
+        return damp / 3
+    return damp
+
+
+def _update_delta(delta, gain_ratio, step_size):
+    """GaussNewton.py:506-517."""
+    if gain_ratio > 0.75:
+        return max(delta, 2 * step_size)
+    with np.errstate(over='ignore'):
+        sigma = 0.25 / (1 + np.exp(-14 * (gain_ratio - 0.75))) + 0.75
+    return min(sigma * delta, delta)
+
+
+def _dogleg_coeffs(dots, delta):
+    """dogleg (GaussNewton.py:766-795) expressed as y_dl = a*y + b*grad from the FP64 inner
+    products [y.y, y.g, g.g, g.Jg]."""
+    yy, yg, gg, gjg = dots
+    alpha = gg / gjg
+    c1 = abs(alpha) * np.sqrt(gg)
+    c2 = alpha * yg - alpha * alpha * gg
+    c3 = yy - 2 * alpha * yg + alpha * alpha * gg
+    c4 = delta ** 2 - c1 ** 2
+    m = c2 ** 2 + c3 * c4
+    if np.sqrt(yy) < delta:
+        return 1.0, 0.0
+    if c1 >= delta:
+        return 0.0, (delta / c1) * alpha
+    if m < 0:
+        return 1.0, 0.0
+    c5 = np.sqrt(m)
+    beta = (-c2 + c5) / c3 if c2 <= 0 else c4 / (c2 + c5)
+    return beta, alpha * (1 - beta)
+
+
+def _dogleg_steps(prob, x, y, error, Tsize, damp, symm, factors_norm):
+    """compute_dogleg_steps (GaussNewton.py:713-763) with device vectors.  `x` already had y
+    removed.  Returns (best_F, best_x, best_y, best_error) as device tensors / float."""
+    dev = prob.dev
+    best_x, best_y, best_error = x.clone(), y.clone(), error
+    best_F = prob.F.clone()
+    gain_ratio, delta, count = 1.0, 1.0, 0
+    sc = prob.tmp_scal
+    while gain_ratio > 0:
+        old_error = error
+        dev.dogleg_dots(y, prob.grad, prob.jtjg, sc[0:4])
+        a, b = _dogleg_coeffs(sc[0:4].cpu().numpy(), delta)
+        y_new = torch.empty_like(y)
+        dev.axpby(a, y, b, prob.grad, y_new)
+        x_new = torch.empty_like(x)
+        dev.dgn_update(prob.dims, prob.R, x, y_new, 1.0, prob.grad, damp, False, symm, factors_norm,
+                       x_new, prob.F, sc[4:8])
+        prob.error_sq(prob.F, sc[8:9])
+        h = sc[4:9].cpu().numpy()
+        error = float(np.sqrt(h[4])) / Tsize
+        gain_ratio = _gain_ratio(damp, old_error, error, Tsize, float(h[0]))
+        if error < old_error:
+            best_x, best_y, best_error = x_new.clone(), y_new.clone(), error
+            best_F = prob.F.clone()
+        delta = _update_delta(delta, gain_ratio, float(np.sqrt(h[1])))
+        x, y = x_new, y_new
+        count += 1
+        if count > 10:
+            break
+    return best_F, best_x, best_y, best_error
+
+
+def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
+    """Damped Gauss-Newton from the flat factors F0 (device, working dtype).  Returns
+    (best flat factors, step_sizes, errors, improv, gradients, stop) like GaussNewton.py:29-251."""
+    dev = prob.dev
+    dims, R = prob.dims, prob.R
+    L = len(dims)
+    init_damp = options.init_damp
+    maxiter = options.maxiter
+    tol, tol_step, tol_improv, tol_grad = options.tol, options.tol_step, options.tol_improv, options.tol_grad
+    tol_jump = options.tol_jump
+    symm, factors_norm = options.symm, options.factors_norm
+    error = 1
+    best_error = np.inf
+    stop = 5
+    damp = init_damp[0] if type(init_damp) == list else init_damp * mean_abs
+    const = 1 + int(maxiter / 10)
+    if R * sum(dims) <= 100:
+        tol = tol_step = tol_improv = tol_grad = 0
+    x = prob.xa
+    x.copy_(F0)
+    prob.F.copy_(F0)
+    prob.best_F.copy_(F0)
+    x_spare = prob.xb
+    step_sizes = np.zeros(maxiter)
+    errors = np.zeros(maxiter)
+    improv = np.zeros(maxiter)
+    gradients = np.zeros(maxiter)
+    it = 0
+    for it in range(maxiter):
+        old_error = error
+        inner = options.inner_method[it] if type(options.inner_method) == list else options.inner_method
+        if inner == 'cg':
+            cap = 1 + (L - 2) * int(options.cg_factor * np.random.randint(1 + it ** 0.4, 2 + it ** 0.9))
+        elif inner == 'cg_static':
+            cap = options.cg_maxiter
+        elif inner in ('als', 'direct'):
+            raise NotImplementedError("inner_method '%s' is not on the MI355X path; use 'cg' or "
+                                      "'cg_static'" % inner)
+        else:
+            sys.exit("Wrong inner method name. Must be 'cg', 'cg_static', 'als' or 'direct'.")
+        # --- compute_step: cg ------------------------------------------------------------------
+        dev.gramians(dims, R, prob.F, prob.Gr, prob.P1, prob.P2, prob.gamma)
+        dev.mttkrp(prob.T, dims, prob.F, R, prob.N, prec=_mttkrp_prec(prob))
+        dev.neg_grad(dims, R, prob.F, prob.P1, prob.N, prob.grad)
+        dev.cg(dims, R, prob.F, prob.P1, prob.P2, prob.gamma, damp, prob.grad, cap, options.cg_tol,
+               prob.y, prob.jtjg, prob.scal[_S_CG:_S_CG + 2])
+        # --- x = x + y; factors = transform(equalize(x2cpd(x))); error ----------------------------
+        dev.dgn_update(dims, R, x, prob.y, 1.0, prob.grad, damp, True, symm, factors_norm, x_spare,
+                       prob.F, prob.scal[_S_UPD:_S_UPD + 4])
+        prob.error_sq(prob.F, prob.scal[_S_ERR:_S_ERR + 1])
+        h = prob.scal[:8].cpu().numpy()
+        old_x = x
+        x, x_spare = x_spare, x
+        error = float(np.sqrt(h[_S_ERR])) / Tsize
+        gain_den = float(h[_S_UPD])
+        dx2 = float(h[_S_UPD + 1])
+        x2 = float(h[_S_UPD + 2])
+        gmax = float(h[_S_UPD + 3])
+        if it > 3 and error > tol_jump * old_error:
+            x_back = torch.empty_like(x)
+            dev.axpby(1.0, x, -1.0, prob.y, x_back)
+            bF, bx, by, error = _dogleg_steps(prob, x_back, prob.y.clone(), error, Tsize, damp, symm,
+                                              factors_norm)
+            prob.F.copy_(bF)
+            x.copy_(bx)
+            prob.y.copy_(by)
+            # x - old_x statistics for the gain ratio / step size of this iteration
+            d = torch.empty_like(x)
+            dev.axpby(1.0, x, -1.0, old_x, d)
+            scratch_x, scratch_F = torch.empty_like(x), torch.empty_like(x)
+            dev.dgn_update(dims, R, old_x, d, 1.0, prob.grad, damp, False, False, 0.0, scratch_x,
+                           scratch_F, prob.tmp_scal[0:4])
+            hs = prob.tmp_scal[0:4].cpu().numpy()
+            gain_den, dx2 = float(hs[0]), float(hs[1])
+        gain_ratio = _gain_ratio(damp, old_error, error, Tsize, gain_den)
+        if trace is not None:
+            trace.append((error, damp, gain_ratio, int(h[_S_CG])))
+        damp = _update_damp(damp, init_damp, gain_ratio, it)
+        if error < best_error:
+            best_error = error
+            prob.best_F.copy_(prob.F)
+        errors[it] = error
+        step_sizes[it] = np.sqrt(dx2) / np.sqrt(x2)
+        gradients[it] = gmax
+        improv[it] = errors[it] if it == 0 else abs(errors[it] - errors[it - 1])
+        if it > 1:
+            if errors[it] < tol:
+                stop = 0
+                break
+            if step_sizes[it] < tol_step:
+                stop = 1
+                break
+            if improv[it] < tol_improv:
+                stop = 2
+                break
+            if gradients[it] < tol_grad:
+                stop = 3
+                break
+            if it > 2 * const and it % const == 0:
+                mean1 = np.mean(errors[it - 2 * const: it - const])
+                mean2 = np.mean(errors[it - const: it])
+                if mean1 - mean2 <= tol_improv:
+                    stop = 4
+                    break
+                mean3 = np.mean(improv[it - const: it])
+                if mean3 < 1e-3 * mean2:
+                    stop = 7
+                    break
+            if error > max(1, Tsize ** 2) / (1e-16 + tol):
+                stop = 6
+                break
+    k = it + 1
+    return prob.best_F.clone(), step_sizes[:k], errors[:k], improv[:k], gradients[:k], stop

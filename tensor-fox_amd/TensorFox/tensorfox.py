@@ -1,0 +1,249 @@
+"""Drop-in `cpd(T, R, options)` for dense tensors on one MI355X (TensorFox.py:69-365 -> tricpd
+:424-655): sort dims -> MLSVD -> initialization -> dGN on the core -> uncompression -> optional
+refinement on the full tensor -> final error -> original mode order.
+
+Precision forms (option `precision`):
+  'fp64'  -- every array and product in FP64 (the default for float64 inputs);
+  'mixed' -- tensor and factors stored FP32, products FP32 on the device, eigensolve, CG inner
+             products / norms and the final relative error in FP64 (the default for float32
+             inputs).
+"""
+
+import sys
+import time
+from copy import deepcopy
+
+import numpy as np
+import torch
+
+from . import _lib
+from .auxiliar import Output, consistency, make_options, sort_dims, unsort_dims
+from .compression import mlsvd
+from .device import get_device
+from .gauss_newton import Problem, dGN
+
+_CLEAN_RNG = np.random.RandomState(98765)
+
+
+def _equalize(F, R):
+    """Conversion.py:361-387 on host factors."""
+    L = len(F)
+    for r in range(R):
+        nr = np.array([np.linalg.norm(F[l][:, r]) for l in range(L)])
+        if np.prod(nr) != 0.0:
+            num = np.prod(nr) ** (1 / L)
+            for l in range(L):
+                F[l][:, r] = (num / nr[l]) * F[l][:, r]
+    return F
+
+
+def _transform(F, symm, factors_norm):
+    """Conversion.py:412-444 on host factors."""
+    L = len(F)
+    if symm:
+        s = F[0]
+        for l in range(1, L):
+            s = s + F[l]
+        F[0] = s / L
+        for l in range(1, L):
+            F[l] = F[0]
+    if factors_norm > 0:
+        for l in range(L):
+            F[l] = factors_norm * (1 / np.linalg.norm(F[l])) * F[l]
+    return F
+
+
+def _clean_zeros(F, R):
+    """Initialization.py:361-386; the reference draws these under Numba's own RNG, so a private
+    generator is used to leave the global NumPy stream untouched."""
+    for f in F:
+        idx = np.argwhere(f == 0.0)
+        for i, r in idx:
+            f[i, r] = 1e-4 * _CLEAN_RNG.randn() * np.linalg.norm(f)
+    return F
+
+
+def _flat(F):
+    return np.concatenate([f.flatten('F') for f in F])
+
+
+def _unflat(x, dims, R):
+    out, s = [], 0
+    for d in dims:
+        out.append(np.array(x[s:s + R * d].reshape(R, d).T))
+        s += R * d
+    return out
+
+
+def _precision_of(T, options):
+    if options.precision is not None:
+        if options.precision not in ('mixed', 'fp64'):
+            raise ValueError("options.precision must be 'mixed' or 'fp64'")
+        return options.precision
+    dt = T.dtype
+    is32 = (dt == np.float32) if isinstance(T, np.ndarray) else (dt == torch.float32)
+    return 'mixed' if is32 else 'fp64'
+
+
+def _to_device(dev, T, ordering, tdt):
+    """Sorted, contiguous copy of T on the device in the storage dtype."""
+    L = T.ndim
+    identity = all(int(ordering[l]) == l for l in range(L))
+    if isinstance(T, torch.Tensor):
+        X = T if T.is_cuda else T.to(dev.dev)
+        if not identity:
+            X = X.permute(*[int(o) for o in ordering])
+        return X.to(tdt).contiguous().view(-1)
+    A = np.asarray(T)
+    if not identity:
+        A = np.moveaxis(A, ordering, np.arange(L))
+    A = np.ascontiguousarray(A, dtype=np.float32 if tdt == torch.float32 else np.float64)
+    return torch.from_numpy(A).to(dev.dev).view(-1)
+
+
+def _sync():
+    torch.cuda.synchronize()
+
+
+def tricpd(T, R, options):
+    """Dense dGN path of TensorFox.py:424-655 on the GPU."""
+    t_all = time.perf_counter()
+    timings = {}
+    dev = get_device(options.device)
+    precision = _precision_of(T, options)
+    wdt = torch.float32 if precision == 'mixed' else torch.float64
+    prec = _lib.TFX_PREC_NATIVE
+    L = T.ndim
+    dims, ordering = sort_dims(T.shape)
+    Tdev = _to_device(dev, T, ordering, wdt)
+    N = int(np.prod(dims))
+    stats = dev.zeros(2, torch.float64)
+    dev.tensor_stats(Tdev, stats)
+    st = stats.cpu().numpy()
+    Tsize = float(np.sqrt(st[1]))
+    mean_abs = float(st[0]) / N
+    display = options.display
+    t0 = time.perf_counter()
+    timings['upload'] = t0 - t_all
+
+    # compression
+    S, S_dims, U, sigmas = mlsvd(dev, Tdev, dims, R, options, wdt, prec)
+    if options.symm:
+        R_min = min(S_dims)
+        view = S.view(*S_dims)
+        for l in range(L):
+            view = view.narrow(l, 0, R_min)
+        S = view.contiguous().view(-1)
+        S_dims = [R_min] * L
+        U = [u[:R_min].contiguous() for u in U]
+    _sync()
+    t1 = time.perf_counter()
+    timings['mlsvd'] = t1 - t0
+    if display > 0:
+        print('-----------------------------------------------------------------------------------------------')
+        print('Computing MLSVD')
+        if tuple(S_dims) == tuple(dims):
+            print('    No compression detected')
+            print('    Working with dimensions', tuple(dims))
+        else:
+            print('    Compression detected')
+            print('    Compressing from', tuple(dims), 'to', tuple(S_dims))
+
+    # initialization (Initialization.py:20-94)
+    init = options.initialization
+    if type(init) == list:
+        F0 = dev.empty(R * sum(S_dims), wdt)
+        off = 0
+        for l in range(L):
+            B = dev.upload(np.asarray(init[ordering[l]], dtype=np.float64).flatten('F'), wdt)
+            dev.gemm_small(True, S_dims[l], R, dims[l], U[l], B, F0[off:off + R * S_dims[l]])
+            off += R * S_dims[l]
+    elif init == 'random':
+        F = [np.random.randn(S_dims[l], R) for l in range(L)]
+        F = _clean_zeros(F, R)
+        F = _equalize(F, R)
+        F = _transform(F, options.symm, options.factors_norm)
+        F0 = dev.upload(_flat(F), wdt)
+    elif init in ('smart', 'smart_random'):
+        raise NotImplementedError("initialization '%s' is not on the MI355X path" % init)
+    else:
+        sys.exit('Error with init parameter initialization.')
+    Sstats = dev.zeros(2, torch.float64)
+    dev.tensor_stats(S, Sstats)
+    sst = Sstats.cpu().numpy()
+    Ssize = float(np.sqrt(sst[1]))
+    S_mean_abs = float(sst[0]) / int(np.prod(S_dims))
+    t2 = time.perf_counter()
+    timings['init'] = t2 - t1
+
+    # main dGN on the core
+    if options.method != 'dGN':
+        raise NotImplementedError("method '%s' is not on the MI355X path" % options.method)
+    prob = Problem(dev, S, S_dims, R, wdt, prec)
+    Fm, ss_m, e_m, im_m, g_m, stop_m = dGN(prob, F0, options, Ssize, S_mean_abs)
+    _sync()
+    t3 = time.perf_counter()
+    timings['main'] = t3 - t2
+
+    # uncompression (TensorFox.py:573-574)
+    Ffull = dev.empty(R * sum(dims), wdt)
+    off_s, off_f = 0, 0
+    for l in range(L):
+        dev.gemm_small(False, dims[l], R, S_dims[l], U[l], Fm[off_s:off_s + R * S_dims[l]],
+                       Ffull[off_f:off_f + R * dims[l]])
+        off_s += R * S_dims[l]
+        off_f += R * dims[l]
+    _sync()
+    t4 = time.perf_counter()
+    timings['uncompress'] = t4 - t3
+
+    # refinement (TensorFox.py:583-611)
+    if options.refine:
+        if display > 0:
+            print('Computing refinement of solution')
+        probT = Problem(dev, Tdev, dims, R, wdt, prec)
+        Ffull, ss_r, e_r, im_r, g_r, stop_r = dGN(probT, Ffull, options, Tsize, mean_abs)
+    else:
+        ss_r, e_r, im_r, g_r, stop_r = np.array([0]), np.array([0]), np.array([0]), np.array([0]), 8
+    _sync()
+    t5 = time.perf_counter()
+    timings['refine'] = t5 - t4
+
+    # final error in FP64 and host factors
+    F64 = dev.empty(R * sum(dims), torch.float64)
+    dev.convert(Ffull, F64)
+    err2 = dev.zeros(1, torch.float64)
+    dev.residual_sq(Tdev, dims, F64, R, err2)
+    final_err = float(np.sqrt(err2.cpu().numpy()[0])) / Tsize
+    factors = _unflat(F64.cpu().numpy(), dims, R)
+    factors = unsort_dims(factors, ordering)
+    t6 = time.perf_counter()
+    timings['final'] = t6 - t5
+    timings['total'] = t6 - t_all
+    if options.refine:
+        num_steps = ss_m.size + ss_r.size
+        rel_error = float(e_r[-1])
+    else:
+        num_steps = ss_m.size
+        rel_error = final_err
+    out = Output(num_steps, rel_error, [ss_m, ss_r], [e_m, e_r], [im_m, im_r], [g_m, g_r],
+                 [stop_m, stop_r], options, timings=timings, precision=precision)
+    out.final_rel_error = final_err
+    out.compressed_dims = tuple(S_dims)
+    if display > 0:
+        print('Final results')
+        print('    Number of steps =', out.num_steps)
+        print('    Relative error =', out.rel_error)
+    return factors, out
+
+
+def cpd(T, R, options=False):
+    """Canonical polyadic decomposition of a dense tensor, same call and outputs as
+    TensorFox.cpd (TensorFox.py:69-365).  T: NumPy array or torch tensor (CPU or GPU), float32
+    or float64; R: int; options: any object whose attributes override the defaults."""
+    dims_orig = tuple(T.shape)
+    options = make_options(options)
+    consistency(R, dims_orig, options)
+    if options.method == 'dGN':
+        return tricpd(T, R, options)
+    raise NotImplementedError("method '%s' is not on the MI355X path (dense 'dGN' only)" % options.method)

@@ -1,0 +1,533 @@
+// Device side of one damped Gauss-Newton iteration (GaussNewton.py:254-627): Gram matrices and
+// their Hadamard products, regularization, minus the gradient, the split-preconditioned CG with
+// the Gram-Hadamard approximate-Hessian matvec, the point update with equalization, and the
+// small vector kernels of the dogleg fallback.
+//
+// Flat layout (x, grad, y, factors): element (i, r) of mode l at off_l + r*I_l + i.
+#include "common.h"
+
+namespace {
+
+__device__ inline int mode_of(const Shape& s, int64_t e, int64_t* within) {
+  int l = 0;
+  while (l + 1 < s.L && e >= s.off[l + 1]) ++l;
+  *within = e - s.off[l];
+  return l;
+}
+
+// Gr[l][r][c] = sum_i F_l[i, r] F_l[i, c].  One workgroup per (l, r).
+template <typename S>
+__global__ __launch_bounds__(256) void k_gram_factors(Shape s, const S* __restrict__ F,
+                                                       S* __restrict__ Gr) {
+  __shared__ S red[4][64];
+  const int l = blockIdx.x / s.R, r = blockIdx.x % s.R;
+  const int64_t I = s.d[l];
+  const S* Fl = F + s.off[l];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < s.R; c0 += 64) {
+    const int c = c0 + lane;
+    S acc = S(0);
+    if (c < s.R)
+      for (int64_t i = grp; i < I; i += 4) acc += Fl[(int64_t)r * I + i] * Fl[(int64_t)c * I + i];
+    __syncthreads();
+    red[grp][lane] = acc;
+    __syncthreads();
+    if (grp == 0 && c < s.R)
+      Gr[((int64_t)l * s.R + r) * s.R + c] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  }
+}
+
+// P1[l] = o_{k != l} Gr_k, P2[l][m] = o_{k != l, m} Gr_k (ones on l == m), and
+// gamma[l, r] = max|P1_{L-1}| * sqrt|P1_l[r, r]| (GaussNewton.py:584-606 scales with the LAST
+// mode's max).  One workgroup.
+template <typename S>
+__global__ __launch_bounds__(1024) void k_gram_hadamard(Shape s, const S* __restrict__ Gr,
+                                                         S* __restrict__ P1, S* __restrict__ P2,
+                                                         S* __restrict__ gamma) {
+  __shared__ S red[16];
+  const int L = s.L, R = s.R;
+  const int64_t RR = (int64_t)R * R;
+  for (int64_t e = threadIdx.x; e < (int64_t)L * RR; e += blockDim.x) {
+    const int l = (int)(e / RR);
+    const int64_t rc = e % RR;
+    S p = S(1);
+    for (int k = 0; k < L; ++k)
+      if (k != l) p *= Gr[k * RR + rc];
+    P1[e] = p;
+    for (int m = 0; m < L; ++m) {
+      S q = S(1);
+      if (m != l)
+        for (int k = 0; k < L; ++k)
+          if (k != l && k != m) q *= Gr[k * RR + rc];
+      P2[((int64_t)l * L + m) * RR + rc] = q;
+    }
+  }
+  __syncthreads();
+  S mx = S(0);
+  for (int64_t e = threadIdx.x; e < RR; e += blockDim.x) {
+    const S v = fabs(P1[(L - 1) * RR + e]);
+    mx = v > mx ? v : mx;
+  }
+  mx = block_max(mx, red);
+  for (int e = threadIdx.x; e < L * R; e += blockDim.x) {
+    const int l = e / R, r = e % R;
+    gamma[e] = mx * sqrt(fabs(P1[l * RR + (int64_t)r * R + r]));
+  }
+}
+
+// grad = N - F_l P1_l (minus the gradient), flat layout.
+template <typename S>
+__global__ void k_neg_grad(Shape s, const S* __restrict__ F, const S* __restrict__ P1,
+                           const S* __restrict__ N, S* __restrict__ g) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= s.n) return;
+  int64_t w;
+  const int l = mode_of(s, e, &w);
+  const int64_t I = s.d[l];
+  const int r = (int)(w / I);
+  const int64_t i = w % I;
+  const S* Fl = F + s.off[l];
+  const S* P = P1 + (int64_t)l * s.R * s.R;
+  S acc = S(0);
+  for (int c = 0; c < s.R; ++c) acc += Fl[(int64_t)c * I + i] * P[(int64_t)c * s.R + r];
+  g[e] = N[e] - acc;
+}
+
+// (J^T J) v for the flat vector v (GaussNewton.py:342-346, 551-581):
+//   A_m[c][r] = sum_i v_m[i, c] F_m[i, r]
+//   tmp_l     = sum_{m != l} P2[l][m] o A_m
+//   out_l[i, r] = sum_c F_l[i, c] tmp_l[c][r] + sum_c v_l[i, c] P1_l[c][r]
+// Runs inside one workgroup; the caller synchronizes before reading `out`.
+template <typename S>
+__device__ void wg_matvec(const Shape& s, const S* __restrict__ F, const S* __restrict__ P1,
+                          const S* __restrict__ P2, const S* v, S* A, S* tmp, S* out) {
+  const int L = s.L, R = s.R;
+  const int64_t RR = (int64_t)R * R;
+  const int nt = blockDim.x;
+  for (int64_t e = threadIdx.x; e < (int64_t)L * RR; e += nt) {
+    const int m = (int)(e / RR);
+    const int c = (int)((e % RR) / R), r = (int)(e % R);
+    const int64_t I = s.d[m];
+    const S* vm = v + s.off[m] + (int64_t)c * I;
+    const S* Fm = F + s.off[m] + (int64_t)r * I;
+    S acc = S(0);
+    for (int64_t i = 0; i < I; ++i) acc += vm[i] * Fm[i];
+    A[e] = acc;
+  }
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < (int64_t)L * RR; e += nt) {
+    const int l = (int)(e / RR);
+    const int64_t rc = e % RR;
+    S acc = S(0);
+    for (int m = 0; m < L; ++m)
+      if (m != l) acc += P2[((int64_t)l * L + m) * RR + rc] * A[m * RR + rc];
+    tmp[e] = acc;
+  }
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < s.n; e += nt) {
+    int64_t w;
+    const int l = mode_of(s, e, &w);
+    const int64_t I = s.d[l];
+    const int r = (int)(w / I);
+    const int64_t i = w % I;
+    const S* Fl = F + s.off[l];
+    const S* vl = v + s.off[l];
+    const S* tl = tmp + l * RR;
+    const S* pl = P1 + l * RR;
+    S acc = S(0);
+    for (int c = 0; c < R; ++c) acc += Fl[(int64_t)c * I + i] * tl[(int64_t)c * R + r] + vl[(int64_t)c * I + i] * pl[(int64_t)c * R + r];
+    out[e] = acc;
+  }
+}
+
+template <typename S>
+__device__ inline S gamma_of(const Shape& s, const S* gamma, int64_t e) {
+  int64_t w;
+  const int l = mode_of(s, e, &w);
+  return gamma[l * s.R + (int)(w / s.d[l])];
+}
+
+// Persistent split-preconditioned CG in one workgroup (GaussNewton.py:315-398).
+template <typename S>
+__global__ __launch_bounds__(1024) void k_cg(Shape s, const S* __restrict__ F,
+                                              const S* __restrict__ P1, const S* __restrict__ P2,
+                                              const S* __restrict__ gamma, double damp,
+                                              const S* __restrict__ grad, int maxiter, double tol,
+                                              S* __restrict__ y, S* __restrict__ jtjg,
+                                              double* __restrict__ scal, S* ws, int tmp_in_lds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[16];
+  const int64_t n = s.n;
+  const int64_t LRR = (int64_t)s.L * s.R * s.R;
+  S* rv = ws;            // residual
+  S* pv = rv + n;        // search direction
+  S* qv = pv + n;        // M p
+  S* zv = qv + n;        // M (J^T J + damp Gamma) M p
+  S* yc = zv + n;        // CG iterate
+  S* A = yc + n;
+  S* tmp = tmp_in_lds ? (S*)smem : A + LRR;
+  const S dampS = (S)damp;
+  const int t = threadIdx.x, nt = blockDim.x;
+
+  if (n > 100) {
+    if ((int64_t)maxiter > n) maxiter = (int)n;
+  } else {
+    tol = 0.0;
+  }
+  // J^T J grad, kept for the dogleg fallback
+  wg_matvec<S>(s, F, P1, P2, grad, A, tmp, jtjg);
+  __syncthreads();
+  double acc = 0.0;
+  for (int64_t e = t; e < n; e += nt) {
+    const S g = gamma_of(s, gamma, e);
+    const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
+    const S rr = M * grad[e];
+    rv[e] = rr;
+    pv[e] = rr;
+    yc[e] = S(0);
+    acc += (double)rr * (double)rr;
+  }
+  double rn = block_sum(acc, red);
+  if (rn == 0.0) rn = 1e-6;
+  int itn = 0;
+  for (itn = 0; itn < maxiter; ++itn) {
+    for (int64_t e = t; e < n; e += nt) {
+      const S g = gamma_of(s, gamma, e);
+      const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
+      qv[e] = M * pv[e];
+    }
+    __syncthreads();
+    wg_matvec<S>(s, F, P1, P2, qv, A, tmp, zv);
+    // z = M (z + damp Gamma q); each thread finishes its own elements
+    acc = 0.0;
+    for (int64_t e = t; e < n; e += nt) {
+      const S g = gamma_of(s, gamma, e);
+      const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
+      const S z = M * (zv[e] + dampS * g * qv[e]);
+      zv[e] = z;
+      acc += (double)pv[e] * (double)z;
+    }
+    double den = block_sum(acc, red);
+    if (den == 0.0) den = 1e-6;
+    const double alpha = rn / den;
+    const S aS = (S)alpha;
+    acc = 0.0;
+    for (int64_t e = t; e < n; e += nt) {
+      yc[e] += aS * pv[e];
+      const S r = rv[e] - aS * zv[e];
+      rv[e] = r;
+      acc += (double)r * (double)r;
+    }
+    const double rn_new = block_sum(acc, red);
+    const double beta = rn_new / rn;
+    rn = rn_new;
+    const S bS = (S)beta;
+    for (int64_t e = t; e < n; e += nt) {
+      // first iteration: P aliases the residual array in the reference, so P_old == r_new
+      const S pold = itn == 0 ? rv[e] : pv[e];
+      pv[e] = rv[e] + bS * pold;
+    }
+    if (rn <= tol) break;
+  }
+  if (itn == maxiter) itn = maxiter - 1;  // loop variable of the reference after a full run
+  for (int64_t e = t; e < n; e += nt) {
+    const S g = gamma_of(s, gamma, e);
+    const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
+    y[e] = M * yc[e];
+  }
+  if (t == 0) {
+    scal[0] = (double)(itn + 1);
+    scal[1] = rn;
+  }
+}
+
+// x_new = x + alpha y; F = x2cpd(x_new) (+ equalize) then transform; gain-ratio scalars.
+template <typename S>
+__global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restrict__ x,
+                                                      const S* __restrict__ y, double alpha,
+                                                      const S* __restrict__ g, double damp, int eq,
+                                                      int symm, double factors_norm,
+                                                      S* __restrict__ xn, S* __restrict__ F,
+                                                      double* __restrict__ scal) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* cn = (double*)smem;   // L x R column norms
+  __shared__ double red[16];
+  __shared__ double fnorm[TFX_MAXL];
+  const int64_t n = s.n;
+  const int L = s.L, R = s.R;
+  const int t = threadIdx.x, nt = blockDim.x;
+  double gain = 0.0, dy2 = 0.0, x2 = 0.0, gmax = 0.0;
+  for (int64_t e = t; e < n; e += nt) {
+    const double xo = (double)x[e];
+    const S dyS = (S)alpha * y[e];
+    const S v = x[e] + dyS;
+    xn[e] = v;
+    F[e] = v;
+    const double dy = (double)(v - x[e]);   // x_new - x_old as the reference forms it
+    gain += dy * ((double)g[e] + damp * dy);
+    dy2 += dy * dy;
+    x2 += xo * xo;
+    const double ag = fabs((double)g[e]);
+    gmax = ag > gmax ? ag : gmax;
+  }
+  gain = block_sum(gain, red);
+  dy2 = block_sum(dy2, red);
+  x2 = block_sum(x2, red);
+  gmax = block_max(gmax, red);
+  if (t == 0) {
+    scal[0] = gain;
+    scal[1] = dy2;
+    scal[2] = x2;
+    scal[3] = gmax;
+  }
+  __syncthreads();
+  if (eq) {
+    for (int e = t; e < L * R; e += nt) {
+      const int l = e / R, r = e % R;
+      const int64_t I = s.d[l];
+      const S* col = F + s.off[l] + (int64_t)r * I;
+      double a = 0.0;
+      for (int64_t i = 0; i < I; ++i) a += (double)col[i] * (double)col[i];
+      cn[e] = sqrt(a);
+    }
+    __syncthreads();
+    for (int64_t e = t; e < n; e += nt) {
+      int64_t w;
+      const int l = mode_of(s, e, &w);
+      const int r = (int)(w / s.d[l]);
+      double prod = 1.0;
+      for (int k = 0; k < L; ++k) prod *= cn[k * R + r];
+      if (prod != 0.0) F[e] = (S)(pow(prod, 1.0 / L) / cn[l * R + r] * (double)F[e]);
+    }
+    __syncthreads();
+  }
+  if (symm) {
+    const int64_t I = s.d[0];
+    for (int64_t e = t; e < (int64_t)R * I; e += nt) {
+      S acc = F[e];
+      for (int l = 1; l < L; ++l) acc += F[s.off[l] + e];
+      acc = acc / (S)L;
+      for (int l = 0; l < L; ++l) F[s.off[l] + e] = acc;
+    }
+    __syncthreads();
+  }
+  if (factors_norm > 0) {
+    for (int l = 0; l < L; ++l) {
+      const int64_t len = (int64_t)R * s.d[l];
+      double a = 0.0;
+      for (int64_t e = t; e < len; e += nt) a += (double)F[s.off[l] + e] * (double)F[s.off[l] + e];
+      a = block_sum(a, red);
+      if (t == 0) fnorm[l] = sqrt(a);
+      __syncthreads();
+    }
+    for (int64_t e = t; e < n; e += nt) {
+      int64_t w;
+      const int l = mode_of(s, e, &w);
+      F[e] = (S)(factors_norm * (1.0 / fnorm[l]) * (double)F[e]);
+    }
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(1024) void k_dogleg_dots(int64_t n, const S* __restrict__ y,
+                                                       const S* __restrict__ g,
+                                                       const S* __restrict__ jg,
+                                                       double* __restrict__ dots) {
+  __shared__ double red[16];
+  double a = 0, b = 0, c = 0, d = 0;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const double ye = y[e], ge = g[e], je = jg[e];
+    a += ye * ye;
+    b += ye * ge;
+    c += ge * ge;
+    d += ge * je;
+  }
+  a = block_sum(a, red);
+  b = block_sum(b, red);
+  c = block_sum(c, red);
+  d = block_sum(d, red);
+  if (threadIdx.x == 0) {
+    dots[0] = a;
+    dots[1] = b;
+    dots[2] = c;
+    dots[3] = d;
+  }
+}
+
+template <typename S>
+__global__ void k_axpby(int64_t n, double a, const S* __restrict__ y, double b,
+                        const S* __restrict__ g, S* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = (S)(a * (double)y[e] + b * (double)g[e]);
+}
+
+template <typename A, typename B>
+__global__ void k_convert(int64_t n, const A* __restrict__ src, B* __restrict__ dst) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) dst[e] = (B)src[e];
+}
+
+template <typename S>
+int gramians_t(tfx_ctx* ctx, const Shape& s, const S* F, S* Gr, S* P1, S* P2, S* gamma,
+               hipStream_t st) {
+  hipLaunchKernelGGL(k_gram_factors<S>, dim3(s.L * s.R), dim3(256), 0, st, s, F, Gr);
+  TFX_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_gram_hadamard<S>, dim3(1), dim3(1024), 0, st, s, Gr, P1, P2, gamma);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+template <typename S>
+int cg_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, const S* gamma,
+         double damp, const S* grad, int maxiter, double tol, S* y, S* jtjg, double* scal,
+         hipStream_t st) {
+  const int64_t LRR = (int64_t)s.L * s.R * s.R;
+  void* wsp;
+  int rc = tfx_workspace(ctx, sizeof(S) * (5 * s.n + 2 * LRR) + 256, st, &wsp);
+  if (rc) return rc;
+  const size_t tmp_bytes = sizeof(S) * LRR;
+  const int tmp_in_lds = tmp_bytes <= 120 * 1024;
+  const size_t lds = tmp_in_lds ? tmp_bytes : 0;
+  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(lds > 0 ? lds : 1)));
+  hipLaunchKernelGGL(k_cg<S>, dim3(1), dim3(1024), lds, st, s, F, P1, P2, gamma, damp, grad,
+                     maxiter, tol, y, jtjg, scal, (S*)wsp, tmp_in_lds);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+}  // namespace
+
+#define TFX_SHAPE_OR_RETURN(s)                       \
+  Shape s;                                           \
+  {                                                  \
+    int rc_ = tfx_make_shape(ctx, L, dims, R, &s);   \
+    if (rc_) return rc_;                             \
+  }
+
+extern "C" int tfx_gramians(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R,
+                            const void* F_flat, void* Gr, void* P1, void* P2, void* gamma,
+                            hipStream_t stream) {
+  if (!ctx || !F_flat || !Gr || !P1 || !P2 || !gamma) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  TFX_SHAPE_OR_RETURN(s);
+  if (dtype == TFX_F32)
+    return gramians_t<float>(ctx, s, (const float*)F_flat, (float*)Gr, (float*)P1, (float*)P2, (float*)gamma, stream);
+  if (dtype == TFX_F64)
+    return gramians_t<double>(ctx, s, (const double*)F_flat, (double*)Gr, (double*)P1, (double*)P2, (double*)gamma, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+}
+
+extern "C" int tfx_neg_grad(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R,
+                            const void* F_flat, const void* P1, const void* N_flat,
+                            void* grad_flat, hipStream_t stream) {
+  if (!ctx || !F_flat || !P1 || !N_flat || !grad_flat) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  TFX_SHAPE_OR_RETURN(s);
+  const unsigned nb = (unsigned)((s.n + 255) / 256);
+  if (dtype == TFX_F32)
+    hipLaunchKernelGGL(k_neg_grad<float>, dim3(nb), dim3(256), 0, stream, s, (const float*)F_flat,
+                       (const float*)P1, (const float*)N_flat, (float*)grad_flat);
+  else if (dtype == TFX_F64)
+    hipLaunchKernelGGL(k_neg_grad<double>, dim3(nb), dim3(256), 0, stream, s, (const double*)F_flat,
+                       (const double*)P1, (const double*)N_flat, (double*)grad_flat);
+  else
+    return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+extern "C" int tfx_cg(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R,
+                      const void* F_flat, const void* P1, const void* P2, const void* gamma,
+                      double damp, const void* grad_flat, int maxiter, double tol, void* y_flat,
+                      void* jtjg_flat, double* scal, hipStream_t stream) {
+  if (!ctx || !F_flat || !P1 || !P2 || !gamma || !grad_flat || !y_flat || !jtjg_flat || !scal)
+    return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  if (maxiter < 1) return tfx_set_error(ctx, TFX_E_ARG, "maxiter %d < 1", maxiter);
+  TFX_SHAPE_OR_RETURN(s);
+  if (dtype == TFX_F32)
+    return cg_t<float>(ctx, s, (const float*)F_flat, (const float*)P1, (const float*)P2,
+                       (const float*)gamma, damp, (const float*)grad_flat, maxiter, tol,
+                       (float*)y_flat, (float*)jtjg_flat, scal, stream);
+  if (dtype == TFX_F64)
+    return cg_t<double>(ctx, s, (const double*)F_flat, (const double*)P1, (const double*)P2,
+                        (const double*)gamma, damp, (const double*)grad_flat, maxiter, tol,
+                        (double*)y_flat, (double*)jtjg_flat, scal, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+}
+
+extern "C" int tfx_dgn_update(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R,
+                              const void* x_flat, const void* y_flat, double alpha,
+                              const void* grad_flat, double damp, int eq, int symm,
+                              double factors_norm, void* x_new_flat, void* F_flat, double* scal,
+                              hipStream_t stream) {
+  if (!ctx || !x_flat || !y_flat || !grad_flat || !x_new_flat || !F_flat || !scal)
+    return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  TFX_SHAPE_OR_RETURN(s);
+  if (symm)
+    for (int l = 1; l < L; ++l)
+      if (dims[l] != dims[0]) return tfx_set_error(ctx, TFX_E_ARG, "symm needs equal dims");
+  const size_t lds = sizeof(double) * (size_t)L * R;
+  if (lds > 60 * 1024) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "L*R too large");
+  if (dtype == TFX_F32)
+    hipLaunchKernelGGL(k_dgn_update<float>, dim3(1), dim3(1024), lds, stream, s,
+                       (const float*)x_flat, (const float*)y_flat, alpha, (const float*)grad_flat,
+                       damp, eq, symm, factors_norm, (float*)x_new_flat, (float*)F_flat, scal);
+  else if (dtype == TFX_F64)
+    hipLaunchKernelGGL(k_dgn_update<double>, dim3(1), dim3(1024), lds, stream, s,
+                       (const double*)x_flat, (const double*)y_flat, alpha,
+                       (const double*)grad_flat, damp, eq, symm, factors_norm,
+                       (double*)x_new_flat, (double*)F_flat, scal);
+  else
+    return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+extern "C" int tfx_dogleg_dots(tfx_ctx* ctx, int dtype, int64_t n, const void* y, const void* g,
+                               const void* jg, double* dots, hipStream_t stream) {
+  if (!ctx || !y || !g || !jg || !dots) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  if (dtype == TFX_F32)
+    hipLaunchKernelGGL(k_dogleg_dots<float>, dim3(1), dim3(1024), 0, stream, n, (const float*)y,
+                       (const float*)g, (const float*)jg, dots);
+  else if (dtype == TFX_F64)
+    hipLaunchKernelGGL(k_dogleg_dots<double>, dim3(1), dim3(1024), 0, stream, n, (const double*)y,
+                       (const double*)g, (const double*)jg, dots);
+  else
+    return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+extern "C" int tfx_axpby(tfx_ctx* ctx, int dtype, int64_t n, double a, const void* y, double b,
+                         const void* g, void* out, hipStream_t stream) {
+  if (!ctx || !y || !g || !out) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (n == 0) return TFX_OK;
+  if (dtype == TFX_F32)
+    hipLaunchKernelGGL(k_axpby<float>, dim3(nb), dim3(256), 0, stream, n, a, (const float*)y, b,
+                       (const float*)g, (float*)out);
+  else if (dtype == TFX_F64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(nb), dim3(256), 0, stream, n, a, (const double*)y, b,
+                       (const double*)g, (double*)out);
+  else
+    return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+extern "C" int tfx_convert(tfx_ctx* ctx, int64_t n, const void* src, int src_dtype, void* dst,
+                           int dst_dtype, hipStream_t stream) {
+  if (!ctx || !src || !dst) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  if (n == 0) return TFX_OK;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (src_dtype == TFX_F64 && dst_dtype == TFX_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, stream, n, (const double*)src, (float*)dst);
+  else if (src_dtype == TFX_F32 && dst_dtype == TFX_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, stream, n, (const float*)src, (double*)dst);
+  else if (src_dtype == dst_dtype) {
+    TFX_HIP(ctx, hipMemcpyAsync(dst, src, n * (src_dtype == TFX_F64 ? 8 : 4), hipMemcpyDeviceToDevice, stream));
+    return TFX_OK;
+  } else
+    return tfx_set_error(ctx, TFX_E_ARG, "bad dtypes %d -> %d", src_dtype, dst_dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}

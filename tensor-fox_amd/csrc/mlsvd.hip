@@ -1,0 +1,476 @@
+// MLSVD building blocks (Compression.py:48-237): Gram matrix of a mode unfolding, top-k FP64
+// eigenpairs by subspace iteration + Rayleigh-Ritz (parallel cyclic Jacobi), the mode product
+// Y = X x_mode U^T with the squared row norms of its unfolding, and a small GEMM for the
+// uncompression U_l W_l.
+//
+// A tensor X with dims d[0..L-1] is viewed around `mode` as (P, I, Q):
+//   P = prod_{m < mode} d[m], I = d[mode], Q = prod_{m > mode} d[m],
+// element (p, a, q) at (p*I + a)*Q + q.  The unfolding X_(mode) has rows a and columns (p, q).
+#include "common.h"
+
+namespace {
+
+struct PIQ {
+  int64_t P, I, Q;
+};
+
+PIQ piq_of(const Shape& s, int mode) {
+  PIQ v{1, s.d[mode], 1};
+  for (int m = 0; m < mode; ++m) v.P *= s.d[m];
+  for (int m = mode + 1; m < s.L; ++m) v.Q *= s.d[m];
+  return v;
+}
+
+constexpr int GT = 32;   // Gram output tile
+constexpr int GK = 32;   // reduction chunk
+
+// G[a][b] += sum over a chunk of (p, q) of X[p, a, q] X[p, b, q]; one workgroup per
+// (tile pair with tb >= ta, k-split); FP64 atomics into G (zeroed by the caller).
+template <typename S>
+__global__ __launch_bounds__(256) void k_gram_mode(const S* __restrict__ X, PIQ v, int ntile,
+                                                    int64_t kchunk, double* __restrict__ G) {
+  __shared__ S Xa[GK][GT + 1];
+  __shared__ S Xb[GK][GT + 1];
+  // decode (ta, tb) from the triangular pair index
+  int pair = blockIdx.x;
+  int ta = 0;
+  while (pair >= ntile - ta) {
+    pair -= ntile - ta;
+    ++ta;
+  }
+  const int tb = ta + pair;
+  const int64_t KT = v.P * v.Q;
+  const int64_t kb = (int64_t)blockIdx.y * kchunk;
+  const int64_t ke = min(KT, kb + kchunk);
+  const int t = threadIdx.x;
+  const int ox = t % 16, oy = t / 16;   // each thread: outputs (2 ox + {0,1}, 2 oy + {0,1})
+  double acc[2][2] = {{0, 0}, {0, 0}};
+  S sacc[2][2] = {{0, 0}, {0, 0}};
+  for (int64_t k0 = kb; k0 < ke; k0 += GK) {
+    __syncthreads();
+    for (int e = t; e < GK * GT; e += 256) {
+      int kk, a;
+      if (v.Q >= GK) {        // k contiguous: consecutive threads walk k
+        kk = e % GK;
+        a = e / GK;
+      } else {                // a contiguous when Q == 1
+        a = e % GT;
+        kk = e / GT;
+      }
+      const int64_t k = k0 + kk;
+      S va = S(0), vb = S(0);
+      if (k < ke) {
+        const int64_t p = k / v.Q, q = k % v.Q;
+        const int64_t ra = (int64_t)ta * GT + a, rb = (int64_t)tb * GT + a;
+        if (ra < v.I) va = X[(p * v.I + ra) * v.Q + q];
+        if (rb < v.I) vb = X[(p * v.I + rb) * v.Q + q];
+      }
+      Xa[kk][a] = va;
+      Xb[kk][a] = vb;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < GK; ++kk) {
+      const S a0 = Xa[kk][2 * oy], a1 = Xa[kk][2 * oy + 1];
+      const S b0 = Xb[kk][2 * ox], b1 = Xb[kk][2 * ox + 1];
+      sacc[0][0] += a0 * b0;
+      sacc[0][1] += a0 * b1;
+      sacc[1][0] += a1 * b0;
+      sacc[1][1] += a1 * b1;
+    }
+    // fold into FP64 every chunk so that long reductions keep FP32 partials short
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] += (double)sacc[i][j];
+        sacc[i][j] = S(0);
+      }
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      const int64_t ra = (int64_t)ta * GT + 2 * oy + i, rb = (int64_t)tb * GT + 2 * ox + j;
+      if (ra < v.I && rb < v.I) {
+        atomicAdd(&G[ra * v.I + rb], acc[i][j]);
+        if (ta != tb) atomicAdd(&G[rb * v.I + ra], acc[i][j]);
+      }
+    }
+}
+
+// Y[p, s, q] = sum_a U[a + s*I] X[p, a, q]
+template <typename S>
+__global__ void k_ttm(const S* __restrict__ X, PIQ v, const double* __restrict__ U, int k,
+                      S* __restrict__ Y) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tot = v.P * k * v.Q;
+  if (e >= tot) return;
+  const int64_t q = e % v.Q;
+  const int64_t sidx = (e / v.Q) % k;
+  const int64_t p = e / (v.Q * k);
+  const double* u = U + sidx * v.I;
+  const S* x = X + p * v.I * v.Q + q;
+  double acc = 0.0;
+  S sacc = S(0);
+  for (int64_t a = 0; a < v.I; ++a) {
+    sacc += (S)u[a] * x[a * v.Q];
+    if ((a & 31) == 31) {
+      acc += (double)sacc;
+      sacc = S(0);
+    }
+  }
+  acc += (double)sacc;
+  Y[e] = (S)acc;
+}
+
+// sig2[s] = sum_{p, q} Y[p, s, q]^2 (FP64)
+template <typename S>
+__global__ __launch_bounds__(256) void k_rownorm2(const S* __restrict__ Y, PIQ v,
+                                                   double* __restrict__ sig2) {
+  __shared__ double red[4];
+  const int64_t sidx = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < v.P * v.Q; e += blockDim.x) {
+    const int64_t p = e / v.Q, q = e % v.Q;
+    const double y = (double)Y[(p * v.I + sidx) * v.Q + q];
+    acc += y * y;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) sig2[sidx] = acc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 eigensolver, one workgroup.
+// ---------------------------------------------------------------------------------------------
+
+// C (n x p, col-major) = G (n x n, row-major symmetric) * B (n x p, col-major)
+__device__ void wg_symm_mul(const double* G, int n, const double* B, int p, double* C) {
+  for (int e = threadIdx.x; e < n * p; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    const double* g = G + (int64_t)i * n;
+    const double* b = B + (int64_t)j * n;
+    double acc = 0.0;
+    for (int k = 0; k < n; ++k) acc += g[k] * b[k];
+    C[e] = acc;
+  }
+  __syncthreads();
+}
+
+// Modified Gram-Schmidt (twice) on the p columns of Z (n x p col-major), in place.  Columns
+// that vanish are replaced by a unit vector orthogonal to the previous ones when possible.
+__device__ void wg_orthonormalize(double* Z, int n, int p, double* red) {
+  for (int j = 0; j < p; ++j) {
+    double* zj = Z + (int64_t)j * n;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int i = 0; i < j; ++i) {
+        const double* zi = Z + (int64_t)i * n;
+        double a = 0.0;
+        for (int e = threadIdx.x; e < n; e += blockDim.x) a += zi[e] * zj[e];
+        a = block_sum(a, red);
+        for (int e = threadIdx.x; e < n; e += blockDim.x) zj[e] -= a * zi[e];
+        __syncthreads();
+      }
+    }
+    double a = 0.0;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) a += zj[e] * zj[e];
+    a = block_sum(a, red);
+    const double nr = sqrt(a);
+    const double inv = nr > 0.0 ? 1.0 / nr : 0.0;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) zj[e] *= inv;
+    __syncthreads();
+  }
+}
+
+// Parallel cyclic Jacobi (round-robin pairing) on the symmetric m x m matrix H (row-major, in
+// place); V (m x m, col-major) receives the eigenvectors.  Eigenvalues end on the diagonal.
+__device__ void wg_jacobi(double* H, int m, double* V, double* cs, double* red) {
+  const int mm = (m + 1) & ~1;   // even number of players; index m is a dummy when m is odd
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) V[e] = (e % m) == (e / m) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+      const double h = H[e];
+      if ((e % m) == (e / m)) dia += h * h;
+      else off += h * h;
+    }
+    off = block_sum(off, red);
+    dia = block_sum(dia, red);
+    if (off <= 1e-30 * dia || off == 0.0) break;
+    for (int round = 0; round < mm - 1; ++round) {
+      // pairs of this round: (round, mm-1) and ((round+k) % (mm-1), (round-k+mm-1) % (mm-1))
+      for (int k = threadIdx.x; k < mm / 2; k += blockDim.x) {
+        int a, b;
+        if (k == 0) {
+          a = round;
+          b = mm - 1;
+        } else {
+          a = (round + k) % (mm - 1);
+          b = (round - k + mm - 1) % (mm - 1);
+        }
+        if (a > b) {
+          const int tmp = a;
+          a = b;
+          b = tmp;
+        }
+        double c = 1.0, s = 0.0;
+        if (b < m) {
+          const double hab = H[a * m + b];
+          if (hab != 0.0) {
+            const double tau = (H[b * m + b] - H[a * m + a]) / (2.0 * hab);
+            const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + tt * tt);
+            s = tt * c;
+          }
+        }
+        cs[4 * k + 0] = c;
+        cs[4 * k + 1] = s;
+        cs[4 * k + 2] = a;
+        cs[4 * k + 3] = b;
+      }
+      __syncthreads();
+      // rows: H <- J^T H
+      for (int e = threadIdx.x; e < (mm / 2) * m; e += blockDim.x) {
+        const int k = e / m, col = e % m;
+        const int a = (int)cs[4 * k + 2], b = (int)cs[4 * k + 3];
+        if (b >= m) continue;
+        const double c = cs[4 * k], s = cs[4 * k + 1];
+        const double ha = H[a * m + col], hb = H[b * m + col];
+        H[a * m + col] = c * ha - s * hb;
+        H[b * m + col] = s * ha + c * hb;
+      }
+      __syncthreads();
+      // columns: H <- H J, V <- V J
+      for (int e = threadIdx.x; e < (mm / 2) * m; e += blockDim.x) {
+        const int k = e / m, row = e % m;
+        const int a = (int)cs[4 * k + 2], b = (int)cs[4 * k + 3];
+        if (b >= m) continue;
+        const double c = cs[4 * k], s = cs[4 * k + 1];
+        const double ha = H[row * m + a], hb = H[row * m + b];
+        H[row * m + a] = c * ha - s * hb;
+        H[row * m + b] = s * ha + c * hb;
+        const double va = V[a * m + row], vb = V[b * m + row];
+        V[a * m + row] = c * va - s * vb;
+        V[b * m + row] = s * va + c * vb;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ws layout (doubles): Q n*p | Z n*p | H p*p | V p*p | cs 4*p | order p
+__global__ __launch_bounds__(1024) void k_eig_top(const double* __restrict__ G, int n,
+                                                   const double* __restrict__ Q0, int p, int k,
+                                                   int iters, double* __restrict__ U,
+                                                   double* __restrict__ lam, double* ws) {
+  __shared__ double red[16];
+  const bool direct = n <= p;
+  const int m = direct ? n : p;
+  // Q and Z exist only for the subspace iteration; the direct solve starts at H
+  const int64_t qz = direct ? 0 : (int64_t)n * p;
+  double* Q = ws;
+  double* Z = Q + qz;
+  double* H = Z + qz;
+  double* V = H + (int64_t)m * m;
+  double* cs = V + (int64_t)m * m;
+  int* order = (int*)(cs + 4 * (m + 2));
+  if (direct) {
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) H[e] = G[e];
+    __syncthreads();
+  } else {
+    for (int e = threadIdx.x; e < n * p; e += blockDim.x) Q[e] = Q0[e];
+    __syncthreads();
+    wg_orthonormalize(Q, n, p, red);
+    for (int it = 0; it < iters; ++it) {
+      wg_symm_mul(G, n, Q, p, Z);
+      for (int e = threadIdx.x; e < n * p; e += blockDim.x) Q[e] = Z[e];
+      __syncthreads();
+      wg_orthonormalize(Q, n, p, red);
+    }
+    wg_symm_mul(G, n, Q, p, Z);
+    // H = Q^T Z (symmetrized)
+    for (int e = threadIdx.x; e < p * p; e += blockDim.x) {
+      const int i = e / p, j = e % p;
+      double acc = 0.0;
+      for (int r = 0; r < n; ++r) acc += Q[(int64_t)i * n + r] * Z[(int64_t)j * n + r];
+      H[e] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < p * p; e += blockDim.x) {
+      const int i = e / p, j = e % p;
+      if (i < j) {
+        const double a = 0.5 * (H[i * p + j] + H[j * p + i]);
+        H[i * p + j] = a;
+        H[j * p + i] = a;
+      }
+    }
+    __syncthreads();
+  }
+  wg_jacobi(H, m, V, cs, red);
+  // order by descending eigenvalue (selection by rank; ties broken by index)
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double di = H[i * m + i];
+    int rank = 0;
+    for (int j = 0; j < m; ++j) {
+      const double dj = H[j * m + j];
+      if (dj > di || (dj == di && j < i)) ++rank;
+    }
+    order[rank] = i;
+  }
+  __syncthreads();
+  // U[:, c] = (direct ? V : Q V)[:, order[c]] with the largest-|.| entry made positive
+  for (int c = 0; c < k; ++c) {
+    const int src = order[c];
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      double u;
+      if (direct) {
+        u = V[(int64_t)src * m + r];
+      } else {
+        u = 0.0;
+        for (int j = 0; j < p; ++j) u += Q[(int64_t)j * n + r] * V[(int64_t)src * m + j];
+      }
+      U[(int64_t)c * n + r] = u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int best = 0;
+      double bv = -1.0;
+      for (int r = 0; r < n; ++r) {
+        const double a = fabs(U[(int64_t)c * n + r]);
+        if (a > bv) {
+          bv = a;
+          best = r;
+        }
+      }
+      red[0] = U[(int64_t)c * n + best] < 0 ? -1.0 : 1.0;
+      lam[c] = H[src * m + src];
+    }
+    __syncthreads();
+    const double sg = red[0];
+    for (int r = threadIdx.x; r < n; r += blockDim.x) U[(int64_t)c * n + r] *= sg;
+    __syncthreads();
+  }
+}
+
+// C (m x n col-major) = op(A) B; A FP64 col-major (m x kk, or kk x m when transA), B col-major
+template <typename S>
+__global__ void k_gemm_small(int transA, int m, int n, int kk, const double* __restrict__ A,
+                             const S* __restrict__ B, S* __restrict__ C) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  double acc = 0.0;
+  for (int q = 0; q < kk; ++q) {
+    const double a = transA ? A[(int64_t)i * kk + q] : A[(int64_t)q * m + i];
+    acc += a * (double)B[(int64_t)j * kk + q];
+  }
+  C[e] = (S)acc;
+}
+
+template <typename S>
+int gram_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, double* G, hipStream_t st) {
+  const PIQ v = piq_of(s, mode);
+  TFX_HIP(ctx, hipMemsetAsync(G, 0, sizeof(double) * v.I * v.I, st));
+  const int ntile = (int)((v.I + GT - 1) / GT);
+  const int npairs = ntile * (ntile + 1) / 2;
+  const int64_t KT = v.P * v.Q;
+  // enough workgroups to fill the chip: split the reduction
+  int64_t want = (int64_t)ctx->cu_count * 8;
+  int64_t ksplit = (want + npairs - 1) / npairs;
+  int64_t kchunk = (KT + ksplit - 1) / ksplit;
+  kchunk = ((kchunk + GK - 1) / GK) * GK;
+  if (kchunk < GK) kchunk = GK;
+  ksplit = (KT + kchunk - 1) / kchunk;
+  hipLaunchKernelGGL(k_gram_mode<S>, dim3(npairs, (unsigned)ksplit), dim3(256), 0, st, X, v,
+                     ntile, kchunk, G);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+template <typename S>
+int ttm_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, const double* U, int k, S* Y,
+          double* sig2, hipStream_t st) {
+  const PIQ v = piq_of(s, mode);
+  const int64_t tot = v.P * k * v.Q;
+  hipLaunchKernelGGL(k_ttm<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, X, v, U, k, Y);
+  TFX_CHECK_LAUNCH(ctx);
+  if (sig2) {
+    const PIQ vy{v.P, k, v.Q};
+    hipLaunchKernelGGL(k_rownorm2<S>, dim3(k), dim3(256), 0, st, Y, vy, sig2);
+    TFX_CHECK_LAUNCH(ctx);
+  }
+  return TFX_OK;
+}
+
+}  // namespace
+
+int tfx_gram_split16(tfx_ctx* ctx, const float* X, const Shape& s, int mode, double* G,
+                     hipStream_t st);
+int tfx_ttm_split16(tfx_ctx* ctx, const float* X, const Shape& s, int mode, const double* U,
+                    int k, float* Y, double* sig2, hipStream_t st);
+
+extern "C" int tfx_gram_mode(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims,
+                             int mode, double* G, int prec, hipStream_t stream) {
+  if (!ctx || !X || !G) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  Shape s;
+  int rc = tfx_make_shape(ctx, L, dims, 1, &s);
+  if (rc) return rc;
+  if (mode < 0 || mode >= L) return tfx_set_error(ctx, TFX_E_ARG, "mode %d out of range", mode);
+  if (prec == TFX_PREC_SPLIT16) {
+    if (dtype != TFX_F32) return tfx_set_error(ctx, TFX_E_ARG, "split16 Gram needs FP32 data");
+    return tfx_gram_split16(ctx, (const float*)X, s, mode, G, stream);
+  }
+  if (dtype == TFX_F32) return gram_t<float>(ctx, (const float*)X, s, mode, G, stream);
+  if (dtype == TFX_F64) return gram_t<double>(ctx, (const double*)X, s, mode, G, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+}
+
+extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q0, int p, int k,
+                           int iters, double* U, double* lam, hipStream_t stream) {
+  if (!ctx || !G || !U || !lam) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  if (n < 1 || k < 1 || k > n) return tfx_set_error(ctx, TFX_E_ARG, "bad sizes n=%d k=%d", n, k);
+  const bool direct = n <= p;
+  if (!direct && (!Q0 || p < k)) return tfx_set_error(ctx, TFX_E_ARG, "start block needs p >= k columns");
+  if (n > 4096 || p > 512) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "eigensolver limited to n <= 4096, p <= 512");
+  const int m = direct ? n : p;
+  const size_t dbl = (size_t)2 * n * (direct ? 0 : p) + 2 * (size_t)m * m + 4 * (m + 2) + m + 8;
+  void* wsp;
+  int rc = tfx_workspace(ctx, sizeof(double) * dbl + 256, stream, &wsp);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_eig_top, dim3(1), dim3(1024), 0, stream, G, n, Q0, p, k, iters, U, lam,
+                     (double*)wsp);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+extern "C" int tfx_ttm(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims,
+                       int mode, const double* U, int k, void* Y, double* sig2, int prec,
+                       hipStream_t stream) {
+  if (!ctx || !X || !U || !Y) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  Shape s;
+  int rc = tfx_make_shape(ctx, L, dims, 1, &s);
+  if (rc) return rc;
+  if (mode < 0 || mode >= L || k < 1) return tfx_set_error(ctx, TFX_E_ARG, "bad mode/k");
+  if (prec == TFX_PREC_SPLIT16) {
+    if (dtype != TFX_F32) return tfx_set_error(ctx, TFX_E_ARG, "split16 TTM needs FP32 data");
+    return tfx_ttm_split16(ctx, (const float*)X, s, mode, U, k, (float*)Y, sig2, stream);
+  }
+  if (dtype == TFX_F32) return ttm_t<float>(ctx, (const float*)X, s, mode, U, k, (float*)Y, sig2, stream);
+  if (dtype == TFX_F64) return ttm_t<double>(ctx, (const double*)X, s, mode, U, k, (double*)Y, sig2, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+}
+
+extern "C" int tfx_gemm_small(tfx_ctx* ctx, int dtype, int transA, int m, int n, int kk,
+                              const double* A, const void* B, void* C, hipStream_t stream) {
+  if (!ctx || !A || !B || !C) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  if (m < 1 || n < 1 || kk < 1) return tfx_set_error(ctx, TFX_E_ARG, "bad sizes");
+  const int64_t tot = (int64_t)m * n;
+  const unsigned nb = (unsigned)((tot + 255) / 256);
+  if (dtype == TFX_F32)
+    hipLaunchKernelGGL(k_gemm_small<float>, dim3(nb), dim3(256), 0, stream, transA, m, n, kk, A,
+                       (const float*)B, (float*)C);
+  else if (dtype == TFX_F64)
+    hipLaunchKernelGGL(k_gemm_small<double>, dim3(nb), dim3(256), 0, stream, transA, m, n, kk, A,
+                       (const double*)B, (double*)C);
+  else
+    return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}

@@ -1,0 +1,459 @@
+// MTTKRP of every mode in one pass over T, and the CPD residual ||T - [[F]]||^2.
+//
+// T (C order) is read as NR = N / K rows of length K = I_{L-1} (the contiguous last mode).
+// For row rho = (i_0, ..., i_{L-2}):
+//   W[rho, r] = prod_{m < L-1} F_m[i_m, r]              (Khatri-Rao row of the other modes)
+//   Y[rho, r] = sum_k T[rho, k] F_{L-1}[k, r]
+//   N_{L-1}[k, r]  += T[rho, k] W[rho, r]
+//   N_m[i_m, r]    += Y[rho, r] prod_{m' != m, m' < L-1} F_{m'}[i_{m'}, r]     (m < L-1)
+// which together are T_(l) KR_l for every l (compute_grad, GaussNewton.py:411-423), with the
+// Khatri-Rao products never stored.  This file holds the FMA path (FP32 or FP64 arithmetic);
+// the MFMA split-precision path is in mttkrp_mfma.hip.
+#include "common.h"
+
+namespace {
+
+constexpr int RB = 32;   // rows per block step
+constexpr int KC = 32;   // k per chunk
+
+template <typename S> struct RCOf;
+template <> struct RCOf<float> { static constexpr int v = 64; };
+template <> struct RCOf<double> { static constexpr int v = 32; };
+
+// Row strides of the row index over modes 0..L-2 (last of them fastest).
+__device__ inline void row_to_idx(const Shape& s, int64_t rho, int64_t* idx) {
+  for (int m = s.L - 2; m >= 0; --m) {
+    idx[m] = rho % s.d[m];
+    rho /= s.d[m];
+  }
+}
+
+// Pass 1: Y (NR x R, row-major, columns rc0..rc0+RC-1) and per-workgroup partials of N_{L-1}.
+template <typename S>
+__global__ __launch_bounds__(256) void k_mttkrp_rows(const S* __restrict__ T, Shape s,
+                                                      const S* __restrict__ F, int rc0,
+                                                      int64_t rows_per_wg, S* __restrict__ Y,
+                                                      S* __restrict__ part, int np_in_lds) {
+  constexpr int RC = RCOf<S>::v;
+  constexpr int JN = RC / 8;
+  extern __shared__ __align__(16) unsigned char smem[];
+  S* Tt = (S*)smem;                 // RB x (KC+1)
+  S* Fl = Tt + RB * (KC + 1);       // KC x RC
+  S* W = Fl + KC * RC;              // RB x RC
+  S* Np_lds = W + RB * RC;          // K x RC (when it fits)
+
+  const int t = threadIdx.x;
+  const int L = s.L, R = s.R;
+  const int64_t K = s.d[L - 1];
+  const int64_t NR = s.N / K;
+  const int rce = min(RC, R - rc0);
+  const int64_t w = blockIdx.x;
+  S* Np = np_in_lds ? Np_lds : part + w * K * RC;
+
+  for (int64_t e = t; e < K * RC; e += 256) Np[e] = S(0);
+
+  const int64_t rho_begin = w * rows_per_wg;
+  const int64_t rho_end = min(NR, rho_begin + rows_per_wg);
+  const S* Flast = F + s.off[L - 1];
+
+  for (int64_t rho0 = rho_begin; rho0 < rho_end; rho0 += RB) {
+    const int nrows = (int)min((int64_t)RB, rho_end - rho0);
+    __syncthreads();
+    // Khatri-Rao rows of the leading modes
+    for (int e = t; e < RB * RC; e += 256) {
+      const int rr = e / RC, r = e % RC;
+      S v = S(0);
+      if (rr < nrows && r < rce) {
+        int64_t idx[TFX_MAXL];
+        row_to_idx(s, rho0 + rr, idx);
+        v = S(1);
+        for (int m = 0; m < L - 1; ++m) v *= F[s.off[m] + (int64_t)(rc0 + r) * s.d[m] + idx[m]];
+      }
+      W[rr * RC + r] = v;
+    }
+    S yacc[JN];
+#pragma unroll
+    for (int j = 0; j < JN; ++j) yacc[j] = S(0);
+    const int yr = t >> 3, rl = t & 7;
+
+    for (int64_t k0 = 0; k0 < K; k0 += KC) {
+      __syncthreads();
+      for (int e = t; e < RB * KC; e += 256) {
+        const int rr = e / KC, kk = e % KC;
+        S v = S(0);
+        if (rr < nrows && k0 + kk < K) v = T[(rho0 + rr) * K + k0 + kk];
+        Tt[rr * (KC + 1) + kk] = v;
+      }
+      for (int e = t; e < KC * RC; e += 256) {
+        const int r = e / KC, kk = e % KC;
+        S v = S(0);
+        if (r < rce && k0 + kk < K) v = Flast[(int64_t)(rc0 + r) * K + k0 + kk];
+        Fl[kk * RC + r] = v;
+      }
+      __syncthreads();
+      // Y[rho, r] += sum_kk T[rho, kk] F[kk, r]
+#pragma unroll 4
+      for (int kk = 0; kk < KC; ++kk) {
+        const S tv = Tt[yr * (KC + 1) + kk];
+#pragma unroll
+        for (int j = 0; j < JN; ++j) yacc[j] += tv * Fl[kk * RC + rl + 8 * j];
+      }
+      // N_last[k, r] += sum_rho T[rho, k] W[rho, r]
+      const int kk = t >> 3;
+      if (k0 + kk < K) {
+        S nacc[JN];
+#pragma unroll
+        for (int j = 0; j < JN; ++j) nacc[j] = S(0);
+        for (int rr = 0; rr < nrows; ++rr) {
+          const S tv = Tt[rr * (KC + 1) + kk];
+#pragma unroll
+          for (int j = 0; j < JN; ++j) nacc[j] += tv * W[rr * RC + rl + 8 * j];
+        }
+#pragma unroll
+        for (int j = 0; j < JN; ++j) Np[(k0 + kk) * RC + rl + 8 * j] += nacc[j];
+      }
+    }
+    if (Y && yr < nrows) {
+#pragma unroll
+      for (int j = 0; j < JN; ++j) {
+        const int r = rl + 8 * j;
+        if (r < rce) Y[(rho0 + yr) * R + rc0 + r] = yacc[j];
+      }
+    }
+  }
+  if (np_in_lds) {
+    __syncthreads();
+    S* dst = part + w * K * RC;
+    for (int64_t e = t; e < K * RC; e += 256) dst[e] = Np[e];
+  }
+}
+
+// Pass 2: N_{L-1}[k, rc0 + r] = sum_w part[w][k][r].
+template <typename S>
+__global__ void k_mttkrp_reduce_last(Shape s, int rc0, int nwg, const S* __restrict__ part,
+                                     S* __restrict__ out) {
+  constexpr int RC = RCOf<S>::v;
+  const int64_t K = s.d[s.L - 1];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= K * RC) return;
+  const int64_t k = e / RC;
+  const int r = (int)(e % RC);
+  if (rc0 + r >= s.R) return;
+  S acc = S(0);
+  for (int w = 0; w < nwg; ++w) acc += part[(int64_t)w * K * RC + e];
+  out[s.off[s.L - 1] + (int64_t)(rc0 + r) * K + k] = acc;
+}
+
+// Pass 3: N_m for m < L-1 from Y.  One workgroup per (m, i_m); mode m is skipped unless its
+// bit is set in mask.
+template <typename S>
+__global__ __launch_bounds__(256) void k_mttkrp_rowreduce(Shape s, const S* __restrict__ F,
+                                                           const S* __restrict__ Y,
+                                                           unsigned mask, S* __restrict__ out) {
+  __shared__ S red[4][64];
+  const int L = s.L, R = s.R;
+  int64_t b = blockIdx.x;
+  int m = 0;
+  while (m < L - 1 && b >= s.d[m]) {
+    b -= s.d[m];
+    ++m;
+  }
+  if (m >= L - 1 || !((mask >> m) & 1u)) return;
+  const int64_t i = b;
+  const int64_t NR = s.N / s.d[L - 1];
+  const int64_t cnt = NR / s.d[m];
+  // row strides over modes 0..L-2
+  int64_t rs[TFX_MAXL];
+  rs[L - 2] = 1;
+  for (int q = L - 3; q >= 0; --q) rs[q] = rs[q + 1] * s.d[q + 1];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  for (int r0 = 0; r0 < R; r0 += 64) {
+    const int r = r0 + lane;
+    S acc = S(0);
+    if (r < R) {
+      for (int64_t e = grp; e < cnt; e += 4) {
+        // decompose e over the row modes other than m (last fastest)
+        int64_t rem = e, rho = i * rs[m];
+        S prod = S(1);
+        for (int q = L - 2; q >= 0; --q) {
+          if (q == m) continue;
+          const int64_t iq = rem % s.d[q];
+          rem /= s.d[q];
+          rho += iq * rs[q];
+          prod *= F[s.off[q] + (int64_t)r * s.d[q] + iq];
+        }
+        acc += Y[rho * R + r] * prod;
+      }
+    }
+    __syncthreads();
+    red[grp][lane] = acc;
+    __syncthreads();
+    if (grp == 0 && r < R)
+      out[s.off[m] + (int64_t)r * s.d[m] + i] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+  }
+}
+
+// Residual: per-workgroup partial sums of (T - model)^2 in FP64.
+template <typename TS, typename S, int RMAX>
+__global__ __launch_bounds__(256) void k_residual_rows(const TS* __restrict__ T, Shape s,
+                                                        const S* __restrict__ F,
+                                                        int64_t rows_per_wg,
+                                                        double* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int FS = RMAX + 1;     // padded row pitch: threads of a wave read different k
+  S* W = (S*)smem;                 // RB x RMAX
+  S* Fl = W + RB * RMAX;           // KC x FS
+  __shared__ double red[4];
+  const int t = threadIdx.x;
+  const int L = s.L, R = s.R;
+  const int64_t K = s.d[L - 1];
+  const int64_t NR = s.N / K;
+  const int64_t w = blockIdx.x;
+  const int64_t rho_begin = w * rows_per_wg;
+  const int64_t rho_end = min(NR, rho_begin + rows_per_wg);
+  const S* Flast = F + s.off[L - 1];
+  double acc = 0.0;
+  const int kk = t & 31;
+  const int rq = t >> 5;  // 0..7, rows rq + 8 q
+  for (int64_t rho0 = rho_begin; rho0 < rho_end; rho0 += RB) {
+    const int nrows = (int)min((int64_t)RB, rho_end - rho0);
+    __syncthreads();
+    for (int e = t; e < RB * RMAX; e += 256) {
+      const int rr = e / RMAX, r = e % RMAX;
+      S v = S(0);
+      if (rr < nrows && r < R) {
+        int64_t idx[TFX_MAXL];
+        row_to_idx(s, rho0 + rr, idx);
+        v = S(1);
+        for (int m = 0; m < L - 1; ++m) v *= F[s.off[m] + (int64_t)r * s.d[m] + idx[m]];
+      }
+      W[e] = v;
+    }
+    for (int64_t k0 = 0; k0 < K; k0 += KC) {
+      __syncthreads();
+      for (int e = t; e < KC * RMAX; e += 256) {
+        const int r = e / KC, k = e % KC;
+        S v = S(0);
+        if (r < R && k0 + k < K) v = Flast[(int64_t)r * K + k0 + k];
+        Fl[k * FS + r] = v;
+      }
+      __syncthreads();
+      if (k0 + kk < K) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = rq + 8 * q;
+          if (rr < nrows) {
+            S mdl = S(0);
+            for (int r = 0; r < R; ++r) mdl += W[rr * RMAX + r] * Fl[kk * FS + r];
+            const double d = (double)T[(rho0 + rr) * K + k0 + kk] - (double)mdl;
+            acc += d * d;
+          }
+        }
+      }
+    }
+  }
+  const double tot = block_sum(acc, red);
+  if (t == 0) part[w] = tot;
+}
+
+__global__ void k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
+  const double tot = block_sum(acc, red);
+  if (threadIdx.x == 0) *out = tot;
+}
+
+template <typename S>
+int mttkrp_native(tfx_ctx* ctx, const S* T, const Shape& s, const S* F, unsigned mask, S* out,
+                  hipStream_t stream) {
+  constexpr int RC = RCOf<S>::v;
+  const int L = s.L, R = s.R;
+  const int64_t K = s.d[L - 1];
+  const int64_t NR = s.N / K;
+  const int64_t nblocks = (NR + RB - 1) / RB;
+  int nwg = (int)std::min<int64_t>(nblocks, (int64_t)ctx->cu_count);
+  if (nwg < 1) nwg = 1;
+  const int64_t rows_per_wg = ((nblocks + nwg - 1) / nwg) * RB;
+  nwg = (int)((NR + rows_per_wg - 1) / rows_per_wg);
+  const bool need_rows = (mask & ((1u << (L - 1)) - 1u)) != 0;
+  const size_t y_bytes = need_rows ? align256(sizeof(S) * NR * R) : 0;
+  const size_t p_bytes = align256(sizeof(S) * (size_t)nwg * K * RC);
+  void* wsp;
+  int rc = tfx_workspace(ctx, y_bytes + p_bytes + 256, stream, &wsp);
+  if (rc) return rc;
+  S* Y = need_rows ? (S*)wsp : nullptr;
+  S* part = (S*)((char*)wsp + y_bytes);
+
+  size_t lds_base = sizeof(S) * (RB * (KC + 1) + KC * RC + RB * RC);
+  size_t lds_np = sizeof(S) * (size_t)K * RC;
+  int np_in_lds = (lds_base + lds_np) <= 150 * 1024;
+  size_t lds = lds_base + (np_in_lds ? lds_np : 0);
+  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_mttkrp_rows<S>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  for (int rc0 = 0; rc0 < R; rc0 += RC) {
+    hipLaunchKernelGGL(k_mttkrp_rows<S>, dim3(nwg), dim3(256), lds, stream, T, s, F, rc0,
+                       rows_per_wg, Y, part, np_in_lds);
+    TFX_CHECK_LAUNCH(ctx);
+    if ((mask >> (L - 1)) & 1u) {
+      const int64_t tot = K * RC;
+      hipLaunchKernelGGL(k_mttkrp_reduce_last<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                         0, stream, s, rc0, nwg, part, out);
+      TFX_CHECK_LAUNCH(ctx);
+    }
+  }
+  if (need_rows) {
+    int64_t nb = 0;
+    for (int m = 0; m < L - 1; ++m) nb += s.d[m];
+    hipLaunchKernelGGL(k_mttkrp_rowreduce<S>, dim3((unsigned)nb), dim3(256), 0, stream, s, F, Y,
+                       mask, out);
+    TFX_CHECK_LAUNCH(ctx);
+  }
+  return TFX_OK;
+}
+
+template <typename TS, typename S, int RMAX>
+int residual_native(tfx_ctx* ctx, const TS* T, const Shape& s, const S* F, double* out,
+                    hipStream_t stream) {
+  const int64_t K = s.d[s.L - 1];
+  const int64_t NR = s.N / K;
+  const int64_t nblocks = (NR + RB - 1) / RB;
+  int nwg = (int)std::min<int64_t>(nblocks, (int64_t)ctx->cu_count * 4);
+  if (nwg < 1) nwg = 1;
+  const int64_t rows_per_wg = ((nblocks + nwg - 1) / nwg) * RB;
+  nwg = (int)((NR + rows_per_wg - 1) / rows_per_wg);
+  void* wsp;
+  int rc = tfx_workspace(ctx, align256(sizeof(double) * nwg), stream, &wsp);
+  if (rc) return rc;
+  double* part = (double*)wsp;
+  size_t lds = sizeof(S) * (RB * RMAX + KC * (RMAX + 1));
+  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_residual_rows<TS, S, RMAX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_residual_rows<TS, S, RMAX>), dim3(nwg), dim3(256), lds, stream, T, s, F,
+                     rows_per_wg, part);
+  TFX_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, stream, part, nwg, out);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+template <typename TS, typename S>
+int residual_dispatch(tfx_ctx* ctx, const void* T, const Shape& s, const void* F, double* out,
+                      hipStream_t stream) {
+  if (s.R <= 64) return residual_native<TS, S, 64>(ctx, (const TS*)T, s, (const S*)F, out, stream);
+  return residual_native<TS, S, 256>(ctx, (const TS*)T, s, (const S*)F, out, stream);
+}
+
+__global__ __launch_bounds__(256) void k_stats_f32(const float* __restrict__ x, int64_t n, double* part) {
+  __shared__ double red[4];
+  double a = 0.0, q = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[e];
+    a += fabs(v);
+    q += v * v;
+  }
+  a = block_sum(a, red);
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = q;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_stats_f64(const double* __restrict__ x, int64_t n, double* part) {
+  __shared__ double red[4];
+  double a = 0.0, q = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[e];
+    a += fabs(v);
+    q += v * v;
+  }
+  a = block_sum(a, red);
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = q;
+  }
+}
+
+__global__ void k_stats_final(const double* __restrict__ part, int nb, double* out) {
+  __shared__ double red[4];
+  double a = 0.0, q = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    a += part[2 * i];
+    q += part[2 * i + 1];
+  }
+  a = block_sum(a, red);
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = q;
+  }
+}
+
+}  // namespace
+
+int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const float* F,
+                       unsigned mask, float* out, hipStream_t stream);
+
+extern "C" int tfx_mttkrp(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
+                          const void* F_flat, int R, unsigned mode_mask, void* out_flat, int prec,
+                          hipStream_t stream) {
+  if (!ctx || !T || !F_flat || !out_flat) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  Shape s;
+  int rc = tfx_make_shape(ctx, L, dims, R, &s);
+  if (rc) return rc;
+  if (mode_mask == 0) return TFX_OK;
+  if (prec == TFX_PREC_SPLIT16) {
+    if (dtype != TFX_F32) return tfx_set_error(ctx, TFX_E_ARG, "split16 MTTKRP needs FP32 data");
+    return tfx_mttkrp_split16(ctx, (const float*)T, s, (const float*)F_flat, mode_mask,
+                              (float*)out_flat, stream);
+  }
+  if (dtype == TFX_F32)
+    return mttkrp_native<float>(ctx, (const float*)T, s, (const float*)F_flat, mode_mask,
+                                (float*)out_flat, stream);
+  if (dtype == TFX_F64)
+    return mttkrp_native<double>(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
+                                 (double*)out_flat, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+}
+
+extern "C" int tfx_cpd_residual(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
+                                const void* F_flat, int R, double* out_sqnorm, hipStream_t stream) {
+  return tfx_cpd_residual_mixed(ctx, T, dtype, L, dims, F_flat, dtype, R, out_sqnorm, stream);
+}
+
+extern "C" int tfx_cpd_residual_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L,
+                                      const int64_t* dims, const void* F_flat, int f_dtype, int R,
+                                      double* out_sqnorm, hipStream_t stream) {
+  if (!ctx || !T || !F_flat || !out_sqnorm) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  Shape s;
+  int rc = tfx_make_shape(ctx, L, dims, R, &s);
+  if (rc) return rc;
+  if (R > 256) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "residual supports R <= 256");
+  if (t_dtype == TFX_F32 && f_dtype == TFX_F32) return residual_dispatch<float, float>(ctx, T, s, F_flat, out_sqnorm, stream);
+  if (t_dtype == TFX_F32 && f_dtype == TFX_F64) return residual_dispatch<float, double>(ctx, T, s, F_flat, out_sqnorm, stream);
+  if (t_dtype == TFX_F64 && f_dtype == TFX_F64) return residual_dispatch<double, double>(ctx, T, s, F_flat, out_sqnorm, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "unsupported dtype pair %d/%d", t_dtype, f_dtype);
+}
+
+extern "C" int tfx_tensor_stats(tfx_ctx* ctx, const void* X, int dtype, int64_t n, double* out,
+                                hipStream_t stream) {
+  if (!ctx || !X || !out) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  int nb = (int)std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * 4);
+  if (nb < 1) nb = 1;
+  void* wsp;
+  int rc = tfx_workspace(ctx, sizeof(double) * 2 * nb + 256, stream, &wsp);
+  if (rc) return rc;
+  double* part = (double*)wsp;
+  if (dtype == TFX_F32)
+    hipLaunchKernelGGL(k_stats_f32, dim3(nb), dim3(256), 0, stream, (const float*)X, n, part);
+  else if (dtype == TFX_F64)
+    hipLaunchKernelGGL(k_stats_f64, dim3(nb), dim3(256), 0, stream, (const double*)X, n, part);
+  else
+    return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, stream, part, nb, out);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}

@@ -1,0 +1,107 @@
+"""CPU-side checks of the product: the C-ABI library loads and exports every symbol declared in
+include/tfxhip.h (no compute call is made without a GPU), and the host logic of the GPU driver
+agrees with the oracle."""
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle.tfx_oracle as o
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, 'include', 'tfxhip.h')).read()
+    return sorted(set(re.findall(r'^(?:int|const char\*)\s+(tfx_\w+)\s*\(', text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = _declared_symbols()
+    assert 'tfx_mttkrp' in syms and 'tfx_cg' in syms and len(syms) >= 15
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from TensorFox import _lib
+    lib = _lib.load()
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+    # the ctypes prototypes cover every declared compute symbol
+    for name in _declared_symbols():
+        if name != 'tfx_last_error':
+            assert name in _lib.SIGNATURES, name
+
+
+def test_make_options_matches_oracle():
+    import TensorFox as tfx
+
+    class u:
+        maxiter = 7
+        refine = True
+        inner_method = ['cg', 'cg_static']
+        bi_method = 'cg'
+    a, b = tfx.make_options(u), o.make_options(u)
+    for k in vars(b):
+        assert getattr(a, k) == getattr(b, k), k
+
+
+def test_consistency_messages_match_oracle():
+    import TensorFox as tfx
+    opt = tfx.make_options(False)
+    for R, dims in [(2.0, (3, 3, 3)), (0, (3, 3, 3)), (2, (3, 1, 3)), (5, (2, 2, 3)), (1, (3, 3, 3, 3))]:
+        with pytest.raises(SystemExit) as e1:
+            tfx.consistency(R, dims, opt)
+        with pytest.raises(SystemExit) as e2:
+            o.consistency(R, dims, o.make_options(False))
+        assert str(e1.value) == str(e2.value)
+
+
+def test_sort_dims_matches_oracle():
+    import TensorFox as tfx
+    T = np.zeros((3, 7, 5, 7))
+    d, ordering = tfx.sort_dims(T.shape)
+    Ts, ordering2 = o.sort_dims(T)
+    assert d == Ts.shape and list(ordering) == list(ordering2)
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_dogleg_coefficients_match_oracle(seed):
+    from TensorFox.gauss_newton import _dogleg_coeffs
+    rs = np.random.RandomState(seed)
+    g = rs.randn(12)
+    B = rs.randn(12, 12)
+    H = B @ B.T + 0.1 * np.eye(12)
+    y = np.linalg.solve(H, g) * rs.uniform(0.5, 1.5)
+    Jg = H @ g
+    for delta in [1e-3, 0.1, 0.5, 1.0, 3.0, 100.0]:
+        a, b = _dogleg_coeffs(np.array([y @ y, y @ g, g @ g, g @ Jg]), delta)
+        assert np.allclose(a * y + b * g, o.dogleg(y, g, Jg, delta), rtol=1e-10, atol=1e-12)
+
+
+def test_clean_compression_matches_oracle():
+    from TensorFox.compression import clean_compression
+    for s in [np.array([10.0, 1.0, 1e-9, 1e-10]), np.array([3.0, 2.0, 1.0]), np.array([1.0, 1e-20])]:
+        _, sig, _, dim = o.clean_compression(np.eye(len(s)), s, np.eye(len(s)), 1e-16, 3)
+        assert clean_compression(s, 1e-16, 3) == dim
+
+
+def test_generators_are_seeded():
+    import TensorFox as tfx
+    T1, F1 = tfx.make_lowrank((6, 5, 4), 3, 1e-3, seed=3, col_scale=True)
+    T2, F2 = tfx.make_lowrank((6, 5, 4), 3, 1e-3, seed=3, col_scale=True)
+    assert np.array_equal(T1, T2)
+    T0 = tfx.cpd2tens(F1)
+    assert np.linalg.norm(T1 - T0) / np.linalg.norm(T0) == pytest.approx(1e-3, rel=1e-9)
+    T, F = tfx.make_lowrank((8, 5, 4, 3), 3, 0, seed=1, collinear=0.9)
+    a1, a2 = F[0][:, 0], F[0][:, 1]
+    assert a1 @ a2 / np.linalg.norm(a1) / np.linalg.norm(a2) == pytest.approx(0.9, abs=1e-12)
+
+
+def test_product_does_not_import_oracle():
+    pkg = os.path.join(ROOT, 'tensor-fox_amd', 'TensorFox')
+    for f in os.listdir(pkg):
+        if f.endswith('.py'):
+            src = open(os.path.join(pkg, f)).read()
+            assert 'oracle' not in src, f

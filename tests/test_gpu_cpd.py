@@ -1,0 +1,143 @@
+"""End-to-end parity of the GPU `cpd` with the FP64 oracle (which reproduces the reference's
+seeded trajectories, see test_oracle_golden.py) and with the reference's doctest thresholds.
+
+Parity criterion (per problem, same seed for both sides): final relative error within 1e-6
+absolute of the oracle's, and factors equal up to permutation and scaling with congruence above
+0.999.  The GPU path computes the MLSVD bases by subspace iteration on Gram matrices instead of a
+randomized SVD, so per-iteration trajectories are not compared."""
+
+import numpy as np
+import pytest
+from scipy.optimize import linear_sum_assignment
+
+import oracle.tfx_oracle as o
+from tests.golden.cases import make_opts
+
+pytestmark = pytest.mark.gpu
+
+
+def congruence(A, B):
+    """Min over matched rank-one terms of prod_l |cos(a_l^r, b_l^s)| after the best matching."""
+    R = A[0].shape[1]
+    C = np.ones((R, R))
+    for a, b in zip(A, B):
+        an = a / np.linalg.norm(a, axis=0)
+        bn = b / np.linalg.norm(b, axis=0)
+        C *= np.abs(an.T @ bn)
+    r, c = linear_sum_assignment(-C)
+    return C[r, c].min()
+
+
+def _both(T, R, kw, seed, precision=None):
+    import TensorFox as tfx
+    np.random.seed(seed)
+    fo, oo = o.cpd(np.asarray(T, dtype=np.float64), R, make_opts(kw))
+    kw2 = dict(kw)
+    if precision:
+        kw2['precision'] = precision
+    np.random.seed(seed)
+    fg, og = tfx.cpd(T, R, make_opts(kw2))
+    return fo, oo, fg, og
+
+
+WELL_POSED = [
+    ('c567_refine', dict(refine=True), 5),
+    ('cbig_refine', dict(refine=True), 6),
+    ('c4_refine', dict(refine=True, maxiter=60), 8),
+]
+
+
+@pytest.mark.parametrize('name,kw,seed', WELL_POSED)
+def test_golden_problems_fp64(golden, name, kw, seed):
+    print('cpd fp64', name, flush=True)
+    T = golden[name + '_T']
+    R = int(golden[name + '_R'])
+    fo, oo, fg, og = _both(T, R, kw, seed)
+    assert abs(og.rel_error - float(golden[name + '_rel_error'])) < 1e-6
+    assert abs(og.rel_error - oo.rel_error) < 1e-6
+    assert congruence(fg, fo) > 0.999
+    assert og.stop[1] in range(8)
+
+
+@pytest.mark.parametrize('dims,R,nu,seed', [((100, 100, 100), 10, 1e-6, 0), ((30, 40, 50), 6, 1e-4, 1),
+                                            ((24, 20, 16, 12), 5, 1e-5, 2)])
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_seeded_lowrank(dims, R, nu, seed, precision):
+    import TensorFox as tfx
+    print('cpd', dims, R, precision, flush=True)
+    T, Ftrue = tfx.make_lowrank(dims, R, nu, seed=seed)
+    if precision == 'mixed':
+        T = T.astype(np.float32)
+    fo, oo, fg, og = _both(T, R, dict(refine=True), seed, precision)
+    assert abs(og.rel_error - oo.rel_error) < 1e-6, (og.rel_error, oo.rel_error)
+    assert abs(og.final_rel_error - oo.rel_error) < 1e-6
+    assert congruence(fg, fo) > 0.999
+    assert congruence(fg, Ftrue) > 0.999
+
+
+def _t222():
+    T = np.zeros((2, 2, 2))
+    s = 0
+    for k in range(2):
+        for i in range(2):
+            for j in range(2):
+                T[i, j, k] = s
+                s += 1
+    return T
+
+
+def test_doctest_simple_tensor_thresholds_gpu():
+    """tests/simple_tensor_cpd_options.py:17-52 on the GPU path (FP64)."""
+    import TensorFox as tfx
+    T = _t222()
+    best = 1.0
+    for seed in range(10):
+        np.random.seed(seed)
+        _, out = tfx.cpd(T, 3)
+        best = min(best, out.rel_error)
+    assert best < 1e-14
+
+    class opt:
+        pass
+    opt.initialization = [np.ones((2, 3))] * 3
+    _, out = tfx.cpd(T, 3, opt)
+    assert out.rel_error < 0.14
+    opt.refine = True
+    _, out = tfx.cpd(T, 3, opt)
+    assert out.rel_error < 0.14
+    opt.inner_method, opt.cg_maxiter, opt.cg_tol, opt.refine = 'cg_static', 3, 1e-7, False
+    _, out = tfx.cpd(T, 3, opt)
+    assert out.rel_error < 0.14
+
+
+def test_rank1_known_answer_gpu():
+    """tutorial/6 cell 10: 0.13080484499593875 (rank-1, no refinement)."""
+    import TensorFox as tfx
+    np.random.seed(0)
+    _, out = tfx.cpd(_t222(), 1)
+    assert out.rel_error == pytest.approx(0.13080484499593875, abs=1e-12)
+
+
+def test_swimmer_compression_gpu(swimmer):
+    """tutorial/7 cell 4: the sorted swimmer compresses from (256, 32, 32) to (13, 23, 15)."""
+    import TensorFox as tfx
+
+    class opt:
+        maxiter = 3
+    np.random.seed(0)
+    _, out = tfx.cpd(swimmer, 50, opt)
+    assert out.compressed_dims == (13, 23, 15)
+
+
+def test_dogleg_and_schedules_run(golden):
+    """The dogleg fallback (tol_jump < 1) and scheduled damping / inner methods execute and give
+    errors in the range of the reference's runs."""
+    import TensorFox as tfx
+    for name, kw, seed in [('cjump_dogleg', dict(tol_jump=0.9, maxiter=30), 9),
+                           ('cjump_sched', dict(init_damp=[0.5] * 25, maxiter=25,
+                                                inner_method=['cg'] * 10 + ['cg_static'] * 15), 10)]:
+        T = golden[name + '_T']
+        np.random.seed(seed)
+        _, og = tfx.cpd(T, 3, make_opts(kw))
+        assert og.num_steps == int(golden[name + '_num_steps'])
+        assert og.rel_error < 1.0

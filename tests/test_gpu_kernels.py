@@ -1,0 +1,290 @@
+"""Parity of each libtfxhip kernel with the FP64 oracle on the same seeded inputs (GPU).
+
+Tolerances: FP64 kernels agree with the oracle to ~1e-12 relative (different summation order
+only); FP32 kernels to ~1e-5 relative of the result's scale."""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.tfx_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.float64: 1e-11, torch.float32: 2e-5}
+
+
+@pytest.fixture(scope='module')
+def dev():
+    from TensorFox.device import get_device
+    return get_device(0)
+
+
+def _flat(F):
+    return np.concatenate([f.flatten('F') for f in F])
+
+
+def _unflat(x, dims, R):
+    out, s = [], 0
+    for d in dims:
+        out.append(x[s:s + R * d].reshape(R, d).T)
+        s += R * d
+    return out
+
+
+def _case(dims, R, seed):
+    rs = np.random.RandomState(seed)
+    T = rs.randn(*dims)
+    F = [rs.randn(d, R) for d in dims]
+    return T, F
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+MTTKRP_CASES = [((5, 6, 7), 3), ((20, 30, 40), 10), ((4, 5, 6, 7), 5), ((33, 17, 70), 70),
+                ((2, 2, 2), 3), ((50, 50, 50), 50), ((7, 3), 4), ((3, 4, 2, 3, 2), 2)]
+
+
+@pytest.mark.parametrize('dt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('dims,R', MTTKRP_CASES)
+def test_mttkrp_all_modes(dev, dims, R, dt):
+    print('mttkrp', dims, R, dt, flush=True)
+    T, F = _case(dims, R, 1)
+    Td = dev.upload(T.ravel(), dt)
+    Fd = dev.upload(_flat(F), dt)
+    out = dev.zeros(R * sum(dims), dt)
+    dev.mttkrp(Td, dims, Fd, R, out)
+    got = _unflat(out.cpu().double().numpy(), dims, R)
+    for l in range(len(dims)):
+        ref = o.mttkrp(T, F, l)
+        assert _rel(got[l], ref) < TOL[dt], (l, _rel(got[l], ref))
+
+
+def test_mttkrp_mode_mask(dev):
+    dims, R = (6, 7, 8), 4
+    T, F = _case(dims, R, 2)
+    Td = dev.upload(T.ravel(), torch.float64)
+    Fd = dev.upload(_flat(F), torch.float64)
+    out = dev.zeros(R * sum(dims), torch.float64)
+    dev.mttkrp(Td, dims, Fd, R, out, mask=0b010)
+    got = _unflat(out.cpu().numpy(), dims, R)
+    assert np.all(got[0] == 0) and np.all(got[2] == 0)
+    assert _rel(got[1], o.mttkrp(T, F, 1)) < 1e-12
+
+
+@pytest.mark.parametrize('dt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('dims,R', [((5, 6, 7), 3), ((40, 30, 20), 12), ((4, 5, 6, 7), 5),
+                                    ((9, 10, 11), 100), ((2, 2, 2), 1)])
+def test_residual(dev, dims, R, dt):
+    print('residual', dims, R, dt, flush=True)
+    rs = np.random.RandomState(3)
+    F = [rs.randn(d, R) for d in dims]
+    T = o.cpd_tensor(F) + 1e-3 * rs.randn(*dims)
+    Td = dev.upload(T.ravel(), dt)
+    Fd = dev.upload(_flat(F), dt)
+    out = dev.zeros(1, torch.float64)
+    dev.residual_sq(Td, dims, Fd, R, out)
+    ref = np.sum((T - o.cpd_tensor(F)) ** 2)
+    tol = 1e-9 if dt == torch.float64 else 0.3   # FP32 model values vs a 1e-3 residual
+    assert abs(out.item() - ref) / ref < tol
+
+
+def test_residual_fp32_tensor_fp64_factors(dev):
+    dims, R = (30, 20, 25), 6
+    rs = np.random.RandomState(4)
+    F = [rs.randn(d, R) for d in dims]
+    T = (o.cpd_tensor(F) + 1e-5 * rs.randn(*dims)).astype(np.float32)
+    out = dev.zeros(1, torch.float64)
+    dev.residual_sq(dev.upload(T.ravel(), torch.float32), dims, dev.upload(_flat(F), torch.float64),
+                    R, out)
+    ref = np.sum((T.astype(np.float64) - o.cpd_tensor(F)) ** 2)
+    assert abs(out.item() - ref) / ref < 1e-9
+
+
+@pytest.mark.parametrize('dt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('dims,R', [((4, 5, 6), 3), ((3, 4, 5, 6), 7), ((50, 50, 50), 50)])
+def test_gramians_and_grad(dev, dims, R, dt):
+    print('gramians', dims, R, dt, flush=True)
+    T, F = _case(dims, R, 5)
+    F[0] *= 3.0
+    L = len(dims)
+    Fd = dev.upload(_flat(F), dt)
+    Gr, P1, P2 = dev.empty((L, R, R), dt), dev.empty((L, R, R), dt), dev.empty((L, L, R, R), dt)
+    gamma = dev.empty((L, R), dt)
+    dev.gramians(dims, R, Fd, Gr, P1, P2, gamma)
+    rGr, rP1, rP2 = o.gramians(F)
+    _, rgamma = o.regularization(rP1, list(dims))
+    tol = TOL[dt] * 10
+    assert _rel(Gr.cpu().double().numpy(), rGr) < tol
+    assert _rel(P1.cpu().double().numpy(), rP1) < tol
+    for l in range(L):
+        for m in range(L):
+            if l != m:
+                assert _rel(P2[l, m].cpu().double().numpy(), rP2[l, m]) < tol
+    assert _rel(gamma.cpu().double().numpy(), rgamma) < tol
+    N = dev.zeros(R * sum(dims), dt)
+    Td = dev.upload(T.ravel(), dt)
+    dev.mttkrp(Td, dims, Fd, R, N)
+    g = dev.empty(R * sum(dims), dt)
+    dev.neg_grad(dims, R, Fd, P1, N, g)
+    ref = -o.compute_grad(T, F, rP1)
+    assert _rel(g.cpu().double().numpy(), ref) < TOL[dt] * 10
+
+
+@pytest.mark.parametrize('maxiter', [1, 2, 3, 6, 40])
+def test_cg_matches_reference_vectors(dev, golden, maxiter):
+    """CG against the reference's own output (tests/golden), FP64."""
+    T = golden['cg_T']
+    F = [golden['cg_F%d' % l] for l in range(3)]
+    dims, R = T.shape, 3
+    y, jtjg, scal = _run_cg(dev, T, F, 0.37, maxiter, 0.0, torch.float64)
+    ref = golden['cg_y_%d' % maxiter]
+    assert _rel(y, ref) < 1e-10
+    assert int(scal[0]) == int(golden['cg_itn_%d' % maxiter])
+    assert _rel(jtjg, golden['cg_jtjg']) < 1e-12
+
+
+def _run_cg(dev, T, F, damp, maxiter, tol, dt):
+    dims, R = T.shape, F[0].shape[1]
+    L = len(dims)
+    Fd = dev.upload(_flat(F), dt)
+    Td = dev.upload(T.ravel(), dt)
+    Gr, P1, P2 = dev.empty((L, R, R), dt), dev.empty((L, R, R), dt), dev.empty((L, L, R, R), dt)
+    gamma = dev.empty((L, R), dt)
+    dev.gramians(dims, R, Fd, Gr, P1, P2, gamma)
+    N = dev.zeros(R * sum(dims), dt)
+    dev.mttkrp(Td, dims, Fd, R, N)
+    g = dev.empty(R * sum(dims), dt)
+    dev.neg_grad(dims, R, Fd, P1, N, g)
+    y, jtjg = dev.empty(R * sum(dims), dt), dev.empty(R * sum(dims), dt)
+    scal = dev.zeros(2, torch.float64)
+    dev.cg(dims, R, Fd, P1, P2, gamma, damp, g, maxiter, tol, y, jtjg, scal)
+    return y.cpu().double().numpy(), jtjg.cpu().double().numpy(), scal.cpu().numpy()
+
+
+@pytest.mark.parametrize('dims,R,maxiter', [((12, 10, 14), 4, 7), ((30, 25, 20), 8, 15),
+                                            ((6, 5, 4, 3), 3, 9)])
+def test_cg_matches_oracle(dev, dims, R, maxiter):
+    print('cg', dims, R, flush=True)
+    rs = np.random.RandomState(6)
+    F = [rs.randn(d, R) for d in dims]
+    T = o.cpd_tensor([f + 0.1 * rs.randn(*f.shape) for f in F])
+    y, jtjg, scal = _run_cg(dev, T, F, 0.2, maxiter, 1e-16, torch.float64)
+    ry, rg, rj, ritn, rrn = o.cg(T, [f.copy() for f in F], 0.2, maxiter, 1e-16)
+    assert _rel(y, ry) < 1e-9
+    assert _rel(jtjg, rj) < 1e-11
+    assert int(scal[0]) == ritn
+    y32, _, _ = _run_cg(dev, T, F, 0.2, maxiter, 1e-16, torch.float32)
+    assert _rel(y32, ry) < 1e-3
+
+
+@pytest.mark.parametrize('dt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('symm,fnorm', [(False, 0.0), (True, 0.0), (False, 2.5)])
+def test_dgn_update(dev, dt, symm, fnorm):
+    dims, R = (5, 5, 5), 3
+    rs = np.random.RandomState(7)
+    x = rs.randn(R * sum(dims))
+    y = rs.randn(R * sum(dims))
+    g = rs.randn(R * sum(dims))
+    damp = 0.3
+    xd, yd, gd = dev.upload(x, dt), dev.upload(y, dt), dev.upload(g, dt)
+    xn, Fd = dev.empty(x.size, dt), dev.empty(x.size, dt)
+    scal = dev.zeros(4, torch.float64)
+    dev.dgn_update(dims, R, xd, yd, 1.0, gd, damp, True, symm, fnorm, xn, Fd, scal)
+    ref_x = x + y
+    F = o.x2cpd(ref_x, [np.zeros((d, R)) for d in dims])
+    F = o.transform(F, symm, fnorm)
+    tol = TOL[dt] * 10
+    assert _rel(xn.cpu().double().numpy(), ref_x) < tol
+    assert _rel(Fd.cpu().double().numpy(), _flat(F)) < tol
+    s = scal.cpu().numpy()
+    dx = ref_x - x
+    assert s[0] == pytest.approx(dx @ (g + damp * dx), rel=tol * 10)
+    assert s[1] == pytest.approx(dx @ dx, rel=tol * 10)
+    assert s[2] == pytest.approx(x @ x, rel=tol * 10)
+    assert s[3] == pytest.approx(np.abs(g).max(), rel=tol * 10)
+
+
+@pytest.mark.parametrize('dt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('dims', [(40, 30, 20), (5, 6, 7, 8), (64, 64, 1), (3, 100, 2)])
+def test_gram_mode(dev, dims, dt):
+    print('gram', dims, dt, flush=True)
+    X = np.random.RandomState(8).randn(*dims)
+    Xd = dev.upload(X.ravel(), dt)
+    for l in range(len(dims)):
+        G = dev.empty((dims[l], dims[l]), torch.float64)
+        dev.gram_mode(Xd, dims, l, G)
+        U = o.unfold(X, l + 1)
+        assert _rel(G.cpu().numpy(), U @ U.T) < TOL[dt] * 5
+
+
+@pytest.mark.parametrize('n,k,p,kind', [(40, 5, 15, 'gap'), (400, 50, 60, 'gap'), (12, 3, 13, 'full'),
+                                        (100, 10, 20, 'gap'), (64, 16, 26, 'gap')])
+def test_eig_top(dev, n, k, p, kind):
+    print('eig', n, k, p, flush=True)
+    rs = np.random.RandomState(9)
+    Q, _ = np.linalg.qr(rs.randn(n, n))
+    if kind == 'gap':
+        lam = np.concatenate([10.0 ** rs.uniform(0, 3, k), 1e-8 * rs.rand(n - k)])
+    else:
+        lam = rs.rand(n) * 10
+    G = (Q * lam) @ Q.T
+    G = 0.5 * (G + G.T)
+    Gd = dev.upload(G.ravel(), torch.float64)
+    Q0 = dev.upload(rs.randn(n, p).ravel(order='F'), torch.float64)
+    U = dev.empty((k, n), torch.float64)
+    lam_d = dev.empty(k, torch.float64)
+    dev.eig_top(Gd, n, Q0, p, k, 3, U, lam_d)
+    w, V = np.linalg.eigh(G)
+    w, V = w[::-1][:k], V[:, ::-1][:, :k]
+    assert np.allclose(lam_d.cpu().numpy(), w, rtol=1e-10, atol=1e-10 * w[0])
+    Ug = U.cpu().numpy().T
+    # subspace agreement and sign convention (largest |entry| positive)
+    assert np.linalg.norm(Ug @ (Ug.T @ V) - V) < 1e-8
+    for c in range(k):
+        assert Ug[np.argmax(np.abs(Ug[:, c])), c] > 0
+
+
+@pytest.mark.parametrize('dt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('dims,mode,k', [((20, 30, 40), 0, 5), ((20, 30, 40), 1, 7), ((20, 30, 40), 2, 3),
+                                         ((4, 5, 6, 7), 2, 4)])
+def test_ttm_and_row_norms(dev, dims, mode, k, dt):
+    rs = np.random.RandomState(10)
+    X = rs.randn(*dims)
+    U = rs.randn(dims[mode], k)
+    Xd = dev.upload(X.ravel(), dt)
+    Ud = dev.upload(U.ravel(order='F'), torch.float64)
+    ydims = list(dims)
+    ydims[mode] = k
+    Y = dev.empty(int(np.prod(ydims)), dt)
+    sig2 = dev.empty(k, torch.float64)
+    dev.ttm(Xd, dims, mode, Ud, k, Y, sig2)
+    ref = np.moveaxis(np.tensordot(U.T, np.moveaxis(X, mode, 0), axes=1), 0, mode)
+    got = Y.cpu().double().numpy().reshape(ydims)
+    assert _rel(got, ref) < TOL[dt] * 5
+    assert _rel(sig2.cpu().numpy(), np.sum(o.unfold(got, mode + 1) ** 2, axis=1)) < 1e-10
+
+
+def test_gemm_small(dev):
+    rs = np.random.RandomState(11)
+    A = rs.randn(7, 5)
+    B = rs.randn(5, 3)
+    C = dev.empty(21, torch.float64)
+    dev.gemm_small(False, 7, 3, 5, dev.upload(A.ravel(order='F'), torch.float64),
+                   dev.upload(B.ravel(order='F'), torch.float64), C)
+    assert _rel(C.cpu().numpy().reshape(3, 7).T, A @ B) < 1e-14
+    B2 = rs.randn(7, 3)
+    C2 = dev.empty(15, torch.float64)
+    dev.gemm_small(True, 5, 3, 7, dev.upload(A.ravel(order='F'), torch.float64),
+                   dev.upload(B2.ravel(order='F'), torch.float64), C2)
+    assert _rel(C2.cpu().numpy().reshape(3, 5).T, A.T @ B2) < 1e-14
+
+
+def test_tensor_stats(dev):
+    X = np.random.RandomState(12).randn(1000, 37)
+    out = dev.zeros(2, torch.float64)
+    dev.tensor_stats(dev.upload(X.ravel(), torch.float32), out)
+    X32 = X.astype(np.float32).astype(np.float64)
+    assert out.cpu().numpy() == pytest.approx([np.abs(X32).sum(), (X32 ** 2).sum()], rel=1e-12)
