@@ -1,0 +1,224 @@
+"""Benchmark of the dense CPD hot path on one MI355X.
+
+Workload (a "step" = one full `cpd` call: MLSVD -> dGN on the core -> uncompression ->
+refinement -> final FP64 error): dense 400 x 400 x 400 FP32 tensor, rank 50, factors N(0, 1),
+relative noise 1e-5, seed 2 (an instance on which the reference algorithm converges to the noise
+floor); the tensor is resident in HBM before the timed region.  The same problem with the first
+factor's columns scaled by 10^U(-1, 1) (seed 0), on which the reference algorithm stalls at a
+relative error of about 3e-2, is timed once and reported under "colscale_instance".  The mixed-precision form is timed for `--steps`
+steps; the FP64 GPU form runs the same problem once for the speedup; the dominant kernel (the
+fused MTTKRP of the refinement) is timed with HIP events on the stream it runs on; the CPU
+baseline times the FP64 NumPy restatement of the reference algorithm on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, 'tensor-fox_amd')):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--dims', type=int, nargs='+', default=[400, 400, 400])
+    ap.add_argument('--rank', type=int, default=50)
+    ap.add_argument('--noise', type=float, default=1e-5)
+    ap.add_argument('--seed', type=int, default=2)
+    ap.add_argument('--no-colscale', action='store_true', help='skip the column-scaled instance')
+    ap.add_argument('--no-fp64', action='store_true', help='skip the FP64 GPU form')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--cpu-sample-iters', type=int, default=1)
+    ap.add_argument('--kernel-reps', type=int, default=20)
+    return ap.parse_args()
+
+
+class Opt:
+    refine = True
+
+
+def time_cpd(tfx, Tdev, R, precision, seed):
+    import numpy as np
+    import torch
+    o = Opt()
+    o.precision = precision
+    torch.cuda.synchronize()
+    np.random.seed(seed)
+    t0 = time.perf_counter()
+    factors, out = tfx.cpd(Tdev, R, o)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, factors, out
+
+
+def mttkrp_kernel_time(dev, Tdev, dims, R, prec, reps):
+    """Average duration of one full MTTKRP call (all modes) measured with HIP events recorded on
+    the stream the kernels are launched on (torch's current stream)."""
+    import numpy as np
+    import torch
+    rs = np.random.RandomState(1)
+    F = dev.upload(rs.standard_normal(R * sum(dims)), torch.float32)
+    out = dev.zeros(R * sum(dims), torch.float32)
+    for _ in range(3):
+        dev.mttkrp(Tdev, dims, F, R, out, prec=prec)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        dev.mttkrp(Tdev, dims, F, R, out, prec=prec)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def cpu_baseline(T32, R, sample_iters, seed):
+    """FP64 NumPy restatement of the reference algorithm (the oracle) on the host cores: the
+    MLSVD, `sample_iters` main-stage and `sample_iters` refinement dGN iterations are timed, then
+    scaled to the iteration counts of a full run given by the caller."""
+    import numpy as np
+    import oracle.tfx_oracle as o
+    T = T32.astype(np.float64)
+
+    class opt:
+        refine = True
+        maxiter = sample_iters
+    np.random.seed(seed)
+    t0 = time.perf_counter()
+    Ts, ordering = o.sort_dims(T)
+    S, U, T1, sig = o.mlsvd(Ts, np.linalg.norm(Ts), R, o.make_options(opt))
+    t1 = time.perf_counter()
+    init = o.starting_point(S, U, R, ordering, o.make_options(opt))
+    F, *_ = o.dGN(S, init, R, o.make_options(opt))
+    t2 = time.perf_counter()
+    F = [U[l] @ F[l] for l in range(3)]
+    o.dGN(np.asarray(Ts), F, R, o.make_options(opt))
+    t3 = time.perf_counter()
+    return dict(mlsvd=t1 - t0, main_iter=(t2 - t1) / sample_iters, refine_iter=(t3 - t2) / sample_iters)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU; the CPD hot path here is a single-GPU kernel '
+              'set (multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.',
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import TensorFox as tfx
+    from TensorFox import _lib
+    from TensorFox.device import get_device
+
+    dims, R, nu = tuple(args.dims), args.rank, args.noise
+    log('generating', dims, R, nu)
+    T, Ftrue = tfx.make_lowrank(dims, R, nu, seed=args.seed, dtype=np.float32)
+    dev = get_device(0)
+    Tdev = torch.from_numpy(T).to(dev.dev)
+    torch.cuda.synchronize()
+    T0 = tfx.cpd2tens(Ftrue)
+    T0n = float(np.linalg.norm(T0))
+
+    for w in range(args.warmup):
+        t, _, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
+        log('warmup %d: %.3f s rel_error %.3e' % (w, t, out.rel_error))
+    times, outs, facs = [], [], []
+    torch.cuda.synchronize()
+    for k in range(args.steps):
+        t, f, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
+        times.append(t)
+        outs.append(out)
+        facs.append(f)
+        log('step %d: %.3f s rel_error %.6e final %.6e steps %s stop %s timings %s' % (
+            k, t, out.rel_error, out.final_rel_error, out.num_steps, out.stop,
+            {a: round(b, 4) for a, b in out.timings.items()}))
+    ms = 1e3 * float(np.mean(times))
+    out = outs[-1]
+    model_err = float(np.linalg.norm(tfx.cpd2tens(facs[-1]) - T0)) / T0n
+
+    result = {
+        'metric': 'time to rel.err 1e-6, 400^3 R=50: mixed vs FP64 GPU dGN; MTTKRP GB/s',
+        'value': round(ms / 1e3, 5),
+        'unit': 's',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms, 3),
+        'higher_is_better': False,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32+f64',
+        'data': 'synthetic (seeded low-rank + 1e-5 relative noise)',
+        'config': {'workload': 'dense CPD %s rank %d, noise %g, seed %d, refine, mixed precision' % (
+                       'x'.join(map(str, dims)), R, nu, args.seed),
+                   'dims': list(dims), 'rank': R, 'noise': nu, 'seed': args.seed, 'refine': True,
+                   'precision': 'mixed'},
+        'rel_error': out.rel_error,
+        'final_rel_error': out.final_rel_error,
+        'model_error_vs_noise_free': model_err,
+        'num_steps': int(out.num_steps),
+        'stage_seconds': {a: round(b, 5) for a, b in out.timings.items()},
+    }
+
+    # dominant kernel: fused MTTKRP over the full tensor
+    kt = mttkrp_kernel_time(dev, Tdev, dims, R, _lib.TFX_PREC_NATIVE, args.kernel_reps)
+    N = int(np.prod(dims))
+    alg_bytes = 4 * N + 4 * R * sum(dims) * 2
+    gbs = alg_bytes / kt / 1e9
+    result['roofline'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                          'frac': round(gbs / HBM_PEAK_GBS, 4), 'traffic': None,
+                          'kernel': 'tfx_mttkrp (all modes, FP32)', 'avg_launch_s': kt}
+
+    if not args.no_fp64:
+        t64, f64, out64 = time_cpd(tfx, Tdev, R, 'fp64', args.seed)
+        log('fp64: %.3f s rel_error %.6e timings %s' % (t64, out64.rel_error, out64.timings))
+        result['fp64_seconds'] = round(t64, 5)
+        result['fp64_rel_error'] = out64.rel_error
+        result['speedup_mixed_vs_fp64'] = round(t64 / (ms / 1e3), 3)
+        result['abs_err_diff_vs_fp64'] = abs(out.rel_error - out64.rel_error)
+
+    if not args.no_colscale:
+        Tc, Fc = tfx.make_lowrank(dims, R, nu, seed=0, col_scale=True, dtype=np.float32)
+        Tcd = torch.from_numpy(Tc).to(dev.dev)
+        tm, _, om = time_cpd(tfx, Tcd, R, 'mixed', 0)
+        res = {'workload': 'same, first factor columns x 10^U(-1,1), seed 0', 'mixed_seconds': round(tm, 5),
+               'mixed_rel_error': om.rel_error}
+        if not args.no_fp64:
+            t6, _, o6 = time_cpd(tfx, Tcd, R, 'fp64', 0)
+            res.update(fp64_seconds=round(t6, 5), fp64_rel_error=o6.rel_error)
+        log('colscale instance:', res)
+        result['colscale_instance'] = res
+        del Tcd
+
+    if not args.no_cpu:
+        cores = os.cpu_count()
+        log('cpu baseline on %d cores' % cores)
+        c = cpu_baseline(T, R, args.cpu_sample_iters, args.seed)
+        n_main = int(out.errors[0].size)
+        n_ref = int(out.errors[1].size)
+        est = c['mlsvd'] + n_main * c['main_iter'] + n_ref * c['refine_iter']
+        result['cpu_baseline'] = {
+            'value': round(est, 3), 'unit': 's', 'cores': cores, 'kind': 'port',
+            'sample': 'oracle (FP64 NumPy restatement of the reference) on the same tensor: MLSVD + '
+                      '%d main + %d refinement dGN iterations timed (%.2f s, %.3f s/it, %.3f s/it), '
+                      'scaled to the %d + %d iterations of the GPU run' % (
+                          args.cpu_sample_iters, args.cpu_sample_iters, c['mlsvd'], c['main_iter'],
+                          c['refine_iter'], n_main, n_ref)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()

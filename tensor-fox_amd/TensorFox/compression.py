@@ -14,7 +14,7 @@ import torch
 
 from . import _lib
 
-SUBSPACE_ITERS = 3
+SUBSPACE_ITERS = 2
 _PRIVATE_RNG_SEED = 20240611
 
 

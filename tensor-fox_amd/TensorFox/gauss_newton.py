@@ -39,6 +39,7 @@ class Problem:
         self.F = e(n, wdt)
         self.best_F = e(n, wdt)
         self.scal = dev.zeros(16, torch.float64)
+        self.dogleg_calls = 0
         self.tmp_scal = dev.zeros(16, torch.float64)
 
     def error_sq(self, F, out):
@@ -191,6 +192,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
         x2 = float(h[_S_UPD + 2])
         gmax = float(h[_S_UPD + 3])
         if it > 3 and error > tol_jump * old_error:
+            prob.dogleg_calls += 1
             x_back = torch.empty_like(x)
             dev.axpby(1.0, x, -1.0, prob.y, x_back)
             bF, bx, by, error = _dogleg_steps(prob, x_back, prob.y.clone(), error, Tsize, damp, symm,

@@ -230,6 +230,7 @@ def tricpd(T, R, options):
                  [stop_m, stop_r], options, timings=timings, precision=precision)
     out.final_rel_error = final_err
     out.compressed_dims = tuple(S_dims)
+    out.dogleg_calls = prob.dogleg_calls + (probT.dogleg_calls if options.refine else 0)
     if display > 0:
         print('Final results')
         print('    Number of steps =', out.num_steps)

@@ -241,6 +241,266 @@ __global__ __launch_bounds__(1024) void k_cg(Shape s, const S* __restrict__ F,
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// CG for small problems (the compressed core): one workgroup, every vector held in registers as
+// 4 x 4 tiles (rows i, ranks r) of one mode, F / q / tmp / P1 staged row-major in LDS (or global
+// memory when they do not fit).  Same arithmetic as k_cg, with the matvec GEMMs register-tiled.
+// ---------------------------------------------------------------------------------------------
+constexpr int SMALL_NT = 512;
+
+struct TileMap {
+  int ntile;                 // total number of 4x4 tiles over all modes
+  int first[TFX_MAXL + 1];   // first tile of each mode
+  int rb;                    // tiles along R
+};
+
+__device__ inline void tile_of(const Shape& s, const TileMap& tm, int t, int* l, int* i0, int* r0) {
+  int m = 0;
+  while (m + 1 < s.L && t >= tm.first[m + 1]) ++m;
+  const int w = t - tm.first[m];
+  *l = m;
+  *i0 = 4 * (w / tm.rb);
+  *r0 = 4 * (w % tm.rb);
+}
+
+template <typename S>
+__device__ inline S inv_m(S g, S d) {
+  return S(1) / sqrt(g * g + d * d * (g * g));
+}
+
+// z-tile = F_l tmp_l + q_l P1_l for the tile (l, i0, r0); arrays row-major [i][r] per mode.
+template <typename S>
+__device__ inline void z_tile(const Shape& s, int l, int i0, int r0, const S* F, const S* q,
+                              const S* tmp, const S* P1, S z[16]) {
+  const int R = s.R;
+  const int64_t I = s.d[l];
+  const int64_t off = s.off[l];
+  const int64_t RR = (int64_t)R * R;
+  const S* Fl = F + off;
+  const S* ql = q + off;
+  const S* tl = tmp + l * RR;
+  const S* pl = P1 + l * RR;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) z[e] = S(0);
+  for (int c = 0; c < R; ++c) {
+    S f[4], qq[4], tv[4], pv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const bool ok = i0 + a < I;
+      f[a] = ok ? Fl[(int64_t)(i0 + a) * R + c] : S(0);
+      qq[a] = ok ? ql[(int64_t)(i0 + a) * R + c] : S(0);
+      const bool okr = r0 + a < R;
+      tv[a] = okr ? tl[(int64_t)c * R + r0 + a] : S(0);
+      pv[a] = okr ? pl[(int64_t)c * R + r0 + a] : S(0);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) z[a * 4 + b] += f[a] * tv[b] + qq[a] * pv[b];
+  }
+}
+
+// tmp_l = sum_{m != l} P2[l][m] o (q_m^T F_m), computed per 4 x 4 (c, r) tile for all modes.
+template <typename S>
+__device__ inline void tmp_phase(const Shape& s, const S* F, const S* q, const S* __restrict__ P2,
+                                 S* tmp) {
+  const int R = s.R, L = s.L;
+  const int rb = (R + 3) / 4;
+  const int64_t RR = (int64_t)R * R;
+  for (int t = threadIdx.x; t < rb * rb; t += blockDim.x) {
+    const int c0 = 4 * (t / rb), r0 = 4 * (t % rb);
+    S A[4][16];   // the small-problem kernel is dispatched for L <= 4 only
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (m >= L) break;
+      const int64_t I = s.d[m];
+      const S* Fm = F + s.off[m];
+      const S* qm = q + s.off[m];
+      S acc[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[e] = S(0);
+      for (int64_t i = 0; i < I; ++i) {
+        S qa[4], fb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          qa[a] = c0 + a < R ? qm[i * R + c0 + a] : S(0);
+          fb[a] = r0 + a < R ? Fm[i * R + r0 + a] : S(0);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a * 4 + b] += qa[a] * fb[b];
+      }
+#pragma unroll
+      for (int e = 0; e < 16; ++e) A[m][e] = acc[e];
+    }
+    for (int l = 0; l < L; ++l) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int c = c0 + a, r = r0 + b;
+          if (c < R && r < R) {
+            S acc = S(0);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+              if (m < L && m != l) acc += P2[((int64_t)l * L + m) * RR + (int64_t)c * R + r] * A[m][a * 4 + b];
+            tmp[l * RR + (int64_t)c * R + r] = acc;
+          }
+        }
+    }
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(SMALL_NT) void k_cg_small(
+    Shape s, TileMap tm, const S* __restrict__ Fg, const S* __restrict__ P1g,
+    const S* __restrict__ P2, const S* __restrict__ gamma, double damp,
+    const S* __restrict__ grad, int maxiter, double tol, S* __restrict__ y, S* __restrict__ jtjg,
+    double* __restrict__ scal, S* gws, unsigned lds_mask) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[SMALL_NT / 64];
+  const int L = s.L, R = s.R;
+  const int64_t n = s.n;
+  const int64_t LRR = (int64_t)L * R * R;
+  // carve q, F, tmp, P1 into LDS (bit set) or global workspace
+  S* lp = (S*)smem;
+  S* gp = gws;
+  auto carve = [&](int bit, int64_t len) -> S* {
+    S* r;
+    if ((lds_mask >> bit) & 1u) {
+      r = lp;
+      lp += len;
+    } else {
+      r = gp;
+      gp += len;
+    }
+    return r;
+  };
+  S* q = carve(0, n);
+  S* F = carve(1, n);
+  S* tmp = carve(2, LRR);
+  S* P1 = carve(3, LRR);
+  const int t = threadIdx.x;
+  // stage F (row-major per mode) and P1
+  for (int64_t e = t; e < n; e += blockDim.x) {
+    int64_t w;
+    const int l = mode_of(s, e, &w);
+    const int64_t I = s.d[l];
+    const int r = (int)(w / I);
+    const int64_t i = w % I;
+    F[s.off[l] + i * R + r] = Fg[e];
+    q[s.off[l] + i * R + r] = grad[e];   // J^T J grad first (no preconditioner)
+  }
+  for (int64_t e = t; e < LRR; e += blockDim.x) P1[e] = P1g[e];
+  __syncthreads();
+
+  const bool own = t < tm.ntile;
+  int tl = 0, ti0 = 0, tr0 = 0;
+  if (own) tile_of(s, tm, t, &tl, &ti0, &tr0);
+  const int64_t I = s.d[tl];
+  const S dampS = (S)damp;
+  S Mv[4], Gv[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const bool ok = own && tr0 + b < R;
+    Gv[b] = ok ? gamma[tl * R + tr0 + b] : S(0);
+    Mv[b] = ok ? inv_m(Gv[b], dampS) : S(0);
+  }
+  auto valid = [&](int a, int b) { return own && ti0 + a < I && tr0 + b < R; };
+  auto flat = [&](int a, int b) { return s.off[tl] + (int64_t)(tr0 + b) * I + ti0 + a; };
+
+  // J^T J grad (dogleg) -> jtjg
+  tmp_phase<S>(s, F, q, P2, tmp);
+  __syncthreads();
+  S z[16];
+  if (own) {
+    z_tile<S>(s, tl, ti0, tr0, F, q, tmp, P1, z);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (valid(a, b)) jtjg[flat(a, b)] = z[a * 4 + b];
+  }
+  if (n > 100) {
+    if ((int64_t)maxiter > n) maxiter = (int)n;
+  } else {
+    tol = 0.0;
+  }
+  S rv[16], pv[16], yc[16];
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int e = a * 4 + b;
+      const S g = valid(a, b) ? grad[flat(a, b)] : S(0);
+      rv[e] = Mv[b] * g;
+      pv[e] = rv[e];
+      yc[e] = S(0);
+      acc += (double)rv[e] * (double)rv[e];
+    }
+  double rn = block_sum(acc, red);
+  if (rn == 0.0) rn = 1e-6;
+  int itn;
+  for (itn = 0; itn < maxiter; ++itn) {
+    __syncthreads();  // previous readers of q / tmp are done
+    if (own) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (valid(a, b)) q[s.off[tl] + (int64_t)(ti0 + a) * R + tr0 + b] = Mv[b] * pv[a * 4 + b];
+    }
+    __syncthreads();
+    tmp_phase<S>(s, F, q, P2, tmp);
+    __syncthreads();
+    acc = 0.0;
+    if (own) {
+      z_tile<S>(s, tl, ti0, tr0, F, q, tmp, P1, z);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int e = a * 4 + b;
+          const S qe = Mv[b] * pv[e];
+          z[e] = valid(a, b) ? Mv[b] * (z[e] + dampS * Gv[b] * qe) : S(0);
+          acc += (double)pv[e] * (double)z[e];
+        }
+    }
+    double den = block_sum(acc, red);
+    if (den == 0.0) den = 1e-6;
+    const double alpha = rn / den;
+    const S aS = (S)alpha;
+    acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      yc[e] += aS * pv[e];
+      rv[e] -= aS * z[e];
+      acc += (double)rv[e] * (double)rv[e];
+    }
+    const double rn_new = block_sum(acc, red);
+    const double beta = rn_new / rn;
+    rn = rn_new;
+    const S bS = (S)beta;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) pv[e] = rv[e] + bS * (itn == 0 ? rv[e] : pv[e]);
+    if (rn <= tol) break;
+  }
+  if (itn == maxiter) itn = maxiter - 1;
+  if (own) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (valid(a, b)) y[flat(a, b)] = Mv[b] * yc[a * 4 + b];
+  }
+  if (t == 0) {
+    scal[0] = (double)(itn + 1);
+    scal[1] = rn;
+  }
+}
+
 // x_new = x + alpha y; F = x2cpd(x_new) (+ equalize) then transform; gain-ratio scalars.
 template <typename S>
 __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restrict__ x,
@@ -382,6 +642,41 @@ int cg_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, con
          double damp, const S* grad, int maxiter, double tol, S* y, S* jtjg, double* scal,
          hipStream_t st) {
   const int64_t LRR = (int64_t)s.L * s.R * s.R;
+  // small problems: register-tiled single-workgroup kernel
+  TileMap tm;
+  tm.rb = (s.R + 3) / 4;
+  int nt = 0;
+  for (int l = 0; l < s.L; ++l) {
+    tm.first[l] = nt;
+    nt += (int)((s.d[l] + 3) / 4) * tm.rb;
+  }
+  tm.first[s.L] = nt;
+  tm.ntile = nt;
+  if (nt <= SMALL_NT && s.L <= 4) {
+    const int64_t lens[4] = {s.n, s.n, LRR, LRR};
+    unsigned mask = 0;
+    size_t lds = 0, gbytes = 0;
+    for (int b = 0; b < 4; ++b) {
+      const size_t bytes = sizeof(S) * (size_t)lens[b];
+      if (lds + bytes <= 150 * 1024) {
+        mask |= 1u << b;
+        lds += bytes;
+      } else {
+        gbytes += bytes;
+      }
+    }
+    void* wsp = nullptr;
+    if (gbytes) {
+      int rc = tfx_workspace(ctx, gbytes + 256, st, &wsp);
+      if (rc) return rc;
+    }
+    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_small<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(lds > 0 ? lds : 1)));
+    hipLaunchKernelGGL(k_cg_small<S>, dim3(1), dim3(SMALL_NT), lds, st, s, tm, F, P1, P2, gamma,
+                       damp, grad, maxiter, tol, y, jtjg, scal, (S*)wsp, mask);
+    TFX_CHECK_LAUNCH(ctx);
+    return TFX_OK;
+  }
   void* wsp;
   int rc = tfx_workspace(ctx, sizeof(S) * (5 * s.n + 2 * LRR) + 256, st, &wsp);
   if (rc) return rc;

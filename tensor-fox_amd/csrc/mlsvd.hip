@@ -153,20 +153,28 @@ __device__ void wg_symm_mul(const double* G, int n, const double* B, int p, doub
   __syncthreads();
 }
 
-// Modified Gram-Schmidt (twice) on the p columns of Z (n x p col-major), in place.  Columns
-// that vanish are replaced by a unit vector orthogonal to the previous ones when possible.
-__device__ void wg_orthonormalize(double* Z, int n, int p, double* red) {
+// Classical Gram-Schmidt applied twice (CGS2) on the p columns of Z (n x p col-major), in place.
+// For column j the j inner products with the previous columns are computed one wavefront each
+// (no workgroup barrier per product), then subtracted in one pass.  `dots` holds >= p doubles.
+__device__ void wg_orthonormalize(double* Z, int n, int p, double* red, double* dots) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   for (int j = 0; j < p; ++j) {
     double* zj = Z + (int64_t)j * n;
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int i = 0; i < j; ++i) {
+    for (int pass = 0; pass < 2 && j > 0; ++pass) {
+      for (int i = wave; i < j; i += nwave) {
         const double* zi = Z + (int64_t)i * n;
         double a = 0.0;
-        for (int e = threadIdx.x; e < n; e += blockDim.x) a += zi[e] * zj[e];
-        a = block_sum(a, red);
-        for (int e = threadIdx.x; e < n; e += blockDim.x) zj[e] -= a * zi[e];
-        __syncthreads();
+        for (int e = lane; e < n; e += 64) a += zi[e] * zj[e];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if (lane == 0) dots[i] = a;
       }
+      __syncthreads();
+      for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        double v = zj[e];
+        for (int i = 0; i < j; ++i) v -= dots[i] * Z[(int64_t)i * n + e];
+        zj[e] = v;
+      }
+      __syncthreads();
     }
     double a = 0.0;
     for (int e = threadIdx.x; e < n; e += blockDim.x) a += zj[e] * zj[e];
@@ -255,98 +263,83 @@ __device__ void wg_jacobi(double* H, int m, double* V, double* cs, double* red) 
   }
 }
 
-// ws layout (doubles): Q n*p | Z n*p | H p*p | V p*p | cs 4*p | order p
-__global__ __launch_bounds__(1024) void k_eig_top(const double* __restrict__ G, int n,
-                                                   const double* __restrict__ Q0, int p, int k,
-                                                   int iters, double* __restrict__ U,
-                                                   double* __restrict__ lam, double* ws) {
-  __shared__ double red[16];
-  const bool direct = n <= p;
-  const int m = direct ? n : p;
-  // Q and Z exist only for the subspace iteration; the direct solve starts at H
-  const int64_t qz = direct ? 0 : (int64_t)n * p;
-  double* Q = ws;
-  double* Z = Q + qz;
-  double* H = Z + qz;
-  double* V = H + (int64_t)m * m;
-  double* cs = V + (int64_t)m * m;
-  int* order = (int*)(cs + 4 * (m + 2));
-  if (direct) {
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) H[e] = G[e];
-    __syncthreads();
-  } else {
-    for (int e = threadIdx.x; e < n * p; e += blockDim.x) Q[e] = Q0[e];
-    __syncthreads();
-    wg_orthonormalize(Q, n, p, red);
-    for (int it = 0; it < iters; ++it) {
-      wg_symm_mul(G, n, Q, p, Z);
-      for (int e = threadIdx.x; e < n * p; e += blockDim.x) Q[e] = Z[e];
-      __syncthreads();
-      wg_orthonormalize(Q, n, p, red);
+// Orthonormalize the p columns of Z (n x p, col-major, global) in place with CGS2.
+__global__ __launch_bounds__(256) void k_orth(double* Z, int n, int p) {
+  __shared__ double red[4];
+  __shared__ double dots[512];
+  wg_orthonormalize(Z, n, p, red, dots);
+}
+
+// Jacobi eigensolver of the symmetric m x m matrix H (row-major, global, destroyed), one
+// workgroup with H and V staged in LDS when they fit.  Writes the eigenvectors of the k largest
+// eigenvalues, in descending order, to Vk (m x k col-major) and the eigenvalues to lam.
+__global__ __launch_bounds__(256) void k_jacobi_top(double* H, int m, int k, double* Vg,
+                                                     double* Vk, double* lam, int in_lds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[4];
+  __shared__ double cs[4 * 260];
+  __shared__ int order[512];
+  double* Hs = in_lds ? (double*)smem : H;
+  double* Vs = in_lds ? (double*)smem + m * m : Vg;
+  // symmetrized copy (the Rayleigh-Ritz product is symmetric only up to rounding)
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int i = e / m, j = e % m;
+    if (i <= j) {
+      const double a = i == j ? H[e] : 0.5 * (H[i * m + j] + H[j * m + i]);
+      Hs[i * m + j] = a;
+      Hs[j * m + i] = a;
     }
-    wg_symm_mul(G, n, Q, p, Z);
-    // H = Q^T Z (symmetrized)
-    for (int e = threadIdx.x; e < p * p; e += blockDim.x) {
-      const int i = e / p, j = e % p;
-      double acc = 0.0;
-      for (int r = 0; r < n; ++r) acc += Q[(int64_t)i * n + r] * Z[(int64_t)j * n + r];
-      H[e] = acc;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < p * p; e += blockDim.x) {
-      const int i = e / p, j = e % p;
-      if (i < j) {
-        const double a = 0.5 * (H[i * p + j] + H[j * p + i]);
-        H[i * p + j] = a;
-        H[j * p + i] = a;
-      }
-    }
-    __syncthreads();
   }
-  wg_jacobi(H, m, V, cs, red);
-  // order by descending eigenvalue (selection by rank; ties broken by index)
+  __syncthreads();
+  wg_jacobi(Hs, m, Vs, cs, red);
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const double di = H[i * m + i];
+    const double di = Hs[i * m + i];
     int rank = 0;
     for (int j = 0; j < m; ++j) {
-      const double dj = H[j * m + j];
+      const double dj = Hs[j * m + j];
       if (dj > di || (dj == di && j < i)) ++rank;
     }
     order[rank] = i;
   }
   __syncthreads();
-  // U[:, c] = (direct ? V : Q V)[:, order[c]] with the largest-|.| entry made positive
-  for (int c = 0; c < k; ++c) {
-    const int src = order[c];
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-      double u;
-      if (direct) {
-        u = V[(int64_t)src * m + r];
-      } else {
-        u = 0.0;
-        for (int j = 0; j < p; ++j) u += Q[(int64_t)j * n + r] * V[(int64_t)src * m + j];
-      }
-      U[(int64_t)c * n + r] = u;
+  for (int e = threadIdx.x; e < m * k; e += blockDim.x) {
+    const int r = e % m, c = e / m;
+    Vk[e] = Vs[order[c] * m + r];
+  }
+  for (int c = threadIdx.x; c < k; c += blockDim.x) lam[c] = Hs[order[c] * m + order[c]];
+}
+
+// Make the largest-|.| entry of every column of U (n x k col-major) positive; one workgroup per
+// column.
+__global__ __launch_bounds__(256) void k_sign_fix(double* U, int n) {
+  __shared__ double bv[256];
+  __shared__ int bi[256];
+  double* u = U + (int64_t)blockIdx.x * n;
+  double best = -1.0;
+  int idx = 0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double a = fabs(u[r]);
+    if (a > best) {
+      best = a;
+      idx = r;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int best = 0;
-      double bv = -1.0;
-      for (int r = 0; r < n; ++r) {
-        const double a = fabs(U[(int64_t)c * n + r]);
-        if (a > bv) {
-          bv = a;
-          best = r;
-        }
+  }
+  bv[threadIdx.x] = best;
+  bi[threadIdx.x] = idx;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double b2 = bv[threadIdx.x + o];
+      const int i2 = bi[threadIdx.x + o];
+      if (b2 > bv[threadIdx.x] || (b2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {
+        bv[threadIdx.x] = b2;
+        bi[threadIdx.x] = i2;
       }
-      red[0] = U[(int64_t)c * n + best] < 0 ? -1.0 : 1.0;
-      lam[c] = H[src * m + src];
     }
-    __syncthreads();
-    const double sg = red[0];
-    for (int r = threadIdx.x; r < n; r += blockDim.x) U[(int64_t)c * n + r] *= sg;
     __syncthreads();
   }
+  const double sg = u[bi[0]] < 0 ? -1.0 : 1.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) u[r] *= sg;
 }
 
 // C (m x n col-major) = op(A) B; A FP64 col-major (m x kk, or kk x m when transA), B col-major
@@ -428,14 +421,56 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   if (n < 1 || k < 1 || k > n) return tfx_set_error(ctx, TFX_E_ARG, "bad sizes n=%d k=%d", n, k);
   const bool direct = n <= p;
   if (!direct && (!Q0 || p < k)) return tfx_set_error(ctx, TFX_E_ARG, "start block needs p >= k columns");
-  if (n > 4096 || p > 512) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "eigensolver limited to n <= 4096, p <= 512");
+  if (n > 8192 || p > 512) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "eigensolver limited to n <= 8192, p <= 512");
   const int m = direct ? n : p;
-  const size_t dbl = (size_t)2 * n * (direct ? 0 : p) + 2 * (size_t)m * m + 4 * (m + 2) + m + 8;
+  if (m > 512) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "Rayleigh-Ritz size %d > 512", m);
+  // workspace (doubles): Q n*p | Z n*p | H m*m | V m*m | Vk m*k
+  const size_t qz = direct ? 0 : (size_t)n * p;
+  const size_t dbl = 2 * qz + 2 * (size_t)m * m + (size_t)m * k + 64;
   void* wsp;
-  int rc = tfx_workspace(ctx, sizeof(double) * dbl + 256, stream, &wsp);
+  int rc = tfx_workspace(ctx, sizeof(double) * dbl, stream, &wsp);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_eig_top, dim3(1), dim3(1024), 0, stream, G, n, Q0, p, k, iters, U, lam,
-                     (double*)wsp);
+  double* Q = (double*)wsp;
+  double* Z = Q + qz;
+  double* H = Z + qz;
+  double* V = H + (size_t)m * m;
+  double* Vk = V + (size_t)m * m;
+  auto gemm = [&](int transA, int mm_, int nn_, int kk_, const double* A_, const double* B_, double* C_) {
+    const int64_t tot = (int64_t)mm_ * nn_;
+    hipLaunchKernelGGL(k_gemm_small<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
+                       transA, mm_, nn_, kk_, A_, B_, C_);
+  };
+  if (direct) {
+    TFX_HIP(ctx, hipMemcpyAsync(H, G, sizeof(double) * n * n, hipMemcpyDeviceToDevice, stream));
+  } else {
+    // subspace iteration: Q <- orth(G Q), `iters` times, from Q0
+    TFX_HIP(ctx, hipMemcpyAsync(Q, Q0, sizeof(double) * n * p, hipMemcpyDeviceToDevice, stream));
+    for (int it = 0; it < iters; ++it) {
+      gemm(0, n, p, n, G, Q, Z);
+      hipLaunchKernelGGL(k_orth, dim3(1), dim3(256), 0, stream, Z, n, p);
+      double* t = Q;
+      Q = Z;
+      Z = t;
+    }
+    if (iters == 0) hipLaunchKernelGGL(k_orth, dim3(1), dim3(256), 0, stream, Q, n, p);
+    // Rayleigh-Ritz: H = Q^T (G Q)
+    gemm(0, n, p, n, G, Q, Z);
+    gemm(1, p, p, n, Q, Z, H);
+  }
+  TFX_CHECK_LAUNCH(ctx);
+  const size_t lds = sizeof(double) * 2 * (size_t)m * m;
+  const int in_lds = lds <= 140 * 1024;
+  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_jacobi_top, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(in_lds ? lds : 1)));
+  hipLaunchKernelGGL(k_jacobi_top, dim3(1), dim3(256), in_lds ? lds : 0, stream, H, m, k, V, Vk, lam, in_lds);
+  TFX_CHECK_LAUNCH(ctx);
+  if (direct) {
+    TFX_HIP(ctx, hipMemcpyAsync(U, Vk, sizeof(double) * n * k, hipMemcpyDeviceToDevice, stream));
+  } else {
+    gemm(0, n, k, p, Q, Vk, U);
+    TFX_CHECK_LAUNCH(ctx);
+  }
+  hipLaunchKernelGGL(k_sign_fix, dim3(k), dim3(256), 0, stream, U, n);
   TFX_CHECK_LAUNCH(ctx);
   return TFX_OK;
 }
