@@ -183,7 +183,10 @@ def main():
                           'kernel': 'tfx_mttkrp (all modes, FP32)', 'avg_launch_s': kt}
 
     if not args.no_fp64:
-        t64, f64, out64 = time_cpd(tfx, Tdev, R, 'fp64', args.seed)
+        T64 = Tdev.to(torch.float64)          # the FP64 form's input, resident before timing
+        time_cpd(tfx, T64, R, 'fp64', args.seed)
+        t64, f64, out64 = time_cpd(tfx, T64, R, 'fp64', args.seed)
+        del T64
         log('fp64: %.3f s rel_error %.6e timings %s' % (t64, out64.rel_error, out64.timings))
         result['fp64_seconds'] = round(t64, 5)
         result['fp64_rel_error'] = out64.rel_error
@@ -197,7 +200,7 @@ def main():
         res = {'workload': 'same, first factor columns x 10^U(-1,1), seed 0', 'mixed_seconds': round(tm, 5),
                'mixed_rel_error': om.rel_error}
         if not args.no_fp64:
-            t6, _, o6 = time_cpd(tfx, Tcd, R, 'fp64', 0)
+            t6, _, o6 = time_cpd(tfx, Tcd.to(torch.float64), R, 'fp64', 0)
             res.update(fp64_seconds=round(t6, 5), fp64_rel_error=o6.rel_error)
         log('colscale instance:', res)
         result['colscale_instance'] = res

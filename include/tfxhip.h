@@ -59,6 +59,12 @@ int tfx_mttkrp(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dim
                const void* F_flat, int R, unsigned mode_mask, void* out_flat, int prec,
                hipStream_t stream);
 
+/* As tfx_mttkrp with the tensor in t_dtype and factors / outputs in f_dtype (FP32 tensor with FP64
+ * factors is the mixed-precision refinement; TFX_PREC_SPLIT16 requires exactly that pair). */
+int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L, const int64_t* dims,
+                     const void* F_flat, int f_dtype, int R, unsigned mode_mask, void* out_flat,
+                     int prec, hipStream_t stream);
+
 /* ||T - [[F]]||_F^2, accumulated in FP64, written to *out_sqnorm (device).
  * Replaces cpd2unfold1 + fastnorm (Conversion.py:117-141, Critical.py:14-22). */
 int tfx_cpd_residual(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
@@ -135,6 +141,11 @@ int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q0, int p, i
  * (MultilinearAlgebra.py:72-105). */
 int tfx_ttm(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims, int mode,
             const double* U, int k, void* Y, double* sig2, int prec, hipStream_t stream);
+
+/* As tfx_ttm with X in x_dtype and Y in y_dtype (FP32 tensor -> FP64 partial core). */
+int tfx_ttm_mixed(tfx_ctx* ctx, const void* X, int x_dtype, int L, const int64_t* dims, int mode,
+                  const double* U, int k, void* Y, int y_dtype, double* sig2, int prec,
+                  hipStream_t stream);
 
 /* C (m x n, column-major, dtype) = op(A) * B with A (FP64, column-major, m x kk or kk x m when
  * transA) and B (dtype, column-major kk x n).  Used for the uncompression U_l W_l

@@ -492,7 +492,13 @@ def cg(T, factors, damp, maxiter, tol):
     """Preconditioned CG on (J^T J + damp Gamma) y = -grad f, returning M*y.
     Follows GaussNewton.py:315-359 and cg_iterations GaussNewton.py:362-398, including the
     aliasing of P and residual_cg in the first iteration."""
-    L = len(factors)
+    _, P1, _ = gramians(factors)
+    grad = -compute_grad(T, factors, P1)
+    return cg_from_grad(factors, grad, damp, maxiter, tol)
+
+
+def cg_from_grad(factors, grad, damp, maxiter, tol):
+    """The body of cg (GaussNewton.py:324-359) for a given right-hand side grad."""
     R = factors[0].shape[1]
     dims = [f.shape[0] for f in factors]
     n = R * sum(dims)
@@ -504,7 +510,6 @@ def cg(T, factors, damp, maxiter, tol):
     Gamma, gamma = regularization(P1, dims)
     M = precond(Gamma, gamma, damp, dims)
     y = np.zeros(n)
-    grad = -compute_grad(T, factors, P1)
     JT_J_grad = _matvec_P(factors, P1, P2, grad)
     r = M * grad
     P = r                      # alias, as in the reference

@@ -24,6 +24,8 @@ SIGNATURES = {
     'tfx_ctx_destroy': [_c_vp],
     'tfx_device_info': [_c_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)],
     'tfx_mttkrp': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_uint, _c_vp, _c_int, _c_vp],
+    'tfx_mttkrp_mixed': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_int, _c_uint, _c_vp, _c_int, _c_vp],
+    'tfx_ttm_mixed': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp],
     'tfx_cpd_residual': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_vp, _c_vp],
     'tfx_cpd_residual_mixed': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_int, _c_vp, _c_vp],
     'tfx_tensor_stats': [_c_vp, _c_vp, _c_int, _c_i64, _c_vp, _c_vp],

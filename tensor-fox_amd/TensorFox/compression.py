@@ -79,7 +79,7 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
             U, lam = _top_eigvecs(dev, S, S_dims, l, n_comp, prec, draw)
             new_dims = list(S_dims)
             new_dims[l] = n_comp
-            Snew = dev.empty(int(np.prod(new_dims)), S.dtype)
+            Snew = dev.empty(int(np.prod(new_dims)), wdt)
             sig2 = dev.empty(n_comp, torch.float64)
             dev.ttm(S, S_dims, l, U, n_comp, Snew, sig2, prec=prec)
             sigma = np.sqrt(np.maximum(sig2.cpu().numpy(), 0.0))
@@ -87,7 +87,7 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
             if dim < n_comp:
                 U = U[:dim].contiguous()
                 new_dims[l] = dim
-                Snew = dev.empty(int(np.prod(new_dims)), S.dtype)
+                Snew = dev.empty(int(np.prod(new_dims)), wdt)
                 dev.ttm(S, S_dims, l, U, dim, Snew, None, prec=prec)
                 sigma = sigma[:dim]
             U_list.append(U)
@@ -110,7 +110,7 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
             k = U_list[l].shape[0]
             new_dims = list(S_dims)
             new_dims[l] = k
-            Snew = dev.empty(int(np.prod(new_dims)), S.dtype)
+            Snew = dev.empty(int(np.prod(new_dims)), wdt)
             dev.ttm(S, S_dims, l, U_list[l], k, Snew, None, prec=prec)
             S, S_dims = Snew, new_dims
     else:

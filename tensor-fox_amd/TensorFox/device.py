@@ -62,8 +62,8 @@ class Device:
         L = len(dims)
         if mask is None:
             mask = (1 << L) - 1
-        self._call('tfx_mttkrp', _p(T), dtype_code(T), L, _lib.dims_arg(dims), _p(F), int(R),
-                   ctypes.c_uint(mask), _p(out), int(prec), self.stream())
+        self._call('tfx_mttkrp_mixed', _p(T), dtype_code(T), L, _lib.dims_arg(dims), _p(F),
+                   dtype_code(F), int(R), ctypes.c_uint(mask), _p(out), int(prec), self.stream())
 
     def residual_sq(self, T, dims, F, R, out):
         """out (float64 device scalar) = ||T - [[F]]||^2."""
@@ -110,8 +110,8 @@ class Device:
                    self.stream())
 
     def ttm(self, X, dims, mode, U, k, Y, sig2=None, prec=_lib.TFX_PREC_NATIVE):
-        self._call('tfx_ttm', _p(X), dtype_code(X), len(dims), _lib.dims_arg(dims), int(mode), _p(U),
-                   int(k), _p(Y), _p(sig2), int(prec), self.stream())
+        self._call('tfx_ttm_mixed', _p(X), dtype_code(X), len(dims), _lib.dims_arg(dims), int(mode),
+                   _p(U), int(k), _p(Y), dtype_code(Y), _p(sig2), int(prec), self.stream())
 
     def gemm_small(self, transA, m, n, kk, A, B, C):
         self._call('tfx_gemm_small', dtype_code(B), int(bool(transA)), int(m), int(n), int(kk), _p(A),

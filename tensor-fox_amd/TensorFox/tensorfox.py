@@ -3,10 +3,11 @@
 refinement on the full tensor -> final error -> original mode order.
 
 Precision forms (option `precision`):
-  'fp64'  -- every array and product in FP64 (the default for float64 inputs);
-  'mixed' -- tensor and factors stored FP32, products FP32 on the device, eigensolve, CG inner
-             products / norms and the final relative error in FP64 (the default for float32
-             inputs).
+  'fp64'  -- the tensor is stored in FP64 and every product is FP64 (default for float64 input);
+  'mixed' -- the tensor is stored in FP32 (half the HBM traffic of every pass over it); the
+             passes over it (Gram, mode product, MTTKRP, residual) read FP32 and accumulate in
+             FP64, with the split 16-bit MFMA form where it is built; the latency-bound stages
+             (core dGN, CG, eigensolve, factors) run in FP64 (default for float32 input).
 """
 
 import sys
@@ -111,11 +112,12 @@ def tricpd(T, R, options):
     timings = {}
     dev = get_device(options.device)
     precision = _precision_of(T, options)
-    wdt = torch.float32 if precision == 'mixed' else torch.float64
+    wdt = torch.float64                                   # factors, CG, core
+    tdt = torch.float32 if precision == 'mixed' else torch.float64   # the tensor in HBM
     prec = _lib.TFX_PREC_NATIVE
     L = T.ndim
     dims, ordering = sort_dims(T.shape)
-    Tdev = _to_device(dev, T, ordering, wdt)
+    Tdev = _to_device(dev, T, ordering, tdt)
     N = int(np.prod(dims))
     stats = dev.zeros(2, torch.float64)
     dev.tensor_stats(Tdev, stats)

@@ -96,9 +96,9 @@ __global__ __launch_bounds__(256) void k_gram_mode(const S* __restrict__ X, PIQ 
 }
 
 // Y[p, s, q] = sum_a U[a + s*I] X[p, a, q]
-template <typename S>
+template <typename S, typename SY>
 __global__ void k_ttm(const S* __restrict__ X, PIQ v, const double* __restrict__ U, int k,
-                      S* __restrict__ Y) {
+                      SY* __restrict__ Y) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t tot = v.P * k * v.Q;
   if (e >= tot) return;
@@ -117,7 +117,7 @@ __global__ void k_ttm(const S* __restrict__ X, PIQ v, const double* __restrict__
     }
   }
   acc += (double)sacc;
-  Y[e] = (S)acc;
+  Y[e] = (SY)acc;
 }
 
 // sig2[s] = sum_{p, q} Y[p, s, q]^2 (FP64)
@@ -201,7 +201,8 @@ __device__ void wg_jacobi(double* H, int m, double* V, double* cs, double* red) 
     }
     off = block_sum(off, red);
     dia = block_sum(dia, red);
-    if (off <= 1e-30 * dia || off == 0.0) break;
+    // converged when the off-diagonal norm is below 1e-13 of the diagonal's
+    if (off <= 1e-26 * dia || off == 0.0) break;
     for (int round = 0; round < mm - 1; ++round) {
       // pairs of this round: (round, mm-1) and ((round+k) % (mm-1), (round-k+mm-1) % (mm-1))
       for (int k = threadIdx.x; k < mm / 2; k += blockDim.x) {
@@ -263,9 +264,20 @@ __device__ void wg_jacobi(double* H, int m, double* V, double* cs, double* red) 
   }
 }
 
-// Orthonormalize the p columns of Z (n x p, col-major, global) in place with CGS2.
-__global__ __launch_bounds__(256) void k_orth(double* Z, int n, int p) {
+// Scale every column of Z (n x p col-major) to unit norm; one workgroup per column.
+__global__ __launch_bounds__(256) void k_colnorm(double* Z, int n) {
   __shared__ double red[4];
+  double* z = Z + (int64_t)blockIdx.x * n;
+  double a = 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) a += z[e] * z[e];
+  a = block_sum(a, red);
+  const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) z[e] *= inv;
+}
+
+// Orthonormalize the p columns of Z (n x p, col-major, global) in place with CGS2.
+__global__ __launch_bounds__(1024) void k_orth(double* Z, int n, int p) {
+  __shared__ double red[16];
   __shared__ double dots[512];
   wg_orthonormalize(Z, n, p, red, dots);
 }
@@ -377,16 +389,16 @@ int gram_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, double* G, hipStr
   return TFX_OK;
 }
 
-template <typename S>
-int ttm_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, const double* U, int k, S* Y,
+template <typename S, typename SY>
+int ttm_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, const double* U, int k, SY* Y,
           double* sig2, hipStream_t st) {
   const PIQ v = piq_of(s, mode);
   const int64_t tot = v.P * k * v.Q;
-  hipLaunchKernelGGL(k_ttm<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, X, v, U, k, Y);
+  hipLaunchKernelGGL((k_ttm<S, SY>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, X, v, U, k, Y);
   TFX_CHECK_LAUNCH(ctx);
   if (sig2) {
     const PIQ vy{v.P, k, v.Q};
-    hipLaunchKernelGGL(k_rownorm2<S>, dim3(k), dim3(256), 0, st, Y, vy, sig2);
+    hipLaunchKernelGGL(k_rownorm2<SY>, dim3(k), dim3(256), 0, st, Y, vy, sig2);
     TFX_CHECK_LAUNCH(ctx);
   }
   return TFX_OK;
@@ -397,7 +409,7 @@ int ttm_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, const double* U, i
 int tfx_gram_split16(tfx_ctx* ctx, const float* X, const Shape& s, int mode, double* G,
                      hipStream_t st);
 int tfx_ttm_split16(tfx_ctx* ctx, const float* X, const Shape& s, int mode, const double* U,
-                    int k, float* Y, double* sig2, hipStream_t st);
+                    int k, double* Y, double* sig2, hipStream_t st);
 
 extern "C" int tfx_gram_mode(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims,
                              int mode, double* G, int prec, hipStream_t stream) {
@@ -445,14 +457,16 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   } else {
     // subspace iteration: Q <- orth(G Q), `iters` times, from Q0
     TFX_HIP(ctx, hipMemcpyAsync(Q, Q0, sizeof(double) * n * p, hipMemcpyDeviceToDevice, stream));
+    // power steps without re-orthonormalization (as the reference's range finder does), each
+    // column rescaled to unit norm so that long chains cannot overflow, then one CGS2
     for (int it = 0; it < iters; ++it) {
       gemm(0, n, p, n, G, Q, Z);
-      hipLaunchKernelGGL(k_orth, dim3(1), dim3(256), 0, stream, Z, n, p);
+      hipLaunchKernelGGL(k_colnorm, dim3(p), dim3(256), 0, stream, Z, n);
       double* t = Q;
       Q = Z;
       Z = t;
     }
-    if (iters == 0) hipLaunchKernelGGL(k_orth, dim3(1), dim3(256), 0, stream, Q, n, p);
+    hipLaunchKernelGGL(k_orth, dim3(1), dim3(1024), 0, stream, Q, n, p);
     // Rayleigh-Ritz: H = Q^T (G Q)
     gemm(0, n, p, n, G, Q, Z);
     gemm(1, p, p, n, Q, Z, H);
@@ -475,21 +489,29 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   return TFX_OK;
 }
 
-extern "C" int tfx_ttm(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims,
-                       int mode, const double* U, int k, void* Y, double* sig2, int prec,
-                       hipStream_t stream) {
+extern "C" int tfx_ttm_mixed(tfx_ctx* ctx, const void* X, int x_dtype, int L, const int64_t* dims,
+                             int mode, const double* U, int k, void* Y, int y_dtype, double* sig2,
+                             int prec, hipStream_t stream) {
   if (!ctx || !X || !U || !Y) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
   Shape s;
   int rc = tfx_make_shape(ctx, L, dims, 1, &s);
   if (rc) return rc;
   if (mode < 0 || mode >= L || k < 1) return tfx_set_error(ctx, TFX_E_ARG, "bad mode/k");
   if (prec == TFX_PREC_SPLIT16) {
-    if (dtype != TFX_F32) return tfx_set_error(ctx, TFX_E_ARG, "split16 TTM needs FP32 data");
-    return tfx_ttm_split16(ctx, (const float*)X, s, mode, U, k, (float*)Y, sig2, stream);
+    if (x_dtype != TFX_F32 || y_dtype != TFX_F64)
+      return tfx_set_error(ctx, TFX_E_ARG, "split16 TTM needs FP32 input and FP64 output");
+    return tfx_ttm_split16(ctx, (const float*)X, s, mode, U, k, (double*)Y, sig2, stream);
   }
-  if (dtype == TFX_F32) return ttm_t<float>(ctx, (const float*)X, s, mode, U, k, (float*)Y, sig2, stream);
-  if (dtype == TFX_F64) return ttm_t<double>(ctx, (const double*)X, s, mode, U, k, (double*)Y, sig2, stream);
-  return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  if (x_dtype == TFX_F32 && y_dtype == TFX_F32) return ttm_t<float, float>(ctx, (const float*)X, s, mode, U, k, (float*)Y, sig2, stream);
+  if (x_dtype == TFX_F32 && y_dtype == TFX_F64) return ttm_t<float, double>(ctx, (const float*)X, s, mode, U, k, (double*)Y, sig2, stream);
+  if (x_dtype == TFX_F64 && y_dtype == TFX_F64) return ttm_t<double, double>(ctx, (const double*)X, s, mode, U, k, (double*)Y, sig2, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "unsupported dtype pair %d/%d", x_dtype, y_dtype);
+}
+
+extern "C" int tfx_ttm(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims,
+                       int mode, const double* U, int k, void* Y, double* sig2, int prec,
+                       hipStream_t stream) {
+  return tfx_ttm_mixed(ctx, X, dtype, L, dims, mode, U, k, Y, dtype, sig2, prec, stream);
 }
 
 extern "C" int tfx_gemm_small(tfx_ctx* ctx, int dtype, int transA, int m, int n, int kk,

@@ -29,8 +29,8 @@ __device__ inline void row_to_idx(const Shape& s, int64_t rho, int64_t* idx) {
 }
 
 // Pass 1: Y (NR x R, row-major, columns rc0..rc0+RC-1) and per-workgroup partials of N_{L-1}.
-template <typename S>
-__global__ __launch_bounds__(256) void k_mttkrp_rows(const S* __restrict__ T, Shape s,
+template <typename TS, typename S>
+__global__ __launch_bounds__(256) void k_mttkrp_rows(const TS* __restrict__ T, Shape s,
                                                       const S* __restrict__ F, int rc0,
                                                       int64_t rows_per_wg, S* __restrict__ Y,
                                                       S* __restrict__ part, int np_in_lds) {
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_rows(const S* __restrict__ T, Sh
       for (int e = t; e < RB * KC; e += 256) {
         const int rr = e / KC, kk = e % KC;
         S v = S(0);
-        if (rr < nrows && k0 + kk < K) v = T[(rho0 + rr) * K + k0 + kk];
+        if (rr < nrows && k0 + kk < K) v = (S)T[(rho0 + rr) * K + k0 + kk];
         Tt[rr * (KC + 1) + kk] = v;
       }
       for (int e = t; e < KC * RC; e += 256) {
@@ -264,8 +264,8 @@ __global__ void k_sum_parts(const double* __restrict__ part, int n, double* __re
   if (threadIdx.x == 0) *out = tot;
 }
 
-template <typename S>
-int mttkrp_native(tfx_ctx* ctx, const S* T, const Shape& s, const S* F, unsigned mask, S* out,
+template <typename TS, typename S>
+int mttkrp_native(tfx_ctx* ctx, const TS* T, const Shape& s, const S* F, unsigned mask, S* out,
                   hipStream_t stream) {
   constexpr int RC = RCOf<S>::v;
   const int L = s.L, R = s.R;
@@ -289,10 +289,10 @@ int mttkrp_native(tfx_ctx* ctx, const S* T, const Shape& s, const S* F, unsigned
   size_t lds_np = sizeof(S) * (size_t)K * RC;
   int np_in_lds = (lds_base + lds_np) <= 150 * 1024;
   size_t lds = lds_base + (np_in_lds ? lds_np : 0);
-  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_mttkrp_rows<S>,
+  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_mttkrp_rows<TS, S>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   for (int rc0 = 0; rc0 < R; rc0 += RC) {
-    hipLaunchKernelGGL(k_mttkrp_rows<S>, dim3(nwg), dim3(256), lds, stream, T, s, F, rc0,
+    hipLaunchKernelGGL((k_mttkrp_rows<TS, S>), dim3(nwg), dim3(256), lds, stream, T, s, F, rc0,
                        rows_per_wg, Y, part, np_in_lds);
     TFX_CHECK_LAUNCH(ctx);
     if ((mask >> (L - 1)) & 1u) {
@@ -393,29 +393,39 @@ __global__ void k_stats_final(const double* __restrict__ part, int nb, double* o
 
 }  // namespace
 
-int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const float* F,
-                       unsigned mask, float* out, hipStream_t stream);
+int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
+                       unsigned mask, double* out, hipStream_t stream);
 
-extern "C" int tfx_mttkrp(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
-                          const void* F_flat, int R, unsigned mode_mask, void* out_flat, int prec,
-                          hipStream_t stream) {
+extern "C" int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L,
+                                const int64_t* dims, const void* F_flat, int f_dtype, int R,
+                                unsigned mode_mask, void* out_flat, int prec, hipStream_t stream) {
   if (!ctx || !T || !F_flat || !out_flat) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
   Shape s;
   int rc = tfx_make_shape(ctx, L, dims, R, &s);
   if (rc) return rc;
   if (mode_mask == 0) return TFX_OK;
   if (prec == TFX_PREC_SPLIT16) {
-    if (dtype != TFX_F32) return tfx_set_error(ctx, TFX_E_ARG, "split16 MTTKRP needs FP32 data");
-    return tfx_mttkrp_split16(ctx, (const float*)T, s, (const float*)F_flat, mode_mask,
-                              (float*)out_flat, stream);
+    if (t_dtype != TFX_F32 || f_dtype != TFX_F64)
+      return tfx_set_error(ctx, TFX_E_ARG, "split16 MTTKRP needs an FP32 tensor and FP64 factors");
+    return tfx_mttkrp_split16(ctx, (const float*)T, s, (const double*)F_flat, mode_mask,
+                              (double*)out_flat, stream);
   }
-  if (dtype == TFX_F32)
-    return mttkrp_native<float>(ctx, (const float*)T, s, (const float*)F_flat, mode_mask,
-                                (float*)out_flat, stream);
-  if (dtype == TFX_F64)
-    return mttkrp_native<double>(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
-                                 (double*)out_flat, stream);
-  return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  if (t_dtype == TFX_F32 && f_dtype == TFX_F32)
+    return mttkrp_native<float, float>(ctx, (const float*)T, s, (const float*)F_flat, mode_mask,
+                                       (float*)out_flat, stream);
+  if (t_dtype == TFX_F32 && f_dtype == TFX_F64)
+    return mttkrp_native<float, double>(ctx, (const float*)T, s, (const double*)F_flat, mode_mask,
+                                        (double*)out_flat, stream);
+  if (t_dtype == TFX_F64 && f_dtype == TFX_F64)
+    return mttkrp_native<double, double>(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
+                                         (double*)out_flat, stream);
+  return tfx_set_error(ctx, TFX_E_ARG, "unsupported dtype pair %d/%d", t_dtype, f_dtype);
+}
+
+extern "C" int tfx_mttkrp(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
+                          const void* F_flat, int R, unsigned mode_mask, void* out_flat, int prec,
+                          hipStream_t stream) {
+  return tfx_mttkrp_mixed(ctx, T, dtype, L, dims, F_flat, dtype, R, mode_mask, out_flat, prec, stream);
 }
 
 extern "C" int tfx_cpd_residual(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,

@@ -3,7 +3,7 @@
 // the split16 precision fails loudly instead of silently using another path.
 #include "common.h"
 
-int tfx_mttkrp_split16(tfx_ctx* ctx, const float*, const Shape&, const float*, unsigned, float*,
+int tfx_mttkrp_split16(tfx_ctx* ctx, const float*, const Shape&, const double*, unsigned, double*,
                        hipStream_t) {
   return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "split16 MTTKRP not built");
 }
@@ -12,7 +12,7 @@ int tfx_gram_split16(tfx_ctx* ctx, const float*, const Shape&, int, double*, hip
   return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "split16 Gram not built");
 }
 
-int tfx_ttm_split16(tfx_ctx* ctx, const float*, const Shape&, int, const double*, int, float*,
+int tfx_ttm_split16(tfx_ctx* ctx, const float*, const Shape&, int, const double*, int, double*,
                     double*, hipStream_t) {
   return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "split16 TTM not built");
 }

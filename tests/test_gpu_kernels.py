@@ -180,6 +180,34 @@ def test_cg_matches_oracle(dev, dims, R, maxiter):
     assert _rel(y32, ry) < 1e-3
 
 
+@pytest.mark.parametrize('dims,R,maxiter,tol', [((800, 600, 400), 16, 6, 1e-16),     # multi-workgroup CG
+                                                ((800, 600, 400), 16, 40, 1e-3),      # early stop
+                                                ((300, 200, 100), 70, 4, 1e-16),      # R > 64: generic CG
+                                                ((120, 100, 90, 80), 12, 5, 1e-16)])
+def test_cg_large_matches_oracle(dev, dims, R, maxiter, tol):
+    """CG on large problems from a given right-hand side (the oracle's cg body), FP64."""
+    print('cg large', dims, R, flush=True)
+    rs = np.random.RandomState(13)
+    F = [rs.randn(d, R) / np.sqrt(d) for d in dims]
+    g = rs.randn(R * sum(dims))
+    ry, _, rj, ritn, rrn = o.cg_from_grad([f.copy() for f in F], g, 0.3, maxiter, tol)
+    dt = torch.float64
+    L = len(dims)
+    Fd = dev.upload(_flat(F), dt)
+    Gr, P1, P2 = dev.empty((L, R, R), dt), dev.empty((L, R, R), dt), dev.empty((L, L, R, R), dt)
+    gamma = dev.empty((L, R), dt)
+    dev.gramians(dims, R, Fd, Gr, P1, P2, gamma)
+    gd = dev.upload(g, dt)
+    y, jtjg = dev.empty(g.size, dt), dev.empty(g.size, dt)
+    scal = dev.zeros(2, torch.float64)
+    dev.cg(dims, R, Fd, P1, P2, gamma, 0.3, gd, maxiter, tol, y, jtjg, scal)
+    sc = scal.cpu().numpy()
+    assert int(sc[0]) == ritn
+    assert sc[1] == pytest.approx(rrn, rel=1e-8)
+    assert _rel(y.cpu().numpy(), ry) < 1e-9
+    assert _rel(jtjg.cpu().numpy(), rj) < 1e-11
+
+
 @pytest.mark.parametrize('dt', [torch.float64, torch.float32])
 @pytest.mark.parametrize('symm,fnorm', [(False, 0.0), (True, 0.0), (False, 2.5)])
 def test_dgn_update(dev, dt, symm, fnorm):
