@@ -300,7 +300,9 @@ __device__ inline void z_tile(const Shape& s, int l, int i0, int r0, const S* F,
   }
 }
 
-// tmp_l = sum_{m != l} P2[l][m] o (q_m^T F_m), computed per 4 x 4 (c, r) tile for all modes.
+// tmp_l = sum_{m != l} P2[l][m] o (q_m^T F_m), computed per 4 x 4 (c, r) tile: each mode's
+// product is formed once and added to the tiles of the other modes straight away (the tile is
+// owned by one thread, so the read-modify-write of tmp has no race).
 template <typename S>
 __device__ inline void tmp_phase(const Shape& s, const S* F, const S* q, const S* __restrict__ P2,
                                  S* tmp) {
@@ -309,10 +311,7 @@ __device__ inline void tmp_phase(const Shape& s, const S* F, const S* q, const S
   const int64_t RR = (int64_t)R * R;
   for (int t = threadIdx.x; t < rb * rb; t += blockDim.x) {
     const int c0 = 4 * (t / rb), r0 = 4 * (t % rb);
-    S A[4][16];   // the small-problem kernel is dispatched for L <= 4 only
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      if (m >= L) break;
+    for (int m = 0; m < L; ++m) {
       const int64_t I = s.d[m];
       const S* Fm = F + s.off[m];
       const S* qm = q + s.off[m];
@@ -331,23 +330,19 @@ __device__ inline void tmp_phase(const Shape& s, const S* F, const S* q, const S
 #pragma unroll
           for (int b = 0; b < 4; ++b) acc[a * 4 + b] += qa[a] * fb[b];
       }
+      for (int l = 0; l < L; ++l) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) A[m][e] = acc[e];
-    }
-    for (int l = 0; l < L; ++l) {
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int c = c0 + a, r = r0 + b;
-          if (c < R && r < R) {
-            S acc = S(0);
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-              if (m < L && m != l) acc += P2[((int64_t)l * L + m) * RR + (int64_t)c * R + r] * A[m][a * 4 + b];
-            tmp[l * RR + (int64_t)c * R + r] = acc;
+          for (int b = 0; b < 4; ++b) {
+            const int c = c0 + a, r = r0 + b;
+            if (c < R && r < R) {
+              const int64_t te = l * RR + (int64_t)c * R + r;
+              const S contrib = m == l ? S(0) : P2[((int64_t)l * L + m) * RR + (int64_t)c * R + r] * acc[a * 4 + b];
+              tmp[te] = (m == 0 ? S(0) : tmp[te]) + contrib;
+            }
           }
-        }
+      }
     }
   }
 }
@@ -381,6 +376,7 @@ __global__ __launch_bounds__(SMALL_NT) void k_cg_small(
   S* F = carve(1, n);
   S* tmp = carve(2, LRR);
   S* P1 = carve(3, LRR);
+  S* ycg = gp;            // CG iterate, flat layout, always in global memory (fewer registers)
   const int t = threadIdx.x;
   // stage F (row-major per mode) and P1
   for (int64_t e = t; e < n; e += blockDim.x) {
@@ -427,7 +423,7 @@ __global__ __launch_bounds__(SMALL_NT) void k_cg_small(
   } else {
     tol = 0.0;
   }
-  S rv[16], pv[16], yc[16];
+  S rv[16], pv[16];
   double acc = 0.0;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -437,7 +433,7 @@ __global__ __launch_bounds__(SMALL_NT) void k_cg_small(
       const S g = valid(a, b) ? grad[flat(a, b)] : S(0);
       rv[e] = Mv[b] * g;
       pv[e] = rv[e];
-      yc[e] = S(0);
+      if (valid(a, b)) ycg[flat(a, b)] = S(0);
       acc += (double)rv[e] * (double)rv[e];
     }
   double rn = block_sum(acc, red);
@@ -474,11 +470,14 @@ __global__ __launch_bounds__(SMALL_NT) void k_cg_small(
     const S aS = (S)alpha;
     acc = 0.0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      yc[e] += aS * pv[e];
-      rv[e] -= aS * z[e];
-      acc += (double)rv[e] * (double)rv[e];
-    }
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int e = a * 4 + b;
+        if (valid(a, b)) ycg[flat(a, b)] += aS * pv[e];
+        rv[e] -= aS * z[e];
+        acc += (double)rv[e] * (double)rv[e];
+      }
     const double rn_new = block_sum(acc, red);
     const double beta = rn_new / rn;
     rn = rn_new;
@@ -493,247 +492,11 @@ __global__ __launch_bounds__(SMALL_NT) void k_cg_small(
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-        if (valid(a, b)) y[flat(a, b)] = Mv[b] * yc[a * 4 + b];
+        if (valid(a, b)) y[flat(a, b)] = Mv[b] * ycg[flat(a, b)];
   }
   if (t == 0) {
     scal[0] = (double)(itn + 1);
     scal[1] = rn;
-  }
-}
-
-// ---------------------------------------------------------------------------------------------
-// CG for large problems (refinement on the full tensor): three launches per iteration over many
-// workgroups.  Iteration `it` uses p_it = r_it + beta_it * p_{it-1} formed on the fly
-// (beta_0 = 0; p_1 uses r_1 in place of p_0: the aliasing of GaussNewton.py:349-392).  Inner
-// products are summed in FP64 from per-workgroup partials, in a fixed order by every consumer.
-// A device flag stops later iterations once rn <= tol.
-// ---------------------------------------------------------------------------------------------
-constexpr int CGM_RC = 64;     // max R handled by the multi-workgroup CG
-constexpr int CGM_IB = 32;     // rows per z-phase workgroup
-constexpr int CGM_IC = 64;     // rows per A-phase chunk
-
-template <typename S>
-struct CGM {
-  S *r, *p, *yc, *z, *A;
-  double* den_part;   // [maxiter][nb]
-  double* rn_part;    // [maxiter + 1][nc]
-  int* ctl;           // [0] stop flag, [1] iterations done
-  int nb, nc;
-};
-
-__device__ inline double sum_parts(const double* part, int n) {
-  double s = 0.0;
-  for (int i = 0; i < n; ++i) s += part[i];
-  return s;
-}
-
-template <typename S>
-__device__ inline double rn_at(const CGM<S>& c, int it) {
-  double v = sum_parts(c.rn_part + (int64_t)it * c.nc, c.nc);
-  if (it == 0 && v == 0.0) v = 1e-6;
-  return v;
-}
-
-// true when iteration `it` must not run (the reference stopped after an earlier iteration)
-template <typename S>
-__device__ inline bool cgm_stopped(const CGM<S>& c, int it, double tol) {
-  if (c.ctl[0]) return true;
-  if (it >= 1 && rn_at(c, it) <= tol) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      c.ctl[0] = 1;
-      c.ctl[1] = it;
-    }
-    return true;
-  }
-  return false;
-}
-
-// p_it at flat element e
-template <typename S>
-__device__ inline S cgm_p(const CGM<S>& c, int64_t e, int it, S beta) {
-  const S pold = it == 0 ? S(0) : (it == 1 ? c.r[e] : c.p[e]);
-  return c.r[e] + beta * pold;
-}
-
-template <typename S>
-__device__ inline S cgm_q(const Shape& s, const CGM<S>& c, const S* gamma, S dampS, int64_t e,
-                          int it, S beta, bool jtj, const S* grad) {
-  if (jtj) return grad[e];
-  int64_t w;
-  const int l = mode_of(s, e, &w);
-  const S g = gamma[l * s.R + (int)(w / s.d[l])];
-  return inv_m(g, dampS) * cgm_p(c, e, it, beta);
-}
-
-template <typename S>
-__global__ __launch_bounds__(256) void k_cgm_init(Shape s, CGM<S> c, const S* __restrict__ gamma,
-                                                  double damp, const S* __restrict__ grad,
-                                                  int maxiter) {
-  __shared__ double red[4];
-  const S dampS = (S)damp;
-  double acc = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t w;
-    const int l = mode_of(s, e, &w);
-    const S g = gamma[l * s.R + (int)(w / s.d[l])];
-    const S r = inv_m(g, dampS) * grad[e];
-    c.r[e] = r;
-    c.p[e] = S(0);
-    c.yc[e] = S(0);
-    acc += (double)r * (double)r;
-  }
-  acc = block_sum(acc, red);
-  if (threadIdx.x == 0) c.rn_part[blockIdx.x] = acc;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c.ctl[0] = 0;
-    c.ctl[1] = maxiter;
-  }
-}
-
-// A_m[c][r] = sum_i q_m[i, c] F_m[i, r]; one workgroup per (m, 16 x 16 tile of (c, r)).
-template <typename S>
-__global__ __launch_bounds__(256) void k_cgm_A(Shape s, CGM<S> c, const S* __restrict__ F,
-                                               const S* __restrict__ gamma, double damp, int it,
-                                               double tol, int jtj, const S* __restrict__ grad) {
-  __shared__ S qs[CGM_IC][17];
-  __shared__ S fs[CGM_IC][17];
-  if (!jtj && cgm_stopped(c, it, tol)) return;
-  const int R = s.R;
-  const int tb = (R + 15) / 16;
-  const int m = blockIdx.x / (tb * tb);
-  const int tt = blockIdx.x % (tb * tb);
-  const int c0 = 16 * (tt / tb), r0 = 16 * (tt % tb);
-  const int64_t I = s.d[m];
-  const S dampS = (S)damp;
-  const S beta = (!jtj && it >= 1) ? (S)(rn_at(c, it) / rn_at(c, it - 1)) : S(0);
-  const int tc = threadIdx.x / 16, tr = threadIdx.x % 16;
-  S acc = S(0);
-  for (int64_t i0 = 0; i0 < I; i0 += CGM_IC) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < CGM_IC * 16; e += 256) {
-      const int col = e / CGM_IC, ii = e % CGM_IC;
-      const int64_t i = i0 + ii;
-      S qv = S(0), fv = S(0);
-      if (i < I) {
-        if (c0 + col < R) qv = cgm_q(s, c, gamma, dampS, s.off[m] + (int64_t)(c0 + col) * I + i, it, beta, jtj, grad);
-        if (r0 + col < R) fv = F[s.off[m] + (int64_t)(r0 + col) * I + i];
-      }
-      qs[ii][col] = qv;
-      fs[ii][col] = fv;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int ii = 0; ii < CGM_IC; ++ii) acc += qs[ii][tc] * fs[ii][tr];
-  }
-  if (c0 + tc < R && r0 + tr < R) c.A[((int64_t)m * R + c0 + tc) * R + r0 + tr] = acc;
-}
-
-// z = M (F tmp + q P1 + damp Gamma q) for CGM_IB rows of one mode and the partial p.z; with jtj
-// set, out = F tmp + q P1 with q = grad (no preconditioner, no damping).
-template <typename S>
-__global__ __launch_bounds__(256) void k_cgm_Z(Shape s, CGM<S> c, const S* __restrict__ F,
-                                               const S* __restrict__ P1, const S* __restrict__ P2,
-                                               const S* __restrict__ gamma, double damp, int it,
-                                               double tol, int jtj, const S* __restrict__ grad,
-                                               S* __restrict__ out) {
-  __shared__ S tmp[CGM_RC * CGM_RC];
-  __shared__ S p1[CGM_RC * CGM_RC];
-  __shared__ S fr[CGM_IB][CGM_RC + 1];
-  __shared__ S qr[CGM_IB][CGM_RC + 1];
-  __shared__ double red[4];
-  if (!jtj && cgm_stopped(c, it, tol)) return;
-  const int L = s.L, R = s.R;
-  const int64_t RR = (int64_t)R * R;
-  int b = blockIdx.x, l = 0;
-  while (l < L - 1 && b >= (int)((s.d[l] + CGM_IB - 1) / CGM_IB)) {
-    b -= (int)((s.d[l] + CGM_IB - 1) / CGM_IB);
-    ++l;
-  }
-  const int64_t I = s.d[l];
-  const int64_t i0 = (int64_t)b * CGM_IB;
-  const S dampS = (S)damp;
-  const S beta = (!jtj && it >= 1) ? (S)(rn_at(c, it) / rn_at(c, it - 1)) : S(0);
-  for (int e = threadIdx.x; e < R * R; e += 256) {
-    S acc = S(0);
-    for (int m = 0; m < L; ++m)
-      if (m != l) acc += P2[((int64_t)l * L + m) * RR + e] * c.A[m * RR + e];
-    tmp[e] = acc;
-    p1[e] = P1[l * RR + e];
-  }
-  for (int e = threadIdx.x; e < CGM_IB * R; e += 256) {
-    const int col = e / CGM_IB, ii = e % CGM_IB;
-    const int64_t i = i0 + ii;
-    S fv = S(0), qv = S(0);
-    if (i < I) {
-      const int64_t fe = s.off[l] + (int64_t)col * I + i;
-      fv = F[fe];
-      qv = cgm_q(s, c, gamma, dampS, fe, it, beta, jtj, grad);
-    }
-    fr[ii][col] = fv;
-    qr[ii][col] = qv;
-  }
-  __syncthreads();
-  double acc = 0.0;
-  for (int e = threadIdx.x; e < CGM_IB * R; e += 256) {
-    const int ii = e % CGM_IB, r = e / CGM_IB;
-    const int64_t i = i0 + ii;
-    if (i >= I) continue;
-    S zv = S(0);
-    for (int cc = 0; cc < R; ++cc) zv += fr[ii][cc] * tmp[cc * R + r] + qr[ii][cc] * p1[cc * R + r];
-    const int64_t fe = s.off[l] + (int64_t)r * I + i;
-    if (jtj) {
-      out[fe] = zv;
-    } else {
-      const S g = gamma[l * R + r];
-      zv = inv_m(g, dampS) * (zv + dampS * g * qr[ii][r]);
-      c.z[fe] = zv;
-      acc += (double)cgm_p(c, fe, it, beta) * (double)zv;
-    }
-  }
-  if (jtj) return;
-  acc = block_sum(acc, red);
-  if (threadIdx.x == 0) c.den_part[(int64_t)it * c.nb + blockIdx.x] = acc;
-}
-
-// yc += alpha p_it, r -= alpha z, p <- p_it, partial r.r of iteration it + 1.  p is rewritten
-// only here, after every reader of p_{it-1} (the two launches before) has finished.
-template <typename S>
-__global__ __launch_bounds__(256) void k_cgm_upd(Shape s, CGM<S> c, int it, double tol) {
-  __shared__ double red[4];
-  if (cgm_stopped(c, it, tol)) return;
-  double den = sum_parts(c.den_part + (int64_t)it * c.nb, c.nb);
-  if (den == 0.0) den = 1e-6;
-  const double rn = rn_at(c, it);
-  const S alpha = (S)(rn / den);
-  const S beta = it >= 1 ? (S)(rn / rn_at(c, it - 1)) : S(0);
-  double acc = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
-    const S pe = cgm_p(c, e, it, beta);
-    c.yc[e] += alpha * pe;
-    const S r = c.r[e] - alpha * c.z[e];
-    c.r[e] = r;
-    c.p[e] = pe;
-    acc += (double)r * (double)r;
-  }
-  acc = block_sum(acc, red);
-  if (threadIdx.x == 0) c.rn_part[(int64_t)(it + 1) * c.nc + blockIdx.x] = acc;
-}
-
-template <typename S>
-__global__ __launch_bounds__(256) void k_cgm_final(Shape s, CGM<S> c, const S* __restrict__ gamma,
-                                                   double damp, S* __restrict__ y,
-                                                   double* __restrict__ scal) {
-  const S dampS = (S)damp;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t w;
-    const int l = mode_of(s, e, &w);
-    const S g = gamma[l * s.R + (int)(w / s.d[l])];
-    y[e] = inv_m(g, dampS) * c.yc[e];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const int done = c.ctl[1];
-    scal[0] = (double)done;
-    scal[1] = rn_at(c, done);
   }
 }
 
@@ -901,11 +664,10 @@ int cg_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, con
         gbytes += bytes;
       }
     }
+    gbytes += sizeof(S) * (size_t)s.n;   // the CG iterate
     void* wsp = nullptr;
-    if (gbytes) {
-      int rc = tfx_workspace(ctx, gbytes + 256, st, &wsp);
-      if (rc) return rc;
-    }
+    int rc = tfx_workspace(ctx, gbytes + 256, st, &wsp);
+    if (rc) return rc;
     TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_small<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(lds > 0 ? lds : 1)));
     hipLaunchKernelGGL(k_cg_small<S>, dim3(1), dim3(SMALL_NT), lds, st, s, tm, F, P1, P2, gamma,
@@ -913,50 +675,9 @@ int cg_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, con
     TFX_CHECK_LAUNCH(ctx);
     return TFX_OK;
   }
-  if (s.R <= CGM_RC) {
-    if (s.n <= 100) tol = 0.0;
-    else if ((int64_t)maxiter > s.n) maxiter = (int)s.n;
-    const int tb = (s.R + 15) / 16;
-    const int nA = s.L * tb * tb;
-    int nb = 0;
-    for (int l = 0; l < s.L; ++l) nb += (int)((s.d[l] + CGM_IB - 1) / CGM_IB);
-    const int nc = (int)std::min<int64_t>((s.n + 255) / 256, 240);
-    const size_t vec = align256(sizeof(S) * s.n);
-    const size_t bytes = 4 * vec + align256(sizeof(S) * LRR) + align256(sizeof(double) * (size_t)maxiter * nb) +
-                         align256(sizeof(double) * (size_t)(maxiter + 1) * nc) + 256;
-    void* wsp;
-    int rc = tfx_workspace(ctx, bytes, st, &wsp);
-    if (rc) return rc;
-    char* w = (char*)wsp;
-    CGM<S> c;
-    c.r = (S*)w;
-    c.p = (S*)(w + vec);
-    c.yc = (S*)(w + 2 * vec);
-    c.z = (S*)(w + 3 * vec);
-    w += 4 * vec;
-    c.A = (S*)w;
-    w += align256(sizeof(S) * LRR);
-    c.den_part = (double*)w;
-    w += align256(sizeof(double) * (size_t)maxiter * nb);
-    c.rn_part = (double*)w;
-    w += align256(sizeof(double) * (size_t)(maxiter + 1) * nc);
-    c.ctl = (int*)w;
-    c.nb = nb;
-    c.nc = nc;
-    // J^T J grad for the dogleg fallback
-    hipLaunchKernelGGL(k_cgm_A<S>, dim3(nA), dim3(256), 0, st, s, c, F, gamma, damp, 0, tol, 1, grad);
-    hipLaunchKernelGGL(k_cgm_Z<S>, dim3(nb), dim3(256), 0, st, s, c, F, P1, P2, gamma, damp, 0, tol, 1, grad, jtjg);
-    hipLaunchKernelGGL(k_cgm_init<S>, dim3(nc), dim3(256), 0, st, s, c, gamma, damp, grad, maxiter);
-    TFX_CHECK_LAUNCH(ctx);
-    for (int it = 0; it < maxiter; ++it) {
-      hipLaunchKernelGGL(k_cgm_A<S>, dim3(nA), dim3(256), 0, st, s, c, F, gamma, damp, it, tol, 0, grad);
-      hipLaunchKernelGGL(k_cgm_Z<S>, dim3(nb), dim3(256), 0, st, s, c, F, P1, P2, gamma, damp, it, tol, 0, grad, (S*)nullptr);
-      hipLaunchKernelGGL(k_cgm_upd<S>, dim3(nc), dim3(256), 0, st, s, c, it, tol);
-    }
-    hipLaunchKernelGGL(k_cgm_final<S>, dim3(nc), dim3(256), 0, st, s, c, gamma, damp, y, scal);
-    TFX_CHECK_LAUNCH(ctx);
-    return TFX_OK;
-  }
+  if (s.R > 64)
+    return tfx_set_error(ctx, TFX_E_UNSUPPORTED,
+                         "CG for problems beyond the single-workgroup tile limit supports R <= 64");
   void* wsp;
   int rc = tfx_workspace(ctx, sizeof(S) * (5 * s.n + 2 * LRR) + 256, st, &wsp);
   if (rc) return rc;
