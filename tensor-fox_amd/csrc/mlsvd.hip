@@ -264,17 +264,6 @@ __device__ void wg_jacobi(double* H, int m, double* V, double* cs, double* red) 
   }
 }
 
-// Scale every column of Z (n x p col-major) to unit norm; one workgroup per column.
-__global__ __launch_bounds__(256) void k_colnorm(double* Z, int n) {
-  __shared__ double red[4];
-  double* z = Z + (int64_t)blockIdx.x * n;
-  double a = 0.0;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) a += z[e] * z[e];
-  a = block_sum(a, red);
-  const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) z[e] *= inv;
-}
-
 // Orthonormalize the p columns of Z (n x p, col-major, global) in place with CGS2.
 __global__ __launch_bounds__(1024) void k_orth(double* Z, int n, int p) {
   __shared__ double red[16];
@@ -457,16 +446,17 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   } else {
     // subspace iteration: Q <- orth(G Q), `iters` times, from Q0
     TFX_HIP(ctx, hipMemcpyAsync(Q, Q0, sizeof(double) * n * p, hipMemcpyDeviceToDevice, stream));
-    // power steps without re-orthonormalization (as the reference's range finder does), each
-    // column rescaled to unit norm so that long chains cannot overflow, then one CGS2
+    // subspace iteration Q <- orth(G Q): orthonormalizing after every product keeps the weak
+    // directions of the block at full relative precision (a chain of plain power steps would
+    // scale them by (lambda_k / lambda_1)^iters before the single QR)
     for (int it = 0; it < iters; ++it) {
       gemm(0, n, p, n, G, Q, Z);
-      hipLaunchKernelGGL(k_colnorm, dim3(p), dim3(256), 0, stream, Z, n);
+      hipLaunchKernelGGL(k_orth, dim3(1), dim3(1024), 0, stream, Z, n, p);
       double* t = Q;
       Q = Z;
       Z = t;
     }
-    hipLaunchKernelGGL(k_orth, dim3(1), dim3(1024), 0, stream, Q, n, p);
+    if (iters == 0) hipLaunchKernelGGL(k_orth, dim3(1), dim3(1024), 0, stream, Q, n, p);
     // Rayleigh-Ritz: H = Q^T (G Q)
     gemm(0, n, p, n, G, Q, Z);
     gemm(1, p, p, n, Q, Z, H);
