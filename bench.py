@@ -71,8 +71,8 @@ def mttkrp_kernel_time(dev, Tdev, dims, R, prec, reps):
     import numpy as np
     import torch
     rs = np.random.RandomState(1)
-    F = dev.upload(rs.standard_normal(R * sum(dims)), torch.float32)
-    out = dev.zeros(R * sum(dims), torch.float32)
+    F = dev.upload(rs.standard_normal(R * sum(dims)), torch.float64)
+    out = dev.zeros(R * sum(dims), torch.float64)
     for _ in range(3):
         dev.mttkrp(Tdev, dims, F, R, out, prec=prec)
     st = torch.cuda.current_stream()
@@ -174,13 +174,13 @@ def main():
     }
 
     # dominant kernel: fused MTTKRP over the full tensor
-    kt = mttkrp_kernel_time(dev, Tdev, dims, R, _lib.TFX_PREC_NATIVE, args.kernel_reps)
+    kt = mttkrp_kernel_time(dev, Tdev, dims, R, _lib.TFX_PREC_SPLIT16, args.kernel_reps)
     N = int(np.prod(dims))
-    alg_bytes = 4 * N + 4 * R * sum(dims) * 2
+    alg_bytes = 4 * N + 8 * R * sum(dims) * 2
     gbs = alg_bytes / kt / 1e9
     result['roofline'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                           'frac': round(gbs / HBM_PEAK_GBS, 4), 'traffic': None,
-                          'kernel': 'tfx_mttkrp (all modes, FP32)', 'avg_launch_s': kt}
+                          'kernel': 'tfx_mttkrp split16 (all modes, FP32 tensor, bf16x3 MFMA)', 'avg_launch_s': kt}
 
     if not args.no_fp64:
         T64 = Tdev.to(torch.float64)          # the FP64 form's input, resident before timing

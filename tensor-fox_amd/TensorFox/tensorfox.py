@@ -203,7 +203,10 @@ def tricpd(T, R, options):
     if options.refine:
         if display > 0:
             print('Computing refinement of solution')
-        probT = Problem(dev, Tdev, dims, R, wdt, prec)
+        # the refinement streams the full tensor: the mixed form runs its MTTKRP on the bf16
+        # matrix cores (split FP32 operands), the FP64 form with FP64 FMA
+        rprec = _lib.TFX_PREC_SPLIT16 if precision == 'mixed' else _lib.TFX_PREC_NATIVE
+        probT = Problem(dev, Tdev, dims, R, wdt, rprec)
         Ffull, ss_r, e_r, im_r, g_r, stop_r = dGN(probT, Ffull, options, Tsize, mean_abs)
     else:
         ss_r, e_r, im_r, g_r, stop_r = np.array([0]), np.array([0]), np.array([0]), np.array([0]), 8

@@ -396,6 +396,11 @@ __global__ void k_stats_final(const double* __restrict__ part, int nb, double* o
 int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
                        unsigned mask, double* out, hipStream_t stream);
 
+int tfx_mttkrp_native_fd(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
+                         unsigned mask, double* out, hipStream_t stream) {
+  return mttkrp_native<float, double>(ctx, T, s, F, mask, out, stream);
+}
+
 extern "C" int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L,
                                 const int64_t* dims, const void* F_flat, int f_dtype, int R,
                                 unsigned mode_mask, void* out_flat, int prec, hipStream_t stream) {

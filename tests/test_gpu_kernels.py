@@ -62,6 +62,29 @@ def test_mttkrp_all_modes(dev, dims, R, dt):
         assert _rel(got[l], ref) < TOL[dt], (l, _rel(got[l], ref))
 
 
+SPLIT_CASES = [((64, 100, 96), 50), ((130, 70, 400), 50), ((20, 30, 40, 48), 24), ((40, 96, 96), 70),
+               ((300, 128), 20), ((10, 20, 30), 5), ((3, 5, 4, 2, 8), 3)]
+
+
+@pytest.mark.parametrize('dims,R', SPLIT_CASES)
+def test_mttkrp_split16(dev, dims, R):
+    """bf16x3 MFMA MTTKRP (FP32 tensor, FP64 factors): each product carries ~2^-17 relative
+    representation error, so entries agree with the FP64 oracle to ~1e-5 of the result scale."""
+    print('mttkrp split16', dims, R, flush=True)
+    from TensorFox import _lib
+    T, F = _case(dims, R, 21)
+    T32 = T.astype(np.float32)
+    Td = dev.upload(T32.ravel(), torch.float32)
+    Fd = dev.upload(_flat(F), torch.float64)
+    out = dev.zeros(R * sum(dims), torch.float64)
+    dev.mttkrp(Td, dims, Fd, R, out, prec=_lib.TFX_PREC_SPLIT16)
+    got = _unflat(out.cpu().numpy(), dims, R)
+    T64 = T32.astype(np.float64)
+    for l in range(len(dims)):
+        ref = o.mttkrp(T64, F, l)
+        assert _rel(got[l], ref) < 3e-5, (l, _rel(got[l], ref))
+
+
 def test_mttkrp_mode_mask(dev):
     dims, R = (6, 7, 8), 4
     T, F = _case(dims, R, 2)
