@@ -45,6 +45,9 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-sample-iters', type=int, default=1)
     ap.add_argument('--kernel-reps', type=int, default=20)
+    ap.add_argument('--kernel-only', action='store_true',
+                    help='time only the fused MTTKRP on a random resident tensor of the same shape, '
+                         'and a plain device copy of the tensor (the reference for HBM counters)')
     return ap.parse_args()
 
 
@@ -85,6 +88,38 @@ def mttkrp_kernel_time(dev, Tdev, dims, R, prec, reps):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
+def kernel_only(args):
+    """MTTKRP timing alone: FP32 tensor of the workload's shape (random values), FP64 factors,
+    split16 precision; then `reps` plain copies of the tensor (torch's copy kernel), whose counters
+    give the bytes a kernel moving exactly 2|T| reports."""
+    import numpy as np
+    import torch
+    from TensorFox import _lib
+    from TensorFox.device import get_device
+    dims, R = tuple(args.dims), args.rank
+    dev = get_device(0)
+    g = torch.Generator(device=dev.dev)
+    g.manual_seed(0)
+    Tdev = torch.randn(dims, generator=g, device=dev.dev, dtype=torch.float32)
+    kt = mttkrp_kernel_time(dev, Tdev, dims, R, _lib.TFX_PREC_SPLIT16, args.kernel_reps)
+    N = int(np.prod(dims))
+    alg_bytes = 4 * N + 8 * R * sum(dims) * 2
+    dst = torch.empty_like(Tdev)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.kernel_reps):
+        dst.copy_(Tdev)
+    e1.record()
+    e1.synchronize()
+    ct = e0.elapsed_time(e1) / 1e3 / args.kernel_reps
+    res = {'kernel': 'tfx_mttkrp split16', 'dims': list(dims), 'rank': R,
+           'avg_launch_s': kt, 'alg_bytes': alg_bytes, 'achieved_GBs': round(alg_bytes / kt / 1e9, 1),
+           'frac': round(alg_bytes / kt / 1e9 / HBM_PEAK_GBS, 4),
+           'copy_avg_s': ct, 'copy_GBs': round(2 * 4 * N / ct / 1e9, 1)}
+    print(json.dumps(res), flush=True)
+
+
 def cpu_baseline(T32, R, sample_iters, seed):
     """FP64 NumPy restatement of the reference algorithm (the oracle) on the host cores: the
     MLSVD, `sample_iters` main-stage and `sample_iters` refinement dGN iterations are timed, then
@@ -117,6 +152,9 @@ def main():
               'set (multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.',
               file=sys.stderr)
         sys.exit(2)
+    if args.kernel_only:
+        kernel_only(args)
+        return
     import numpy as np
     import torch
     import TensorFox as tfx

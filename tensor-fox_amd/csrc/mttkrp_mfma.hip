@@ -5,16 +5,25 @@
 // lo = bf16(x - hi) (|x - hi - lo| <= 2^-18 |x|); a product is formed as
 // hi*hi + hi*lo + lo*hi with FP32 accumulation (the dropped lo*lo term is ~2^-16 relative).
 //
-// T is read as rows of K = I_{L-1} contiguous values.  A row is (o, j): o the combined index of
-// modes 0..L-3 ("outer"), j = i_{L-2}.  A workgroup owns one block of 64 consecutive j and a
-// range of o; a tile is the 64 rows (o, j0..j0+63), streamed through LDS in chunks of 32 k.
-// Per chunk and per wave w (ranks 16w..16w+15 of the current 64-rank chunk):
-//   Y[rows, r]   += T[rows, k] F_{L-1}[k, r]           (GEMM 1, A = T row-major)
-//   N_{L-1}[k, r] += T[rows, k]^T W[rows, r]           (GEMM 2, A = T transposed)
-// with W[(o, j), r] = prod_{m < L-1} F_m[i_m, r].  After a tile, Y gives the other modes:
-//   N_{L-2}[j, r] += Y Fo[r]                           (per-workgroup partial in LDS, FP64)
-//   N_m[i_m, r]   += sum_j Y F_{L-2}[j, r] Fo_{!m}[r]  (m < L-2; FP64 atomics per tile)
-// N_{L-1} partials (FP32 MFMA accumulators, one per workgroup) are summed in FP64 afterwards.
+// Notation: T is viewed as T[o, j, k] with k = i_{L-1} (contiguous), j = i_{L-2} and o the
+// combined index of modes 0..L-3 ("outer", o = 0 when L = 2).  Fo[o, r] = prod_{m<L-2} F_m[i_m, r].
+//   N_{L-1}[k, r] = sum_o Fo[o, r] Z_o[k, r],   Z_o = T[o]^T F_{L-2}     (GEMM 2)
+//   N_{L-2}[j, r] = sum_o Fo[o, r] Y_o[j, r],   Y_o = T[o] F_{L-1}       (GEMM 1)
+//   P[o, r]       = sum_j F_{L-2}[j, r] Y_o[j, r]
+//   N_m[i, r]     = sum_{o: i_m(o) = i} P[o, r] prod_{q<L-2, q!=m} F_q[i_q(o), r]   (m < L-2)
+// A workgroup owns a brick of 64 k x 64 j x a range of o and streams the tiles T[o, j-block,
+// k-block] (64 x 64 FP32 = 16 KB) through LDS, stored once, row-major, as hi/lo bf16.  GEMM 1
+// reads its A operand (rows of T) with plain 16-byte LDS reads; GEMM 2 needs T transposed and
+// reads it with the gfx950 transposing LDS read (ds_read_b64_tr_b16, see tr_offsets).  Both GEMMs
+// have a B operand that is fixed for the workgroup (the F_{L-1} and F_{L-2} rows of the brick,
+// held in registers as hi/lo fragments), and Fo enters as a per-tile scale of the MFMA results.
+// Bricks that share a (j-block, o range) run on one XCD, so that the 128-byte lines split
+// between neighbouring k-blocks are fetched once into that XCD's L2.  The 8 waves split a tile as
+//   GEMM 1: wave (rb, cp) -> rows 16 rb..16 rb+15, ranks 32 cp..32 cp+31 (2 column blocks)
+//   GEMM 2: wave (rb, cp) -> k    16 rb..16 rb+15, ranks 32 cp..32 cp+31
+// so every wave keeps its N_{L-2} and N_{L-1} brick accumulators in registers for the whole
+// range of o.  P is summed over the 4 row waves with LDS atomics.  The brick partials (FP32) are
+// written once per workgroup and summed in FP64 by k_sum_parts.
 //
 // Register fragment layouts of v_mfma_f32_16x16x32_bf16 (64 lanes, l = lane):
 //   A (16 x 32): row l % 16, eight k values in the slot group l / 16
@@ -24,34 +33,38 @@
 #include "common.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 int tfx_mttkrp_native_fd(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
                          unsigned mask, double* out, hipStream_t stream);
 
 namespace {
 
-constexpr int TR = 64;        // rows per tile (block of j)
-constexpr int KC = 32;        // k per chunk
+constexpr int NT = 512;       // threads per workgroup (8 waves)
+constexpr int TB = 64;        // tile: 64 j x 64 k
 constexpr int RC = 64;        // ranks per launch
-constexpr int PD = 3;         // chunks prefetched ahead
-constexpr int RM_P = 40;      // row-major tile pitch (bf16): 32 + 8
-constexpr int CM_P = 72;      // transposed tile pitch (bf16): 64 + 8
+constexpr int PD = 4;         // tiles prefetched ahead (even: the LDS buffer is (o - ob) & 1)
+constexpr int MAXOC = 128;    // o per workgroup at most (LDS Fo and P tables)
+constexpr int MAXO = TFX_MAXL - 2;
+constexpr int TP = 64;        // tile pitch in bf16 (rows are swizzled, see swz)
+
+constexpr int NXCD = 8;       // XCDs; workgroup g runs on XCD g % NXCD
+
+constexpr size_t LDS_TILE = sizeof(__bf16) * 2 * 2 * TB * TP;   // [2 buf][2 hi/lo][64][TP]
+constexpr size_t LDS_TAB = sizeof(float) * MAXOC * RC;
+constexpr size_t LDS_TOTAL = LDS_TILE + 2 * LDS_TAB;
+static_assert(LDS_TOTAL <= 160 * 1024, "LDS budget");
 
 struct MPlan {
   Shape s;
   int rc0;
   int64_t K, J, NO;
-  int jblocks;
+  int kblocks, jblocks, osplit;
   int64_t ochunk;
-  int nwg;
-};
-
-struct Lds {
-  __bf16* rm;    // [2 buf][2 hi/lo][TR][RM_P]
-  __bf16* cm;    // [2 buf][2 hi/lo][KC][CM_P]
-  double* fj;    // [TR][RC]  F_{L-2} rows of the block
-  double* pj;    // [TR][RC]  partial N_{L-2}
+  int nwg;       // bricks x o ranges
+  int nlaunch;   // nwg rounded up to a multiple of NXCD
 };
 
 __device__ inline void split(float x, __bf16& h, __bf16& l) {
@@ -59,35 +72,7 @@ __device__ inline void split(float x, __bf16& h, __bf16& l) {
   l = (__bf16)(x - (float)h);
 }
 
-// global -> registers: 8 consecutive k of one row (zeros outside the tensor)
-__device__ inline void load_chunk(const float* T, const MPlan& P, int64_t o, int j0, int c,
-                                  float v[8]) {
-  const int t = threadIdx.x;
-  const int row = t >> 2, kq = t & 3;
-  const int64_t j = j0 + row;
-  const int64_t k = (int64_t)c * KC + 8 * kq;
-  if (o < 0 || j >= P.J || k >= P.K) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = 0.f;
-    return;
-  }
-  const float* src = T + (o * P.J + j) * P.K + k;
-  if (k + 8 <= P.K) {
-    const float4 a = *(const float4*)src;
-    const float4 b = *(const float4*)(src + 4);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-  } else {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (k + e < P.K) ? src[e] : 0.f;
-  }
-}
-
-// registers -> LDS buffer `buf` as hi/lo bf16, row-major and transposed
-__device__ inline void store_chunk(const Lds& S, int buf, const float v[8]) {
-  const int t = threadIdx.x;
-  const int row = t >> 2, kq = t & 3;
-  bf16x8 h, l;
+__device__ inline void split8(const float v[8], bf16x8& h, bf16x8& l) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     __bf16 a, b;
@@ -95,16 +80,93 @@ __device__ inline void store_chunk(const Lds& S, int buf, const float v[8]) {
     h[e] = a;
     l[e] = b;
   }
-  __bf16* rmh = S.rm + ((buf * 2 + 0) * TR + row) * RM_P + 8 * kq;
-  __bf16* rml = S.rm + ((buf * 2 + 1) * TR + row) * RM_P + 8 * kq;
-  *(bf16x8*)rmh = h;
-  *(bf16x8*)rml = l;
+}
+
+// global -> registers: thread t reads 8 consecutive k (8 (t & 7)..) of tile row t >> 3 with two
+// 16-byte raw buffer loads.  An out-of-range half gets an offset past the buffer's record count,
+// for which the hardware returns zeros: no branch and no select on the loaded data, so the
+// compiler leaves the loads in flight until the tile is stored.  Needs K % 4 == 0 (a row's tail
+// is then a whole half).
+constexpr int RSRC_FLAGS = 0x00020000;     // gfx9 raw buffer descriptor word 3
+constexpr int RSRC_RECORDS = 0x7ffffff0;
+constexpr int OFF_NONE = 0x7ffffff0;       // >= RSRC_RECORDS: reads as zeros
+
+__device__ inline void load_tile(const float* T, const MPlan& P, int64_t o, int64_t j0,
+                                 int64_t k0, float v[8]) {
+  const int t = threadIdx.x;
+  const int row = t >> 3;
+  const int64_t j = j0 + row;
+  const int64_t k = k0 + 8 * (t & 7);
+  const bool ok0 = o >= 0 && j < P.J && k < P.K;
+  const bool ok1 = ok0 && k + 8 <= P.K;
+  const float* base = T + (o >= 0 ? (o * P.J + j0) * P.K + k0 : 0);
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, RSRC_RECORDS, RSRC_FLAGS);
+  const int off = (int)((int64_t)row * P.K + 8 * (t & 7)) * 4;
+  // (convert whole vectors: an element-wise bit_cast of the load result has been miscompiled
+  // into a single-dword load by this toolchain)
+  const f32x4 a = __builtin_bit_cast(
+      f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok0 ? off : OFF_NONE, 0, 0));
+  const f32x4 b = __builtin_bit_cast(
+      f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok1 ? off + 16 : OFF_NONE, 0, 0));
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    S.cm[((buf * 2 + 0) * KC + 8 * kq + e) * CM_P + row] = h[e];
-    S.cm[((buf * 2 + 1) * KC + 8 * kq + e) * CM_P + row] = l[e];
+  for (int e = 0; e < 4; ++e) {
+    v[e] = a[e];
+    v[4 + e] = b[e];
   }
 }
+
+// Offset of the 16-byte chunk `chunk` (k = 8 chunk..8 chunk+7) of tile row `row`.  Chunks are
+// permuted within a row by an XOR with g(row) = 2 (row & 3) + ((row >> 2) & 1), which keeps
+// the three access patterns free of LDS bank conflicts: 16-byte stores and 16-byte row reads
+// (eight consecutive rows, one chunk each: g takes 8 distinct values on 8 consecutive rows) and
+// the transposing reads (rows j, j+1, j+2, j+3 with j % 8 in {0, 4}, two adjacent chunks).
+__device__ inline int swz(int row, int chunk) {
+  return row * TP + ((chunk ^ (2 * (row & 3) + ((row >> 2) & 1))) << 3);
+}
+
+// registers -> LDS buffer `buf` as hi/lo bf16, rm[j][k]
+__device__ inline void store_tile(__bf16* rm, int buf, const float v[8]) {
+  const int t = threadIdx.x;
+  const int row = t >> 3, kq = t & 7;
+  bf16x8 h, l;
+  split8(v, h, l);
+  *(bf16x8*)(rm + (buf * 2 + 0) * TB * TP + swz(row, kq)) = h;
+  *(bf16x8*)(rm + (buf * 2 + 1) * TB * TP + swz(row, kq)) = l;
+}
+
+// Element offsets of the two transposing reads that give a lane its GEMM 2 A fragment
+// (A = T^T: row k0 + lane % 16, eight j = j0 + 8 (lane / 16) + 0..7) from the row-major tile
+// [j][k].  ds_read_b64_tr_b16 works on groups of 16 lanes: lane i of a group receives element
+// i % 4 of the 8-byte chunks addressed by lanes 4c + i / 4, c = 0..3, of its group.  Lane
+// 4c + q of group G therefore addresses [j0 + 8G + c (+ 4)][k0 + 4q], and lane i gets
+// T[j0 + 8G + c (+ 4), k0 + i]: the first read gives slots 0..3, the second slots 4..7.
+__device__ inline void tr_offsets(int j0, int k0, int lane, int off[2]) {
+  const int G = lane >> 4, c = (lane >> 2) & 3, q = lane & 3;
+  const int row = j0 + 8 * G + c, k = k0 + 4 * q;     // k0 % 8 == 0
+  off[0] = swz(row, k >> 3) + (k & 7);
+  off[1] = swz(row + 4, k >> 3) + (k & 7);
+}
+
+// x[l] + x[l ^ 16] + x[l ^ 32] + x[l ^ 48] in every lane, in VALU: v_permlane16_swap swaps the
+// odd 16-lane rows of its first operand with the even rows of its second, v_permlane32_swap the
+// upper half of the first with the lower half of the second, so with both operands = x the two
+// results add up to x plus its partner row.  Written as inline asm (register permutes, no memory
+// access; s_nop on both sides for the VALU hazards): this toolchain's builtins for these
+// two-result instructions return the first result twice.
+__device__ inline float swap_add16(float x) {
+  float a = x, b = x;
+  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1\n\ts_nop 1" : "+v"(a), "+v"(b));
+  return a + b;
+}
+
+__device__ inline float swap_add32(float x) {
+  float a = x, b = x;
+  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1\n\ts_nop 1" : "+v"(a), "+v"(b));
+  return a + b;
+}
+
+__device__ inline float sum_rows4(float x) { return swap_add32(swap_add16(x)); }
 
 __device__ inline f32x4 mfma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x4 c) {
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
@@ -113,214 +175,338 @@ __device__ inline f32x4 mfma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x4 
   return c;
 }
 
-// decompose the outer row index o over modes 0..L-3 (last fastest)
-__device__ inline void outer_idx(const Shape& s, int64_t o, int64_t* idx) {
-  for (int m = s.L - 3; m >= 0; --m) {
-    idx[m] = o % s.d[m];
-    o /= s.d[m];
+// decompose the outer row index o over modes 0..L-3 (last fastest); static indexing only
+__device__ inline void outer_idx(const Shape& s, int64_t o, int64_t idx[MAXO]) {
+#pragma unroll
+  for (int m = MAXO - 1; m >= 0; --m) {
+    if (m < s.L - 2) {
+      idx[m] = o % s.d[m];
+      o /= s.d[m];
+    } else {
+      idx[m] = 0;
+    }
   }
 }
 
-template <int KCH>
-__global__ __launch_bounds__(256, 1) void k_mttkrp_mfma(const float* __restrict__ T, MPlan P,
-                                                        const double* __restrict__ F,
-                                                        float* __restrict__ npart,
-                                                        double* __restrict__ out, unsigned mask) {
+// Loads of factor entries are unconditional (an entry outside the factor reads col[0], a valid
+// address, and is zeroed afterwards) so that all of them are in flight together instead of each
+// waiting behind a branch.
+__device__ inline double load_or0(const double* col, int64_t x, bool ok) {
+  const double v = col[ok ? x : 0];
+  return ok ? v : 0.0;
+}
+
+// B fragment: 8 consecutive rows x0 + e of a factor column (zeros outside), split hi/lo
+__device__ inline void factor_frag(const double* col, int64_t n, int64_t x0, bool ok, bf16x8& h,
+                                   bf16x8& l) {
+  float v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (float)load_or0(col, x0 + e, ok && x0 + e < n);
+  split8(v, h, l);
+}
+
+__global__ __launch_bounds__(NT, 1) void k_mttkrp_mfma(const float* __restrict__ T, MPlan P,
+                                                       const double* __restrict__ F,
+                                                       float* __restrict__ Pk,
+                                                       float* __restrict__ Pj,
+                                                       float* __restrict__ Po) {
   extern __shared__ __align__(16) unsigned char smem[];
-  Lds S;
-  S.rm = (__bf16*)smem;
-  S.cm = S.rm + 2 * 2 * TR * RM_P;
-  S.fj = (double*)(S.cm + 2 * 2 * KC * CM_P);
-  S.pj = S.fj + TR * RC;
+  __bf16* rm = (__bf16*)smem;
+  float* fo_t = (float*)(smem + LDS_TILE);             // [MAXOC][RC] Fo of the range
+  float* po_t = (float*)(smem + LDS_TILE + LDS_TAB);   // [MAXOC][RC] P of the range
 
   const Shape& s = P.s;
   const int L = s.L, R = s.R;
   const int t = threadIdx.x;
   const int w = t >> 6, lane = t & 63, ln = lane & 15, rg = lane >> 4;
-  const int jb = blockIdx.x % P.jblocks;
-  const int64_t ob = (int64_t)(blockIdx.x / P.jblocks) * P.ochunk;
+  const int rb = w & 3, cp = w >> 2;
+  // brick index: consecutive indices (the k-blocks of one (j-block, o range), k-block fastest)
+  // go to one XCD
+  const int per_xcd = P.nlaunch / NXCD;
+  const int g = (int)(blockIdx.x % NXCD) * per_xcd + (int)(blockIdx.x / NXCD);
+  if (g >= P.nwg) return;                   // padding workgroup (uniform, before any barrier)
+  const int kb = g % P.kblocks;
+  const int jb = (g / P.kblocks) % P.jblocks;
+  const int oc = g / (P.kblocks * P.jblocks);
+  const int64_t j0 = (int64_t)jb * TB, k0 = (int64_t)kb * TB;
+  const int64_t ob = (int64_t)oc * P.ochunk;
   const int64_t oe = ob + P.ochunk < P.NO ? ob + P.ochunk : P.NO;
-  const int j0 = jb * TR;
-  const int rl = 16 * w + ln;          // rank within the chunk owned by this lane's column
-  const int r = P.rc0 + rl;            // global rank
-  const bool rok = r < R;
-  const int Lk = L - 1, Lj = L - 2;
-  const double* Fk = F + s.off[Lk];
+  const int nO = (int)(oe - ob);
 
-  // B fragments of GEMM 1: F_{L-1}[32c + 8rg + e, r]
-  bf16x8 fh[KCH], fl[KCH];
+  // this lane's two rank columns (cb = 0, 1) and their factor columns
+  int rr[2];
+  bool rok[2];
+  const double *Fk[2], *Fj[2];
 #pragma unroll
-  for (int c = 0; c < KCH; ++c) {
+  for (int cb = 0; cb < 2; ++cb) {
+    rr[cb] = 32 * cp + 16 * cb + ln;
+    const int r = P.rc0 + rr[cb];
+    rok[cb] = r < R;
+    Fk[cb] = F + s.off[L - 1] + (int64_t)(rok[cb] ? r : 0) * P.K;
+    Fj[cb] = F + s.off[L - 2] + (int64_t)(rok[cb] ? r : 0) * P.J;
+  }
+  // GEMM 1 B: F_{L-1}[k0 + 32 ch + 8 rg + e, r];  GEMM 2 B: F_{L-2}[j0 + 32 sv + 8 rg + e, r]
+  bf16x8 fkh[2][2], fkl[2][2], fjh[2][2], fjl[2][2];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int64_t k = (int64_t)c * KC + 8 * rg + e;
-      const float v = (rok && k < P.K) ? (float)Fk[(int64_t)r * P.K + k] : 0.f;
-      __bf16 a, b;
-      split(v, a, b);
-      fh[c][e] = a;
-      fl[c][e] = b;
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      factor_frag(Fk[cb], P.K, k0 + 32 * h + 8 * rg, rok[cb], fkh[h][cb], fkl[h][cb]);
+      factor_frag(Fj[cb], P.J, j0 + 32 * h + 8 * rg, rok[cb], fjh[h][cb], fjl[h][cb]);
+    }
+  // F_{L-2} at the rows of this lane's GEMM 1 results: j0 + 16 rb + 4 rg + e
+  float fjo[2][4];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t j = j0 + 16 * rb + 4 * rg + e;
+      fjo[cb][e] = (float)load_or0(Fj[cb], j, rok[cb] && j < P.J);
+    }
+  // Fo over the o range (order 3: one factor entry, order 2: 1), P accumulators
+  constexpr int FOU = MAXOC * RC / NT;
+  if (L <= 3) {
+    float fv[FOU];
+#pragma unroll
+    for (int u = 0; u < FOU; ++u) {
+      const int e = t + NT * u;
+      const int oo = e / RC, r = P.rc0 + e % RC;
+      const bool ok = oo < nO && r < R;
+      fv[u] = L == 3 ? (float)load_or0(F + s.off[0], (int64_t)r * s.d[0] + ob + oo, ok)
+                     : (ok ? 1.f : 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < FOU; ++u) {
+      const int e = t + NT * u;
+      if (e < nO * RC) {
+        fo_t[e] = fv[u];
+        po_t[e] = 0.f;
+      }
+    }
+  } else {
+    for (int e = t; e < nO * RC; e += NT) {
+      const int oo = e / RC, q = e % RC;
+      const int r = P.rc0 + q;
+      double f = r < R ? 1.0 : 0.0;
+      if (r < R) {
+        int64_t idx[MAXO];
+        outer_idx(s, ob + oo, idx);
+#pragma unroll
+        for (int m = 0; m < MAXO; ++m)
+          if (m < L - 2) f *= F[s.off[m] + (int64_t)r * s.d[m] + idx[m]];
+      }
+      fo_t[e] = (float)f;
+      po_t[e] = 0.f;
     }
   }
-  f32x4 nacc[KCH][2];
-#pragma unroll
-  for (int c = 0; c < KCH; ++c) {
-    nacc[c][0] = f32x4{0.f, 0.f, 0.f, 0.f};
-    nacc[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-  }
-  // F_{L-2} rows of this block and the N_{L-2} partial
-  for (int e = t; e < TR * RC; e += 256) {
-    const int jj = e / RC, rr = e % RC;
-    const int64_t j = j0 + jj;
-    double v = 0.0;
-    if (L >= 2 && j < P.J && P.rc0 + rr < R) v = F[s.off[Lj] + (int64_t)(P.rc0 + rr) * P.J + j];
-    S.fj[e] = v;
-    S.pj[e] = 0.0;
-  }
-
-  // Chunk c of a tile always travels through register slot c % PD.  A slot is refilled right
-  // after it is consumed: with chunk c + PD of the same tile, or, after the last chunk of its
-  // residue class, with chunk c % PD of the next tile.
   float pre[PD][8];
 #pragma unroll
-  for (int q = 0; q < PD; ++q)
-    if (q < KCH) load_chunk(T, P, ob < oe ? ob : -1, j0, q, pre[q]);
+  for (int q = 0; q < PD; ++q) {
+    load_tile(T, P, ob + q < oe ? ob + q : -1, j0, k0, pre[q]);
+    __builtin_amdgcn_sched_barrier(0);     // keep the slots' loads in slot order
+  }
   __syncthreads();
 
-  for (int64_t o = ob; o < oe; ++o) {
-    // product of the outer-mode factor rows for this lane's rank
-    int64_t idx[TFX_MAXL];
-    outer_idx(s, o, idx);
-    double Fo = 1.0;
-    for (int m = 0; m < Lj; ++m) Fo *= rok ? F[s.off[m] + (int64_t)r * s.d[m] + idx[m]] : 0.0;
-    // B fragments of GEMM 2: W[32 sv + 8rg + e, rl]
-    bf16x8 wh[2], wl[2];
-#pragma unroll
-    for (int sv = 0; sv < 2; ++sv)
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int jj = 32 * sv + 8 * rg + e;
-        const float v = (float)(Fo * S.fj[jj * RC + rl]);
-        __bf16 a, b;
-        split(v, a, b);
-        wh[sv][e] = a;
-        wl[sv][e] = b;
-      }
-    f32x4 yacc[4];
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb) yacc[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  f32x4 nacc[2], pacc[2];
 #pragma unroll
-    for (int c = 0; c < KCH; ++c) {
-      const int buf = c & 1;
-      store_chunk(S, buf, pre[c % PD]);
-      if (c + PD < KCH)
-        load_chunk(T, P, o, j0, c + PD, pre[c % PD]);
-      else
-        load_chunk(T, P, o + 1 < oe ? o + 1 : -1, j0, c % PD, pre[c % PD]);
-      // one barrier per chunk: buffer c & 1 is written before it and read after it; the other
+  for (int cb = 0; cb < 2; ++cb) {
+    nacc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    pacc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+
+  // Tile o travels through register slot q = (o - ob) % PD, refilled with tile o + PD right
+  // after it is stored.  Full groups of PD tiles run in a loop without early exits (so that the
+  // compiler's count of loads in flight carries across its back edge); the rest follows.
+  // LDS offsets (elements) of this lane's operand reads in buffer 0 (buffer 1 adds 2 TB TP)
+  const int r1 = 16 * rb + ln;
+  int a1_off[2], a2_off[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    a1_off[h] = swz(r1, 4 * h + rg);
+    tr_offsets(32 * h, 16 * rb, lane, a2_off[h]);
+  }
+  typedef __attribute__((address_space(3))) bf16x4 lds_bf16x4;
+
+  auto tile = [&](const int q, const int64_t o) __attribute__((always_inline)) {
+      const int buf = q & 1;
+      store_tile(rm, buf, pre[q]);
+      load_tile(T, P, o + PD < oe ? o + PD : -1, j0, k0, pre[q]);
+      // one barrier per tile: buffer q & 1 is written before it and read after it; the other
       // buffer was last read before this barrier
       __syncthreads();
-      const __bf16* rmh = S.rm + (buf * 2 + 0) * TR * RM_P;
-      const __bf16* rml = S.rm + (buf * 2 + 1) * TR * RM_P;
-      const __bf16* cmh = S.cm + (buf * 2 + 0) * KC * CM_P;
-      const __bf16* cml = S.cm + (buf * 2 + 1) * KC * CM_P;
+      const int oo = (int)(o - ob);
+      float fo[2];
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb) {
-        const bf16x8 ah = *(const bf16x8*)(rmh + (16 * rb + ln) * RM_P + 8 * rg);
-        const bf16x8 al = *(const bf16x8*)(rml + (16 * rb + ln) * RM_P + 8 * rg);
-        yacc[rb] = mfma3(ah, al, fh[c], fl[c], yacc[rb]);
+      for (int cb = 0; cb < 2; ++cb) fo[cb] = fo_t[oo * RC + rr[cb]];
+      // A operands: GEMM 1 rows (16-byte reads), GEMM 2 columns (transposing reads); rows and
+      // columns outside the tensor are zeros in LDS
+      const __bf16* th = rm + (buf * 2 + 0) * TB * TP;
+      const __bf16* tl = rm + (buf * 2 + 1) * TB * TP;
+      bf16x8 a1h[2], a1l[2], a2h[2], a2l[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        a1h[h] = *(const bf16x8*)(th + a1_off[h]);
+        a1l[h] = *(const bf16x8*)(tl + a1_off[h]);
+        a2h[h] = __builtin_shufflevector(
+            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(th + a2_off[h][0])),
+            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(th + a2_off[h][1])),
+            0, 1, 2, 3, 4, 5, 6, 7);
+        a2l[h] = __builtin_shufflevector(
+            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(tl + a2_off[h][0])),
+            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(tl + a2_off[h][1])),
+            0, 1, 2, 3, 4, 5, 6, 7);
+      }
+      // 4 independent accumulator chains (y[0], y[1], z[0], z[1]) interleaved, so that a
+      // dependent MFMA issues 4 instructions after its predecessor
+      f32x4 y[2], z[2];
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        y[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        z[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb)
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int sv = 0; sv < 2; ++sv) {
-          const bf16x8 ah = *(const bf16x8*)(cmh + (16 * kb + ln) * CM_P + 32 * sv + 8 * rg);
-          const bf16x8 al = *(const bf16x8*)(cml + (16 * kb + ln) * CM_P + 32 * sv + 8 * rg);
-          nacc[c][kb] = mfma3(ah, al, wh[sv], wl[sv], nacc[c][kb]);
+        for (int term = 0; term < 3; ++term) {
+          const bf16x8 xa1 = term == 2 ? a1l[h] : a1h[h];
+          const bf16x8 xa2 = term == 2 ? a2l[h] : a2h[h];
+#pragma unroll
+          for (int cb = 0; cb < 2; ++cb) {
+            const bf16x8 xb1 = term == 1 ? fkl[h][cb] : fkh[h][cb];
+            const bf16x8 xb2 = term == 1 ? fjl[h][cb] : fjh[h][cb];
+            y[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa1, xb1, y[cb], 0, 0, 0);
+            z[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa2, xb2, z[cb], 0, 0, 0);
+          }
         }
-    }
-
-    // leading modes from Y: lane holds Y[16 rb + 4 rg + e, rl]
-    double part[TFX_MAXL];
-    for (int m = 0; m < Lj; ++m) part[m] = 0.0;
+      // Fo scales; N_{L-2} (pacc) and N_{L-1} (nacc) brick accumulators; P over this wave's 16
+      // rows (lane rows summed in VALU), added to the LDS table by one lane row
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+      for (int cb = 0; cb < 2; ++cb) {
+        float p = 0.f;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int jj = 16 * rb + 4 * rg + e;
-        if (j0 + jj < P.J && rok) {
-          const double y = (double)yacc[rb][e];
-          S.pj[jj * RC + rl] += y * Fo;
-          const double yf = y * S.fj[jj * RC + rl];
-          for (int m = 0; m < Lj; ++m) part[m] += yf;
+        for (int e = 0; e < 4; ++e) {
+          pacc[cb][e] += fo[cb] * y[cb][e];
+          nacc[cb][e] += fo[cb] * z[cb][e];
+          p += fjo[cb][e] * y[cb][e];
         }
+        p = sum_rows4(p);
+        if (rg == 0) atomicAdd(&po_t[oo * RC + rr[cb]], p);
       }
-    for (int m = 0; m < Lj; ++m) {
-      if (!((mask >> m) & 1u)) continue;
-      // multiply by the product of the outer rows other than m
-      double fo = 1.0;
-      for (int q = 0; q < Lj; ++q)
-        if (q != m) fo *= rok ? F[s.off[q] + (int64_t)r * s.d[q] + idx[q]] : 0.0;
-      double v = part[m] * fo;
-      v += __shfl_xor(v, 16, 64);
-      v += __shfl_xor(v, 32, 64);
-      if (rg == 0 && rok) atomicAdd(&out[s.off[m] + (int64_t)r * s.d[m] + idx[m]], v);
-    }
+  };
+  const int64_t ofull = ob + (int64_t)(nO / PD) * PD;
+  for (int64_t o0 = ob; o0 < ofull; o0 += PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) tile(q, o0 + q);
   }
+#pragma unroll
+  for (int q = 0; q < PD; ++q)
+    if (ofull + q < oe) tile(q, ofull + q);
   __syncthreads();
-  // flush N_{L-2}
-  if ((mask >> Lj) & 1u) {
-    for (int e = t; e < TR * RC; e += 256) {
-      const int jj = e / RC, rr = e % RC;
-      const int64_t j = j0 + jj;
-      if (j < P.J && P.rc0 + rr < R && S.pj[e] != 0.0)
-        atomicAdd(&out[s.off[Lj] + (int64_t)(P.rc0 + rr) * P.J + j], S.pj[e]);
+
+  // brick partials: lane holds rows 16 rb + 4 rg + e of its two rank columns
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t x = 16 * rb + 4 * rg + e;
+      if (k0 + x < P.K)
+        Pk[(((int64_t)jb * P.osplit + oc) * P.K + k0 + x) * RC + rr[cb]] = nacc[cb][e];
+      if (j0 + x < P.J)
+        Pj[(((int64_t)kb * P.osplit + oc) * P.J + j0 + x) * RC + rr[cb]] = pacc[cb][e];
     }
-  }
-  // N_{L-1} partial of this workgroup: lane holds N[32c + 16kb + 4rg + e, rl]
-  float* dst = npart + (int64_t)blockIdx.x * P.K * RC;
-#pragma unroll
-  for (int c = 0; c < KCH; ++c)
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t k = (int64_t)c * KC + 16 * kb + 4 * rg + e;
-        if (k < P.K) dst[k * RC + rl] = nacc[c][kb][e];
-      }
+  if (L > 2)
+    for (int e = t; e < nO * RC; e += NT)
+      Po[(((int64_t)kb * P.jblocks + jb) * P.NO + ob) * RC + e] = po_t[e];
 }
 
-__global__ void k_reduce_npart(const float* __restrict__ npart, int nwg, int64_t K, int rc0, int R,
-                               int64_t off, double* __restrict__ out) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= K * RC) return;
-  const int64_t k = e / RC;
-  const int rr = (int)(e % RC);
-  if (rc0 + rr >= R) return;
+// Sums of the brick partials, all jobs in one launch.  Job b covers rows x < rows_b of a
+// partial array part_b[np_b][rows_b][RC]; its FP64 sum goes to out[off_b + r * rows_b + x]
+// (r = rc0 + q < R), or, for to_ps, to Ps[x * RC + q].
+struct SumJob {
+  const float* part;
+  int np;
+  int to_ps;
+  int64_t rows, off;
+};
+struct SumJobs {
+  SumJob job[3];
+  int n;
+};
+
+__global__ void k_sum_parts(SumJobs J, int rc0, int R, double* __restrict__ out,
+                            double* __restrict__ Ps) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int b = 0;
+  while (b < J.n && e >= J.job[b].rows * RC) e -= J.job[b++].rows * RC;
+  if (b >= J.n) return;
+  const SumJob& jb = J.job[b];
+  const int64_t x = e / RC;
+  const int q = (int)(e % RC);
+  if (!jb.to_ps && rc0 + q >= R) return;
+  const int64_t stride = jb.rows * RC;
+  const float* p0 = jb.part + e;
   double acc = 0.0;
-  for (int b = 0; b < nwg; ++b) acc += (double)npart[(int64_t)b * K * RC + e];
-  out[off + (int64_t)(rc0 + rr) * K + k] = acc;
+  int p = 0;
+  for (; p + 8 <= jb.np; p += 8) {          // eight loads in flight per thread
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p0[(int64_t)(p + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += (double)v[u];
+  }
+  for (; p < jb.np; ++p) acc += (double)p0[(int64_t)p * stride];
+  if (jb.to_ps)
+    Ps[e] = acc;
+  else
+    out[jb.off + (int64_t)(rc0 + q) * jb.rows + x] = acc;
 }
 
-template <int KCH>
-int launch_kch(tfx_ctx* ctx, const float* T, const MPlan& P, const double* F, float* npart,
-               double* out, unsigned mask, hipStream_t st) {
-  const size_t lds = sizeof(__bf16) * (2 * 2 * TR * RM_P + 2 * 2 * KC * CM_P) + sizeof(double) * 2 * TR * RC;
-  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_mttkrp_mfma<KCH>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_mttkrp_mfma<KCH>, dim3(P.nwg), dim3(256), lds, st, T, P, F, npart, out, mask);
-  TFX_CHECK_LAUNCH(ctx);
-  return TFX_OK;
+// Leading modes of an order >= 4 tensor from the summed P[o, r] (FP64, [NO][RC]):
+// out_m[i, r] = sum_{o: i_m(o) = i} P[o, r] prod_{q<L-2, q!=m} F_q[i_q(o), r]
+__global__ void k_outer_modes(const double* __restrict__ Ps, Shape s, int64_t NO, int rc0,
+                              const double* __restrict__ F, unsigned mask,
+                              double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // e enumerates (m, i, q) over the outer modes
+  int64_t rem = e;
+  int m = -1;
+  for (int mm = 0; mm < s.L - 2; ++mm) {
+    const int64_t n = s.d[mm] * RC;
+    if (rem < n) { m = mm; break; }
+    rem -= n;
+  }
+  if (m < 0 || !((mask >> m) & 1u)) return;
+  const int64_t i = rem / RC;
+  const int q = (int)(rem % RC);
+  const int r = rc0 + q;
+  if (r >= s.R) return;
+  // strides of the outer index
+  int64_t stride[MAXO];
+  int64_t st = 1;
+  for (int mm = s.L - 3; mm >= 0; --mm) {
+    stride[mm] = st;
+    st *= s.d[mm];
+  }
+  const int64_t nrest = NO / s.d[m];
+  double acc = 0.0;
+  for (int64_t u = 0; u < nrest; ++u) {
+    // u enumerates the other outer indices (last fastest); insert i at position m
+    int64_t v = u, o = i * stride[m];
+    double f = 1.0;
+    for (int mm = s.L - 3; mm >= 0; --mm) {
+      if (mm == m) continue;
+      const int64_t im = v % s.d[mm];
+      v /= s.d[mm];
+      o += im * stride[mm];
+      f *= F[s.off[mm] + (int64_t)r * s.d[mm] + im];
+    }
+    acc += Ps[o * RC + q] * f;
+  }
+  out[s.off[m] + (int64_t)r * s.d[m] + i] = acc;
 }
 
-typedef int (*launch_fn)(tfx_ctx*, const float*, const MPlan&, const double*, float*, double*,
-                         unsigned, hipStream_t);
-
-const launch_fn kLaunch[17] = {nullptr,
-                               launch_kch<1>,  launch_kch<2>,  launch_kch<3>,  launch_kch<4>,
-                               launch_kch<5>,  launch_kch<6>,  launch_kch<7>,  launch_kch<8>,
-                               launch_kch<9>,  launch_kch<10>, launch_kch<11>, launch_kch<12>,
-                               launch_kch<13>, launch_kch<14>, launch_kch<15>, launch_kch<16>};
+inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
@@ -329,34 +515,60 @@ int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const doubl
   const int L = s.L;
   const int64_t K = s.d[L - 1];
   // shapes outside the tiled kernel's envelope use the FMA path
-  if (L < 2 || K % 4 != 0 || K > 16 * KC) return tfx_mttkrp_native_fd(ctx, T, s, F, mask, out, stream);
+  if (L < 2 || K % 4 != 0) return tfx_mttkrp_native_fd(ctx, T, s, F, mask, out, stream);
   MPlan P;
   P.s = s;
   P.K = K;
   P.J = s.d[L - 2];
   P.NO = s.N / (K * P.J);
-  P.jblocks = (int)((P.J + TR - 1) / TR);
-  const int64_t want = (int64_t)ctx->cu_count;
-  int64_t per = want / P.jblocks;
-  if (per < 1) per = 1;
-  P.ochunk = (P.NO + per - 1) / per;
-  P.nwg = (int)(P.jblocks * ((P.NO + P.ochunk - 1) / P.ochunk));
-  const int kch = (int)((K + KC - 1) / KC);
+  P.kblocks = (int)((K + TB - 1) / TB);
+  P.jblocks = (int)((P.J + TB - 1) / TB);
+  const int64_t bricks = (int64_t)P.kblocks * P.jblocks;
+  // one workgroup per CU when the bricks allow it; at most MAXOC values of o per workgroup
+  int64_t osplit = std::max<int64_t>(1, ctx->cu_count / bricks);
+  osplit = std::min<int64_t>(osplit, P.NO);
+  osplit = std::max<int64_t>(osplit, (P.NO + MAXOC - 1) / MAXOC);
+  P.ochunk = (P.NO + osplit - 1) / osplit;
+  P.osplit = (int)((P.NO + P.ochunk - 1) / P.ochunk);
+  P.nwg = (int)(bricks * P.osplit);
+  P.nlaunch = (P.nwg + NXCD - 1) / NXCD * NXCD;
+  const size_t nk = (size_t)P.jblocks * P.osplit * K * RC;
+  const size_t nj = (size_t)P.kblocks * P.osplit * P.J * RC;
+  const size_t no = L > 2 ? (size_t)bricks * P.NO * RC : 0;
+  const size_t ns = L > 3 ? (size_t)P.NO * RC : 0;
   void* wsp;
-  int rc = tfx_workspace(ctx, sizeof(float) * (size_t)P.nwg * K * RC + 256, stream, &wsp);
+  int rc = tfx_workspace(ctx, align256(4 * nk) + align256(4 * nj) + align256(4 * no) +
+                                  align256(8 * ns) + 256, stream, &wsp);
   if (rc) return rc;
-  float* npart = (float*)wsp;
-  for (int m = 0; m < L - 1; ++m)
-    if ((mask >> m) & 1u)
-      TFX_HIP(ctx, hipMemsetAsync(out + s.off[m], 0, sizeof(double) * s.d[m] * s.R, stream));
+  float* Pk = (float*)wsp;
+  float* Pj = (float*)((char*)Pk + align256(4 * nk));
+  float* Po = (float*)((char*)Pj + align256(4 * nj));
+  double* Ps = (double*)((char*)Po + align256(4 * no));
+  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_mttkrp_mfma,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_TOTAL));
   for (int rc0 = 0; rc0 < s.R; rc0 += RC) {
     P.rc0 = rc0;
-    rc = kLaunch[kch](ctx, T, P, F, npart, out, mask, stream);
-    if (rc) return rc;
-    if ((mask >> (L - 1)) & 1u) {
-      const int64_t tot = K * RC;
-      hipLaunchKernelGGL(k_reduce_npart, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
-                         npart, P.nwg, K, rc0, s.R, s.off[L - 1], out);
+    hipLaunchKernelGGL(k_mttkrp_mfma, dim3(P.nlaunch), dim3(NT), LDS_TOTAL, stream, T, P, F, Pk,
+                       Pj, Po);
+    TFX_CHECK_LAUNCH(ctx);
+    SumJobs J;
+    J.n = 0;
+    if ((mask >> (L - 1)) & 1u) J.job[J.n++] = SumJob{Pk, P.jblocks * P.osplit, 0, K, s.off[L - 1]};
+    if ((mask >> (L - 2)) & 1u)
+      J.job[J.n++] = SumJob{Pj, P.kblocks * P.osplit, 0, P.J, s.off[L - 2]};
+    const bool outer = L > 2 && (mask & ((1u << (L - 2)) - 1u));
+    if (outer) J.job[J.n++] = SumJob{Po, (int)bricks, L > 3 ? 1 : 0, P.NO, s.off[0]};
+    int64_t tot = 0;
+    for (int b = 0; b < J.n; ++b) tot += J.job[b].rows * RC;
+    if (tot > 0) {
+      hipLaunchKernelGGL(k_sum_parts, dim3(nblk(tot)), dim3(256), 0, stream, J, rc0, s.R, out, Ps);
+      TFX_CHECK_LAUNCH(ctx);
+    }
+    if (L > 3 && outer) {
+      int64_t tot = 0;
+      for (int m = 0; m < L - 2; ++m) tot += s.d[m] * RC;
+      hipLaunchKernelGGL(k_outer_modes, dim3(nblk(tot)), dim3(256), 0, stream, Ps, s, P.NO, rc0, F,
+                         mask, out);
       TFX_CHECK_LAUNCH(ctx);
     }
   }
