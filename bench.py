@@ -26,6 +26,12 @@ for p in (ROOT, os.path.join(ROOT, 'tensor-fox_amd')):
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per all-mode split16 MTTKRP call measured with rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE (separate runs of `bench.py --kernel-only`), each scaled by the counts of a
+# plain 256 MB device copy in the same run (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
+# reads 262.4 MB + writes 12.2 MB (brick partials) + the partial sums (6.1 MB read).
+MEASURED_TRAFFIC = {((400, 400, 400), 50): 280.7e6}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -217,7 +223,8 @@ def main():
     alg_bytes = 4 * N + 8 * R * sum(dims) * 2
     gbs = alg_bytes / kt / 1e9
     result['roofline'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                          'frac': round(gbs / HBM_PEAK_GBS, 4), 'traffic': None,
+                          'frac': round(gbs / HBM_PEAK_GBS, 4),
+                          'traffic': MEASURED_TRAFFIC.get((dims, R)),
                           'kernel': 'tfx_mttkrp split16 (all modes, FP32 tensor, bf16x3 MFMA)', 'avg_launch_s': kt}
 
     if not args.no_fp64:

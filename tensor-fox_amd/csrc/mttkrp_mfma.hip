@@ -19,11 +19,13 @@
 // held in registers as hi/lo fragments), and Fo enters as a per-tile scale of the MFMA results.
 // Bricks that share a (j-block, o range) run on one XCD, so that the 128-byte lines split
 // between neighbouring k-blocks are fetched once into that XCD's L2.  The 8 waves split a tile as
-//   GEMM 1: wave (rb, cp) -> rows 16 rb..16 rb+15, ranks 32 cp..32 cp+31 (2 column blocks)
-//   GEMM 2: wave (rb, cp) -> k    16 rb..16 rb+15, ranks 32 cp..32 cp+31
-// so every wave keeps its N_{L-2} and N_{L-1} brick accumulators in registers for the whole
-// range of o.  P is summed over the 4 row waves with LDS atomics.  The brick partials (FP32) are
-// written once per workgroup and summed in FP64 by k_sum_parts.
+//   waves 0..3: GEMM 1 on rows 16 w..16 w+15, all 64 ranks of the chunk (4 column blocks)
+//   waves 4..7: GEMM 2 on k    16 (w-4)..16 (w-4)+15, all 64 ranks
+// so no two waves read the same A operand, each wave runs 4 independent MFMA chains, every SIMD
+// holds one wave of each kind, and every wave keeps its brick accumulator (N_{L-2} rows or
+// N_{L-1} rows) in registers for the whole range of o.  P is summed over the 4 GEMM 1 waves with
+// LDS atomics.  The brick partials (FP32) are written once per workgroup and summed in FP64 by
+// k_sum_parts.
 //
 // Register fragment layouts of v_mfma_f32_16x16x32_bf16 (64 lanes, l = lane):
 //   A (16 x 32): row l % 16, eight k values in the slot group l / 16
@@ -45,7 +47,7 @@ namespace {
 constexpr int NT = 512;       // threads per workgroup (8 waves)
 constexpr int TB = 64;        // tile: 64 j x 64 k
 constexpr int RC = 64;        // ranks per launch
-constexpr int PD = 4;         // tiles prefetched ahead (even: the LDS buffer is (o - ob) & 1)
+constexpr int PD = 2;         // tiles prefetched ahead (even: the LDS buffer is (o - ob) & 1)
 constexpr int MAXOC = 128;    // o per workgroup at most (LDS Fo and P tables)
 constexpr int MAXO = TFX_MAXL - 2;
 constexpr int TP = 64;        // tile pitch in bf16 (rows are swizzled, see swz)
@@ -219,7 +221,8 @@ __global__ __launch_bounds__(NT, 1) void k_mttkrp_mfma(const float* __restrict__
   const int L = s.L, R = s.R;
   const int t = threadIdx.x;
   const int w = t >> 6, lane = t & 63, ln = lane & 15, rg = lane >> 4;
-  const int rb = w & 3, cp = w >> 2;
+  const bool g1 = w < 4;          // GEMM 1 wave (else GEMM 2)
+  const int sl = w & 3;           // its 16-row (GEMM 1) or 16-k (GEMM 2) slice of the tile
   // brick index: consecutive indices (the k-blocks of one (j-block, o range), k-block fastest)
   // go to one XCD
   const int per_xcd = P.nlaunch / NXCD;
@@ -233,36 +236,26 @@ __global__ __launch_bounds__(NT, 1) void k_mttkrp_mfma(const float* __restrict__
   const int64_t oe = ob + P.ochunk < P.NO ? ob + P.ochunk : P.NO;
   const int nO = (int)(oe - ob);
 
-  // this lane's two rank columns (cb = 0, 1) and their factor columns
-  int rr[2];
-  bool rok[2];
-  const double *Fk[2], *Fj[2];
+  // B fragments (all 64 ranks of the chunk, column block cb = ranks 16 cb..16 cb+15):
+  //   GEMM 1: F_{L-1}[k0 + 32 h + 8 rg + e, r],  GEMM 2: F_{L-2}[j0 + 32 h + 8 rg + e, r]
+  const int Lb = g1 ? L - 1 : L - 2;
+  const int64_t nb = g1 ? P.K : P.J, xb = g1 ? k0 : j0;
+  bf16x8 bh[2][4], bl[2][4];
+  float fjo[4][4];      // GEMM 1 waves: F_{L-2}[j0 + 16 sl + 4 rg + e, r] at their result rows
 #pragma unroll
-  for (int cb = 0; cb < 2; ++cb) {
-    rr[cb] = 32 * cp + 16 * cb + ln;
-    const int r = P.rc0 + rr[cb];
-    rok[cb] = r < R;
-    Fk[cb] = F + s.off[L - 1] + (int64_t)(rok[cb] ? r : 0) * P.K;
-    Fj[cb] = F + s.off[L - 2] + (int64_t)(rok[cb] ? r : 0) * P.J;
-  }
-  // GEMM 1 B: F_{L-1}[k0 + 32 ch + 8 rg + e, r];  GEMM 2 B: F_{L-2}[j0 + 32 sv + 8 rg + e, r]
-  bf16x8 fkh[2][2], fkl[2][2], fjh[2][2], fjl[2][2];
+  for (int cb = 0; cb < 4; ++cb) {
+    const int r = P.rc0 + 16 * cb + ln;
+    const bool rok = r < R;
+    const double* col = F + s.off[Lb] + (int64_t)(rok ? r : 0) * nb;
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      factor_frag(Fk[cb], P.K, k0 + 32 * h + 8 * rg, rok[cb], fkh[h][cb], fkl[h][cb]);
-      factor_frag(Fj[cb], P.J, j0 + 32 * h + 8 * rg, rok[cb], fjh[h][cb], fjl[h][cb]);
-    }
-  // F_{L-2} at the rows of this lane's GEMM 1 results: j0 + 16 rb + 4 rg + e
-  float fjo[2][4];
-#pragma unroll
-  for (int cb = 0; cb < 2; ++cb)
+    for (int h = 0; h < 2; ++h) factor_frag(col, nb, xb + 32 * h + 8 * rg, rok, bh[h][cb], bl[h][cb]);
+    const double* colj = F + s.off[L - 2] + (int64_t)(rok ? r : 0) * P.J;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int64_t j = j0 + 16 * rb + 4 * rg + e;
-      fjo[cb][e] = (float)load_or0(Fj[cb], j, rok[cb] && j < P.J);
+      const int64_t j = j0 + 16 * sl + 4 * rg + e;
+      fjo[cb][e] = (float)load_or0(colj, j, g1 && rok && j < P.J);
     }
+  }
   // Fo over the o range (order 3: one factor entry, order 2: 1), P accumulators
   constexpr int FOU = MAXOC * RC / NT;
   if (L <= 3) {
@@ -307,27 +300,29 @@ __global__ __launch_bounds__(NT, 1) void k_mttkrp_mfma(const float* __restrict__
   }
   __syncthreads();
 
-
-  f32x4 nacc[2], pacc[2];
+  // brick accumulator: N_{L-2} rows (GEMM 1 waves) or N_{L-1} rows (GEMM 2 waves),
+  // lane holds rows 16 sl + 4 rg + e of rank column 16 cb + ln
+  f32x4 bacc[4];
 #pragma unroll
-  for (int cb = 0; cb < 2; ++cb) {
-    nacc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    pacc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int cb = 0; cb < 4; ++cb) bacc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // LDS offsets (elements) of this lane's A operand reads in buffer 0: GEMM 1 rows
+  // 16 sl + ln (16-byte reads), GEMM 2 columns k = 16 sl + lane % 16 (transposing reads)
+  int a_off[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (g1) {
+      a_off[h][0] = swz(16 * sl + ln, 4 * h + rg);
+      a_off[h][1] = 0;
+    } else {
+      tr_offsets(32 * h, 16 * sl, lane, a_off[h]);
+    }
   }
+  typedef __attribute__((address_space(3))) bf16x4 lds_bf16x4;
 
   // Tile o travels through register slot q = (o - ob) % PD, refilled with tile o + PD right
   // after it is stored.  Full groups of PD tiles run in a loop without early exits (so that the
   // compiler's count of loads in flight carries across its back edge); the rest follows.
-  // LDS offsets (elements) of this lane's operand reads in buffer 0 (buffer 1 adds 2 TB TP)
-  const int r1 = 16 * rb + ln;
-  int a1_off[2], a2_off[2][2];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    a1_off[h] = swz(r1, 4 * h + rg);
-    tr_offsets(32 * h, 16 * rb, lane, a2_off[h]);
-  }
-  typedef __attribute__((address_space(3))) bf16x4 lds_bf16x4;
-
   auto tile = [&](const int q, const int64_t o) __attribute__((always_inline)) {
       const int buf = q & 1;
       store_tile(rm, buf, pre[q]);
@@ -336,62 +331,55 @@ __global__ __launch_bounds__(NT, 1) void k_mttkrp_mfma(const float* __restrict__
       // buffer was last read before this barrier
       __syncthreads();
       const int oo = (int)(o - ob);
-      float fo[2];
+      float fo[4];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) fo[cb] = fo_t[oo * RC + rr[cb]];
-      // A operands: GEMM 1 rows (16-byte reads), GEMM 2 columns (transposing reads); rows and
-      // columns outside the tensor are zeros in LDS
+      for (int cb = 0; cb < 4; ++cb) fo[cb] = fo_t[oo * RC + 16 * cb + ln];
+      // A operands (rows and columns outside the tensor are zeros in LDS)
       const __bf16* th = rm + (buf * 2 + 0) * TB * TP;
       const __bf16* tl = rm + (buf * 2 + 1) * TB * TP;
-      bf16x8 a1h[2], a1l[2], a2h[2], a2l[2];
+      bf16x8 ah[2], al[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        a1h[h] = *(const bf16x8*)(th + a1_off[h]);
-        a1l[h] = *(const bf16x8*)(tl + a1_off[h]);
-        a2h[h] = __builtin_shufflevector(
-            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(th + a2_off[h][0])),
-            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(th + a2_off[h][1])),
-            0, 1, 2, 3, 4, 5, 6, 7);
-        a2l[h] = __builtin_shufflevector(
-            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(tl + a2_off[h][0])),
-            __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(tl + a2_off[h][1])),
-            0, 1, 2, 3, 4, 5, 6, 7);
+        if (g1) {
+          ah[h] = *(const bf16x8*)(th + a_off[h][0]);
+          al[h] = *(const bf16x8*)(tl + a_off[h][0]);
+        } else {
+          ah[h] = __builtin_shufflevector(
+              __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(th + a_off[h][0])),
+              __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(th + a_off[h][1])),
+              0, 1, 2, 3, 4, 5, 6, 7);
+          al[h] = __builtin_shufflevector(
+              __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(tl + a_off[h][0])),
+              __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(tl + a_off[h][1])),
+              0, 1, 2, 3, 4, 5, 6, 7);
+        }
       }
-      // 4 independent accumulator chains (y[0], y[1], z[0], z[1]) interleaved, so that a
-      // dependent MFMA issues 4 instructions after its predecessor
-      f32x4 y[2], z[2];
+      // 4 independent accumulator chains (one per column block), interleaved
+      f32x4 acc[4];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
-        y[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        z[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-      }
+      for (int cb = 0; cb < 4; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int term = 0; term < 3; ++term) {
-          const bf16x8 xa1 = term == 2 ? a1l[h] : a1h[h];
-          const bf16x8 xa2 = term == 2 ? a2l[h] : a2h[h];
+        for (int term = 0; term < 3; ++term)
 #pragma unroll
-          for (int cb = 0; cb < 2; ++cb) {
-            const bf16x8 xb1 = term == 1 ? fkl[h][cb] : fkh[h][cb];
-            const bf16x8 xb2 = term == 1 ? fjl[h][cb] : fjh[h][cb];
-            y[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa1, xb1, y[cb], 0, 0, 0);
-            z[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa2, xb2, z[cb], 0, 0, 0);
-          }
+          for (int cb = 0; cb < 4; ++cb)
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(term == 2 ? al[h] : ah[h],
+                                                              term == 1 ? bl[h][cb] : bh[h][cb],
+                                                              acc[cb], 0, 0, 0);
+      // Fo scale into the brick accumulator; GEMM 1 waves also add P over their 16 rows (lane
+      // rows summed in VALU) to the LDS table
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bacc[cb][e] += fo[cb] * acc[cb][e];
+        if (g1) {
+          float p = 0.f;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) p += fjo[cb][e] * acc[cb][e];
+          p = sum_rows4(p);
+          if (rg == 0) atomicAdd(&po_t[oo * RC + 16 * cb + ln], p);
         }
-      // Fo scales; N_{L-2} (pacc) and N_{L-1} (nacc) brick accumulators; P over this wave's 16
-      // rows (lane rows summed in VALU), added to the LDS table by one lane row
-#pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
-        float p = 0.f;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          pacc[cb][e] += fo[cb] * y[cb][e];
-          nacc[cb][e] += fo[cb] * z[cb][e];
-          p += fjo[cb][e] * y[cb][e];
-        }
-        p = sum_rows4(p);
-        if (rg == 0) atomicAdd(&po_t[oo * RC + rr[cb]], p);
       }
   };
   const int64_t ofull = ob + (int64_t)(nO / PD) * PD;
@@ -404,16 +392,17 @@ __global__ __launch_bounds__(NT, 1) void k_mttkrp_mfma(const float* __restrict__
     if (ofull + q < oe) tile(q, ofull + q);
   __syncthreads();
 
-  // brick partials: lane holds rows 16 rb + 4 rg + e of its two rank columns
+  // brick partials: lane holds rows 16 sl + 4 rg + e of rank columns 16 cb + ln
 #pragma unroll
-  for (int cb = 0; cb < 2; ++cb)
+  for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int64_t x = 16 * rb + 4 * rg + e;
-      if (k0 + x < P.K)
-        Pk[(((int64_t)jb * P.osplit + oc) * P.K + k0 + x) * RC + rr[cb]] = nacc[cb][e];
-      if (j0 + x < P.J)
-        Pj[(((int64_t)kb * P.osplit + oc) * P.J + j0 + x) * RC + rr[cb]] = pacc[cb][e];
+      const int64_t x = 16 * sl + 4 * rg + e;
+      const int rr = 16 * cb + ln;
+      if (!g1 && k0 + x < P.K)
+        Pk[(((int64_t)jb * P.osplit + oc) * P.K + k0 + x) * RC + rr] = bacc[cb][e];
+      if (g1 && j0 + x < P.J)
+        Pj[(((int64_t)kb * P.osplit + oc) * P.J + j0 + x) * RC + rr] = bacc[cb][e];
     }
   if (L > 2)
     for (int e = t; e < nO * RC; e += NT)
