@@ -1,0 +1,429 @@
+// Split-preconditioned CG (GaussNewton.py:315-398) for problems whose operands do not fit the
+// LDS of one workgroup: the refinement problem on the full tensor and large compressed cores.
+// Every CG iteration is three stream-ordered kernels over many workgroups:
+//   K1  p_j = r + beta (j == 1 ? r : p_{j-1})  (j >= 1, first iteration: p_0 = r), and
+//       A_m = q_m^T F_m with q = M p_j (split over i, FP64 partials per split)
+//   K2  z_l = M (F_l tmp_l + q_l P1_l + damp Gamma q_l),  tmp_l = sum_{m != l} P2[l][m] o A_m,
+//       and the dot p_j . z, whose last arriving workgroup forms alpha = rn / (p . z)
+//   K3  y += alpha p_j, r -= alpha z, and r . r, whose last workgroup forms beta, rn and the
+//       stop flag (rn <= tol)
+// Scalars pass between kernels through a small device block; a dot product is finished by the
+// last workgroup to arrive (atomic counter after a release fence; no workgroup ever waits for
+// another).  Kernels launched after the stop flag is set return at once, so the host launches
+// maxiter iterations without reading anything back.  Dots and norms are FP64; vectors are in
+// the working type S.  J^T J grad (for the dogleg) is the same K1/K2 pair without M and damping.
+#include "common.h"
+
+namespace {
+
+constexpr int CT = 16;        // c / r tile of the R x R products
+constexpr int ICH = 128;      // i per workgroup in K1 (split of the reduction over I_m)
+constexpr int KI = 32;        // i staged per step in K1
+constexpr int ZI = 64;        // rows i per workgroup in K2
+constexpr int ZC = 64;        // c per LDS chunk in K2
+constexpr int NTH = 256;
+
+// device scalar block (double) and counters (unsigned)
+enum { SC_RN = 0, SC_ALPHA, SC_BETA, SC_DONE, SC_ITN, SC_N };
+enum { CN_K0 = 0, CN_K2, CN_K3, CN_N };
+
+struct CgPlan {
+  Shape s;
+  int nrb;                       // r (and c) blocks of 16
+  int nsplit[TFX_MAXL];          // K1 splits of I_m
+  int k1first[TFX_MAXL + 1];     // first K1 block of each mode
+  int k2first[TFX_MAXL + 1];     // first K2 block of each mode
+  int64_t aoff[TFX_MAXL];        // offset of mode m's partials in A_part: [split][c][r]
+  double damp, tol;
+};
+
+template <typename S>
+struct CgBufs {
+  const S* F;
+  const S* P1;
+  const S* P2;
+  const S* gamma;
+  const S* grad;
+  S* r;
+  S* p[2];
+  S* z;
+  S* yc;
+  double* A;        // A_part
+  double* part;     // per-workgroup partial sums
+  double* sc;       // scalar block
+  unsigned* cnt;    // counters
+};
+
+__device__ inline int mode_of_e(const Shape& s, int64_t e, int64_t* within) {
+  int l = 0;
+  while (l + 1 < s.L && e >= s.off[l + 1]) ++l;
+  *within = e - s.off[l];
+  return l;
+}
+
+template <typename S>
+__device__ inline double inv_m(S g, double damp) {
+  const double gd = (double)g;
+  return 1.0 / sqrt(gd * gd + damp * damp * (gd * gd));
+}
+
+__device__ inline double wg_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+  return t;
+}
+
+// Publish this workgroup's partial and report whether it is the last to arrive; the last one
+// sees every partial (release fence before the counter, acquire fence after it).
+__device__ inline bool arrive_last(double v, double* part, unsigned* counter, bool* flag) {
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&part[blockIdx.x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    *flag = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  const bool last = *flag;
+  if (last) __threadfence();
+  return last;
+}
+
+// sum of the partials in index order (same result whichever workgroup came last)
+__device__ inline double sum_parts(const double* part, int n) {
+  double t = 0.0;
+  for (int i = 0; i < n; ++i) t += __hip_atomic_load(&part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return t;
+}
+
+// K0: r = M grad, p_0 = r, y = 0, rn = r . r (1e-6 when zero).
+template <typename S>
+__global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
+  __shared__ double red[NTH / 64];
+  __shared__ bool last;
+  const Shape& s = P.s;
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
+    int64_t w;
+    const int l = mode_of_e(s, e, &w);
+    const int rk = (int)(w / s.d[l]);
+    const double M = inv_m(B.gamma[l * s.R + rk], P.damp);
+    const S rr = (S)(M * (double)B.grad[e]);
+    B.r[e] = rr;
+    B.p[0][e] = rr;
+    B.yc[e] = S(0);
+    acc += (double)rr * (double)rr;
+  }
+  const double v = wg_sum(acc, red);
+  if (arrive_last(v, B.part, &B.cnt[CN_K0], &last) && threadIdx.x == 0) {
+    double rn = sum_parts(B.part, gridDim.x);
+    if (rn == 0.0) rn = 1e-6;
+    B.sc[SC_RN] = rn;
+    B.cnt[CN_K0] = 0;
+  }
+}
+
+// K1: A_part[m][split][c][r] = sum_{i in split} q_m[i][c] F_m[i][r].
+//   raw: q = grad (J^T J grad); else q = M p_j with p_j formed here (j >= 1) and written by the
+//   r-block-0 workgroups.  Block 0 of the raw pass also clears the counters and the stop flag.
+template <typename S>
+__global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, int j) {
+  __shared__ double qs[KI][CT + 1];
+  __shared__ double fs[KI][CT + 1];
+  const Shape& s = P.s;
+  if (raw && blockIdx.x == 0 && threadIdx.x < CN_N) B.cnt[threadIdx.x] = 0u;
+  if (raw && blockIdx.x == 0 && threadIdx.x == 0) {
+    B.sc[SC_DONE] = 0.0;
+    B.sc[SC_ITN] = 0.0;
+  }
+  if (!raw && B.sc[SC_DONE] != 0.0) return;
+  int m = 0;
+  while (m + 1 < s.L && (int)blockIdx.x >= P.k1first[m + 1]) ++m;
+  const int nrb = P.nrb;
+  const int local = (int)blockIdx.x - P.k1first[m];
+  const int split = local / (nrb * nrb);
+  const int cb = (local / nrb) % nrb, rbk = local % nrb;
+  const int c0 = cb * CT, r0 = rbk * CT;
+  const int64_t I = s.d[m];
+  const int64_t ib = (int64_t)split * ICH;
+  const int64_t ie = ib + ICH < I ? ib + ICH : I;
+  const int R = s.R;
+  const S* Fm = B.F + s.off[m];
+  const int64_t off = s.off[m];
+  const double beta = (!raw && j >= 1) ? B.sc[SC_BETA] : 0.0;
+  const S* pold = B.p[(j + 1) & 1];
+  S* pnew = B.p[j & 1];
+  const bool writer = !raw && j >= 1 && rbk == 0;
+  const int t = threadIdx.x;
+  const int tc = t >> 4, tr = t & 15;
+  double acc = 0.0;
+  for (int64_t i0 = ib; i0 < ie; i0 += KI) {
+    // stage q[i0..i0+31][c0..c0+15] and F[i0..i0+31][r0..r0+15] (coalesced along i)
+    for (int u = t; u < KI * CT; u += NTH) {
+      const int cc = u / KI, ii = u % KI;
+      const int64_t i = i0 + ii;
+      double qv = 0.0, fv = 0.0;
+      if (i < ie) {
+        const int c = c0 + cc;
+        if (c < R) {
+          const int64_t e = off + (int64_t)c * I + i;
+          if (raw) {
+            qv = (double)B.grad[e];
+          } else {
+            S pv;
+            if (j == 0) {
+              pv = B.p[0][e];
+            } else {
+              const S rv = B.r[e];
+              pv = (S)((double)rv + beta * (double)(j == 1 ? rv : pold[e]));
+              if (writer) pnew[e] = pv;
+            }
+            qv = inv_m(B.gamma[m * R + c], P.damp) * (double)pv;
+          }
+        }
+        const int rr = r0 + cc;
+        if (rr < R) fv = (double)Fm[(int64_t)rr * I + i];
+      }
+      qs[ii][cc] = qv;
+      fs[ii][cc] = fv;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int ii = 0; ii < KI; ++ii) acc += qs[ii][tc] * fs[ii][tr];
+    __syncthreads();
+  }
+  const int c = c0 + tc, rr = r0 + tr;
+  if (c < R && rr < R) B.A[P.aoff[m] + ((int64_t)split * R + c) * R + rr] = acc;
+}
+
+// K2: z_l[i][r] for rows i0..i0+63 and ranks r0..r0+15 of mode l; c in chunks of ZC.
+template <typename S>
+__global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, int j, S* out) {
+  __shared__ double Fs[ZI][ZC + 1];
+  __shared__ double Qs[ZI][ZC + 1];
+  __shared__ double Ts[ZC][CT];
+  __shared__ double P1s[ZC][CT];
+  __shared__ double red[NTH / 64];
+  __shared__ bool last;
+  const Shape& s = P.s;
+  if (!raw && B.sc[SC_DONE] != 0.0) return;
+  int l = 0;
+  while (l + 1 < s.L && (int)blockIdx.x >= P.k2first[l + 1]) ++l;
+  const int nrb = P.nrb;
+  const int local = (int)blockIdx.x - P.k2first[l];
+  const int ibk = local / nrb, rbk = local % nrb;
+  const int64_t I = s.d[l];
+  const int64_t i0 = (int64_t)ibk * ZI;
+  const int r0 = rbk * CT;
+  const int R = s.R, L = s.L;
+  const int64_t RR = (int64_t)R * R;
+  const int64_t off = s.off[l];
+  const S* pj = B.p[j & 1];
+  const int t = threadIdx.x;
+  const int tr = t & 15, ig = t >> 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int cc0 = 0; cc0 < R; cc0 += ZC) {
+    // tmp_l[c][r] = sum_{m != l} P2[l][m][c][r] sum_split A_m[split][c][r], and P1_l
+    for (int u = t; u < ZC * CT; u += NTH) {
+      const int cc = u / CT, rq = u % CT;
+      const int c = cc0 + cc, rr = r0 + rq;
+      double tv = 0.0, pv = 0.0;
+      if (c < R && rr < R) {
+        for (int m = 0; m < L; ++m) {
+          if (m == l) continue;
+          double a = 0.0;
+          for (int sp = 0; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + ((int64_t)sp * R + c) * R + rr];
+          tv += (double)B.P2[((int64_t)l * L + m) * RR + (int64_t)c * R + rr] * a;
+        }
+        pv = (double)B.P1[(int64_t)l * RR + (int64_t)c * R + rr];
+      }
+      Ts[cc][rq] = tv;
+      P1s[cc][rq] = pv;
+    }
+    // F_l and q_l rows i0..i0+63, columns cc0..cc0+63 (coalesced along i)
+    for (int u = t; u < ZI * ZC; u += NTH) {
+      const int cc = u / ZI, ii = u % ZI;
+      const int64_t i = i0 + ii;
+      const int c = cc0 + cc;
+      double fv = 0.0, qv = 0.0;
+      if (i < I && c < R) {
+        const int64_t e = off + (int64_t)c * I + i;
+        fv = (double)B.F[e];
+        qv = raw ? (double)B.grad[e] : inv_m(B.gamma[l * R + c], P.damp) * (double)pj[e];
+      }
+      Fs[ii][cc] = fv;
+      Qs[ii][cc] = qv;
+    }
+    __syncthreads();
+    const int cn = R - cc0 < ZC ? R - cc0 : ZC;
+    for (int cc = 0; cc < cn; ++cc) {
+      const double tv = Ts[cc][tr], pv = P1s[cc][tr];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] += Fs[ig + 16 * a][cc] * tv + Qs[ig + 16 * a][cc] * pv;
+    }
+    __syncthreads();
+  }
+  double dot = 0.0;
+  const int rr = r0 + tr;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + ig + 16 * a;
+    if (i < I && rr < R) {
+      const int64_t e = off + (int64_t)rr * I + i;
+      if (raw) {
+        out[e] = (S)acc[a];
+      } else {
+        const double g = (double)B.gamma[l * R + rr];
+        const double M = inv_m(B.gamma[l * R + rr], P.damp);
+        const double q = M * (double)pj[e];
+        const S zv = (S)(M * (acc[a] + P.damp * g * q));
+        out[e] = zv;
+        dot += (double)pj[e] * (double)zv;
+      }
+    }
+  }
+  if (raw) return;
+  const double v = wg_sum(dot, red);
+  if (arrive_last(v, B.part, &B.cnt[CN_K2], &last) && threadIdx.x == 0) {
+    double den = sum_parts(B.part, gridDim.x);
+    if (den == 0.0) den = 1e-6;
+    B.sc[SC_ALPHA] = B.sc[SC_RN] / den;
+    B.cnt[CN_K2] = 0;
+  }
+}
+
+// K3: y += alpha p_j, r -= alpha z, rn_new = r . r; beta = rn_new / rn; stop when rn <= tol.
+template <typename S>
+__global__ __launch_bounds__(NTH) void k_cgm_upd(CgPlan P, CgBufs<S> B, int j) {
+  __shared__ double red[NTH / 64];
+  __shared__ bool last;
+  const Shape& s = P.s;
+  if (B.sc[SC_DONE] != 0.0) return;
+  const S aS = (S)B.sc[SC_ALPHA];
+  const S* pj = B.p[j & 1];
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
+    B.yc[e] += aS * pj[e];
+    const S rv = B.r[e] - aS * B.z[e];
+    B.r[e] = rv;
+    acc += (double)rv * (double)rv;
+  }
+  const double v = wg_sum(acc, red);
+  if (arrive_last(v, B.part, &B.cnt[CN_K3], &last) && threadIdx.x == 0) {
+    const double rn_new = sum_parts(B.part, gridDim.x);
+    B.sc[SC_BETA] = rn_new / B.sc[SC_RN];
+    B.sc[SC_RN] = rn_new;
+    if (rn_new <= P.tol) {
+      B.sc[SC_DONE] = 1.0;
+      B.sc[SC_ITN] = (double)j;
+    }
+    B.cnt[CN_K3] = 0;
+  }
+}
+
+// y_out = M y; scal = [iterations done, final residual norm^2]
+template <typename S>
+__global__ __launch_bounds__(NTH) void k_cgm_end(CgPlan P, CgBufs<S> B, int maxiter, S* y,
+                                                 double* scal) {
+  const Shape& s = P.s;
+  for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
+    int64_t w;
+    const int l = mode_of_e(s, e, &w);
+    const int rk = (int)(w / s.d[l]);
+    y[e] = (S)(inv_m(B.gamma[l * s.R + rk], P.damp) * (double)B.yc[e]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int itn = B.sc[SC_DONE] != 0.0 ? (int)B.sc[SC_ITN] : maxiter - 1;
+    scal[0] = (double)(itn + 1);
+    scal[1] = B.sc[SC_RN];
+  }
+}
+
+inline unsigned nblocks_vec(int64_t n) {
+  int64_t b = (n + NTH * 8 - 1) / (NTH * 8);
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+// Multi-kernel CG; same arguments and results as the single-workgroup kernels of tfx_cg.
+template <typename S>
+int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, const S* gamma,
+               double damp, const S* grad, int maxiter, double tol, S* y, S* jtjg, double* scal,
+               hipStream_t st) {
+  CgPlan P;
+  P.s = s;
+  P.damp = damp;
+  P.tol = tol;
+  P.nrb = (s.R + CT - 1) / CT;
+  int k1 = 0, k2 = 0;
+  int64_t aoff = 0;
+  for (int m = 0; m < s.L; ++m) {
+    P.nsplit[m] = (int)((s.d[m] + ICH - 1) / ICH);
+    P.k1first[m] = k1;
+    P.k2first[m] = k2;
+    P.aoff[m] = aoff;
+    k1 += P.nsplit[m] * P.nrb * P.nrb;
+    k2 += (int)((s.d[m] + ZI - 1) / ZI) * P.nrb;
+    aoff += (int64_t)P.nsplit[m] * s.R * s.R;
+  }
+  P.k1first[s.L] = k1;
+  P.k2first[s.L] = k2;
+  const unsigned nbv = nblocks_vec(s.n);
+  const int npart = std::max<int>(k2, (int)nbv);
+  const size_t vec = align256(sizeof(S) * (size_t)s.n);
+  const size_t bytes = 5 * vec + align256(sizeof(double) * (size_t)aoff) +
+                       align256(sizeof(double) * (size_t)npart) + align256(sizeof(double) * SC_N) +
+                       align256(sizeof(unsigned) * CN_N);
+  void* wsp;
+  int rc = tfx_workspace(ctx, bytes + 256, st, &wsp);
+  if (rc) return rc;
+  char* w = (char*)wsp;
+  CgBufs<S> B;
+  B.F = F;
+  B.P1 = P1;
+  B.P2 = P2;
+  B.gamma = gamma;
+  B.grad = grad;
+  B.r = (S*)w; w += vec;
+  B.p[0] = (S*)w; w += vec;
+  B.p[1] = (S*)w; w += vec;
+  B.z = (S*)w; w += vec;
+  B.yc = (S*)w; w += vec;
+  B.A = (double*)w; w += align256(sizeof(double) * (size_t)aoff);
+  B.part = (double*)w; w += align256(sizeof(double) * (size_t)npart);
+  B.sc = (double*)w; w += align256(sizeof(double) * SC_N);
+  B.cnt = (unsigned*)w;
+
+  // J^T J grad for the dogleg (also clears counters and the stop flag)
+  hipLaunchKernelGGL(k_cgm_A<S>, dim3(k1), dim3(NTH), 0, st, P, B, 1, 0);
+  TFX_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_cgm_Z<S>, dim3(k2), dim3(NTH), 0, st, P, B, 1, 0, jtjg);
+  TFX_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_cgm_start<S>, dim3(nbv), dim3(NTH), 0, st, P, B);
+  TFX_CHECK_LAUNCH(ctx);
+  for (int jj = 0; jj < maxiter; ++jj) {
+    hipLaunchKernelGGL(k_cgm_A<S>, dim3(k1), dim3(NTH), 0, st, P, B, 0, jj);
+    TFX_CHECK_LAUNCH(ctx);
+    hipLaunchKernelGGL(k_cgm_Z<S>, dim3(k2), dim3(NTH), 0, st, P, B, 0, jj, B.z);
+    TFX_CHECK_LAUNCH(ctx);
+    hipLaunchKernelGGL(k_cgm_upd<S>, dim3(nbv), dim3(NTH), 0, st, P, B, jj);
+    TFX_CHECK_LAUNCH(ctx);
+  }
+  hipLaunchKernelGGL(k_cgm_end<S>, dim3(nbv), dim3(NTH), 0, st, P, B, maxiter, y, scal);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+template int cg_multi_t<float>(tfx_ctx*, const Shape&, const float*, const float*, const float*,
+                               const float*, double, const float*, int, double, float*, float*,
+                               double*, hipStream_t);
+template int cg_multi_t<double>(tfx_ctx*, const Shape&, const double*, const double*,
+                                const double*, const double*, double, const double*, int, double,
+                                double*, double*, double*, hipStream_t);

@@ -6,6 +6,12 @@
 // Flat layout (x, grad, y, factors): element (i, r) of mode l at off_l + r*I_l + i.
 #include "common.h"
 
+// multi-kernel CG (cg_multi.hip) for problems whose operands do not fit one workgroup's LDS
+template <typename S>
+int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, const S* gamma,
+               double damp, const S* grad, int maxiter, double tol, S* y, S* jtjg, double* scal,
+               hipStream_t st);
+
 namespace {
 
 __device__ inline int mode_of(const Shape& s, int64_t e, int64_t* within) {
@@ -93,158 +99,10 @@ __global__ void k_neg_grad(Shape s, const S* __restrict__ F, const S* __restrict
   g[e] = N[e] - acc;
 }
 
-// (J^T J) v for the flat vector v (GaussNewton.py:342-346, 551-581):
-//   A_m[c][r] = sum_i v_m[i, c] F_m[i, r]
-//   tmp_l     = sum_{m != l} P2[l][m] o A_m
-//   out_l[i, r] = sum_c F_l[i, c] tmp_l[c][r] + sum_c v_l[i, c] P1_l[c][r]
-// Runs inside one workgroup; the caller synchronizes before reading `out`.
-template <typename S>
-__device__ void wg_matvec(const Shape& s, const S* __restrict__ F, const S* __restrict__ P1,
-                          const S* __restrict__ P2, const S* v, S* A, S* tmp, S* out) {
-  const int L = s.L, R = s.R;
-  const int64_t RR = (int64_t)R * R;
-  const int nt = blockDim.x;
-  for (int64_t e = threadIdx.x; e < (int64_t)L * RR; e += nt) {
-    const int m = (int)(e / RR);
-    const int c = (int)((e % RR) / R), r = (int)(e % R);
-    const int64_t I = s.d[m];
-    const S* vm = v + s.off[m] + (int64_t)c * I;
-    const S* Fm = F + s.off[m] + (int64_t)r * I;
-    S acc = S(0);
-    for (int64_t i = 0; i < I; ++i) acc += vm[i] * Fm[i];
-    A[e] = acc;
-  }
-  __syncthreads();
-  for (int64_t e = threadIdx.x; e < (int64_t)L * RR; e += nt) {
-    const int l = (int)(e / RR);
-    const int64_t rc = e % RR;
-    S acc = S(0);
-    for (int m = 0; m < L; ++m)
-      if (m != l) acc += P2[((int64_t)l * L + m) * RR + rc] * A[m * RR + rc];
-    tmp[e] = acc;
-  }
-  __syncthreads();
-  for (int64_t e = threadIdx.x; e < s.n; e += nt) {
-    int64_t w;
-    const int l = mode_of(s, e, &w);
-    const int64_t I = s.d[l];
-    const int r = (int)(w / I);
-    const int64_t i = w % I;
-    const S* Fl = F + s.off[l];
-    const S* vl = v + s.off[l];
-    const S* tl = tmp + l * RR;
-    const S* pl = P1 + l * RR;
-    S acc = S(0);
-    for (int c = 0; c < R; ++c) acc += Fl[(int64_t)c * I + i] * tl[(int64_t)c * R + r] + vl[(int64_t)c * I + i] * pl[(int64_t)c * R + r];
-    out[e] = acc;
-  }
-}
-
-template <typename S>
-__device__ inline S gamma_of(const Shape& s, const S* gamma, int64_t e) {
-  int64_t w;
-  const int l = mode_of(s, e, &w);
-  return gamma[l * s.R + (int)(w / s.d[l])];
-}
-
-// Persistent split-preconditioned CG in one workgroup (GaussNewton.py:315-398).
-template <typename S>
-__global__ __launch_bounds__(1024) void k_cg(Shape s, const S* __restrict__ F,
-                                              const S* __restrict__ P1, const S* __restrict__ P2,
-                                              const S* __restrict__ gamma, double damp,
-                                              const S* __restrict__ grad, int maxiter, double tol,
-                                              S* __restrict__ y, S* __restrict__ jtjg,
-                                              double* __restrict__ scal, S* ws, int tmp_in_lds) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[16];
-  const int64_t n = s.n;
-  const int64_t LRR = (int64_t)s.L * s.R * s.R;
-  S* rv = ws;            // residual
-  S* pv = rv + n;        // search direction
-  S* qv = pv + n;        // M p
-  S* zv = qv + n;        // M (J^T J + damp Gamma) M p
-  S* yc = zv + n;        // CG iterate
-  S* A = yc + n;
-  S* tmp = tmp_in_lds ? (S*)smem : A + LRR;
-  const S dampS = (S)damp;
-  const int t = threadIdx.x, nt = blockDim.x;
-
-  if (n > 100) {
-    if ((int64_t)maxiter > n) maxiter = (int)n;
-  } else {
-    tol = 0.0;
-  }
-  // J^T J grad, kept for the dogleg fallback
-  wg_matvec<S>(s, F, P1, P2, grad, A, tmp, jtjg);
-  __syncthreads();
-  double acc = 0.0;
-  for (int64_t e = t; e < n; e += nt) {
-    const S g = gamma_of(s, gamma, e);
-    const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
-    const S rr = M * grad[e];
-    rv[e] = rr;
-    pv[e] = rr;
-    yc[e] = S(0);
-    acc += (double)rr * (double)rr;
-  }
-  double rn = block_sum(acc, red);
-  if (rn == 0.0) rn = 1e-6;
-  int itn = 0;
-  for (itn = 0; itn < maxiter; ++itn) {
-    for (int64_t e = t; e < n; e += nt) {
-      const S g = gamma_of(s, gamma, e);
-      const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
-      qv[e] = M * pv[e];
-    }
-    __syncthreads();
-    wg_matvec<S>(s, F, P1, P2, qv, A, tmp, zv);
-    // z = M (z + damp Gamma q); each thread finishes its own elements
-    acc = 0.0;
-    for (int64_t e = t; e < n; e += nt) {
-      const S g = gamma_of(s, gamma, e);
-      const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
-      const S z = M * (zv[e] + dampS * g * qv[e]);
-      zv[e] = z;
-      acc += (double)pv[e] * (double)z;
-    }
-    double den = block_sum(acc, red);
-    if (den == 0.0) den = 1e-6;
-    const double alpha = rn / den;
-    const S aS = (S)alpha;
-    acc = 0.0;
-    for (int64_t e = t; e < n; e += nt) {
-      yc[e] += aS * pv[e];
-      const S r = rv[e] - aS * zv[e];
-      rv[e] = r;
-      acc += (double)r * (double)r;
-    }
-    const double rn_new = block_sum(acc, red);
-    const double beta = rn_new / rn;
-    rn = rn_new;
-    const S bS = (S)beta;
-    for (int64_t e = t; e < n; e += nt) {
-      // first iteration: P aliases the residual array in the reference, so P_old == r_new
-      const S pold = itn == 0 ? rv[e] : pv[e];
-      pv[e] = rv[e] + bS * pold;
-    }
-    if (rn <= tol) break;
-  }
-  if (itn == maxiter) itn = maxiter - 1;  // loop variable of the reference after a full run
-  for (int64_t e = t; e < n; e += nt) {
-    const S g = gamma_of(s, gamma, e);
-    const S M = S(1) / sqrt(g * g + dampS * dampS * (g * g));
-    y[e] = M * yc[e];
-  }
-  if (t == 0) {
-    scal[0] = (double)(itn + 1);
-    scal[1] = rn;
-  }
-}
-
 // ---------------------------------------------------------------------------------------------
-// CG for small problems (the compressed core): one workgroup, every vector held in registers as
-// 4 x 4 tiles (rows i, ranks r) of one mode, F / q / tmp / P1 staged row-major in LDS (or global
-// memory when they do not fit).  Same arithmetic as k_cg, with the matvec GEMMs register-tiled.
+// CG for small problems (GaussNewton.py:315-398, 551-581): one workgroup, every vector held in
+// registers as 4 x 4 tiles (rows i, ranks r) of one mode, F / q / tmp / P1 staged row-major in
+// LDS, the matvec GEMMs register-tiled.  Larger problems use the multi-kernel CG (cg_multi.hip).
 // ---------------------------------------------------------------------------------------------
 constexpr int SMALL_NT = 512;
 
@@ -652,44 +510,28 @@ int cg_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, con
   tm.first[s.L] = nt;
   tm.ntile = nt;
   if (nt <= SMALL_NT && s.L <= 4) {
-    const int64_t lens[4] = {s.n, s.n, LRR, LRR};
-    unsigned mask = 0;
-    size_t lds = 0, gbytes = 0;
-    for (int b = 0; b < 4; ++b) {
-      const size_t bytes = sizeof(S) * (size_t)lens[b];
-      if (lds + bytes <= 150 * 1024) {
-        mask |= 1u << b;
-        lds += bytes;
-      } else {
-        gbytes += bytes;
-      }
+    // q, F, tmp and P1 must all sit in LDS: from global memory the register-tiled loops wait on
+    // every load
+    const size_t lds = sizeof(S) * (size_t)(2 * s.n + 2 * LRR);
+    if (lds <= 150 * 1024) {
+      void* wsp = nullptr;
+      int rc = tfx_workspace(ctx, sizeof(S) * (size_t)s.n + 256, st, &wsp);
+      if (rc) return rc;
+      TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_small<S>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_cg_small<S>, dim3(1), dim3(SMALL_NT), lds, st, s, tm, F, P1, P2, gamma,
+                         damp, grad, maxiter, tol, y, jtjg, scal, (S*)wsp, 0xfu);
+      TFX_CHECK_LAUNCH(ctx);
+      return TFX_OK;
     }
-    gbytes += sizeof(S) * (size_t)s.n;   // the CG iterate
-    void* wsp = nullptr;
-    int rc = tfx_workspace(ctx, gbytes + 256, st, &wsp);
-    if (rc) return rc;
-    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_small<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(lds > 0 ? lds : 1)));
-    hipLaunchKernelGGL(k_cg_small<S>, dim3(1), dim3(SMALL_NT), lds, st, s, tm, F, P1, P2, gamma,
-                       damp, grad, maxiter, tol, y, jtjg, scal, (S*)wsp, mask);
-    TFX_CHECK_LAUNCH(ctx);
-    return TFX_OK;
   }
-  if (s.R > 64)
-    return tfx_set_error(ctx, TFX_E_UNSUPPORTED,
-                         "CG for problems beyond the single-workgroup tile limit supports R <= 64");
-  void* wsp;
-  int rc = tfx_workspace(ctx, sizeof(S) * (5 * s.n + 2 * LRR) + 256, st, &wsp);
-  if (rc) return rc;
-  const size_t tmp_bytes = sizeof(S) * LRR;
-  const int tmp_in_lds = tmp_bytes <= 120 * 1024;
-  const size_t lds = tmp_in_lds ? tmp_bytes : 0;
-  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(lds > 0 ? lds : 1)));
-  hipLaunchKernelGGL(k_cg<S>, dim3(1), dim3(1024), lds, st, s, F, P1, P2, gamma, damp, grad,
-                     maxiter, tol, y, jtjg, scal, (S*)wsp, tmp_in_lds);
-  TFX_CHECK_LAUNCH(ctx);
-  return TFX_OK;
+  // larger problems: stream-ordered kernels over many workgroups
+  if (s.n > 100) {
+    if ((int64_t)maxiter > s.n) maxiter = (int)s.n;
+  } else {
+    tol = 0.0;
+  }
+  return cg_multi_t<S>(ctx, s, F, P1, P2, gamma, damp, grad, maxiter, tol, y, jtjg, scal, st);
 }
 
 }  // namespace

@@ -237,6 +237,42 @@ def test_cg_large_matches_oracle(dev, dims, R, maxiter, tol):
     assert _rel(jtjg.cpu().numpy(), rj) < 1e-11
 
 
+@pytest.mark.parametrize('dims,R,maxiter,tol,dtn', [
+    ((50, 50, 50), 50, 10, 1e-16, 'f64'),        # the 400^3 R=50 benchmark's core problem
+    ((130, 70, 40), 33, 8, 1e-16, 'f64'),        # R not a multiple of the 16-wide tiles
+    ((300, 20, 10, 6), 40, 6, 1e-16, 'f64'),     # order 4, I > one 128-row split
+    ((200, 60, 50), 32, 40, 1e-2, 'f64'),        # stops on the tolerance
+    ((400, 80, 60), 40, 6, 1e-16, 'f32')])
+def test_cg_multi_matches_oracle(dev, dims, R, maxiter, tol, dtn):
+    """CG of problems whose operands exceed one workgroup's LDS (the multi-kernel CG) against
+    the oracle's CG body from a given right-hand side."""
+    print('cg multi', dims, R, maxiter, tol, dtn, flush=True)
+    rs = np.random.RandomState(17)
+    F = [rs.randn(d, R) / np.sqrt(d) for d in dims]
+    g = rs.randn(R * sum(dims))
+    ry, _, rj, ritn, rrn = o.cg_from_grad([f.copy() for f in F], g, 0.3, maxiter, tol)
+    dt = torch.float64 if dtn == 'f64' else torch.float32
+    L = len(dims)
+    Fd = dev.upload(_flat(F), dt)
+    Gr, P1, P2 = dev.empty((L, R, R), dt), dev.empty((L, R, R), dt), dev.empty((L, L, R, R), dt)
+    gamma = dev.empty((L, R), dt)
+    dev.gramians(dims, R, Fd, Gr, P1, P2, gamma)
+    gd = dev.upload(g, dt)
+    y, jtjg = dev.empty(g.size, dt), dev.empty(g.size, dt)
+    scal = dev.zeros(2, torch.float64)
+    dev.cg(dims, R, Fd, P1, P2, gamma, 0.3, gd, maxiter, tol, y, jtjg, scal)
+    sc = scal.cpu().numpy()
+    if dtn == 'f64':
+        assert int(sc[0]) == ritn
+        assert sc[1] == pytest.approx(rrn, rel=1e-8)
+        assert _rel(y.cpu().numpy(), ry) < 1e-9
+        assert _rel(jtjg.cpu().numpy(), rj) < 1e-11
+    else:
+        assert int(sc[0]) == ritn
+        assert _rel(y.cpu().double().numpy(), ry) < 1e-3
+        assert _rel(jtjg.cpu().double().numpy(), rj) < 1e-5
+
+
 @pytest.mark.parametrize('dt', [torch.float64, torch.float32])
 @pytest.mark.parametrize('symm,fnorm', [(False, 0.0), (True, 0.0), (False, 2.5)])
 def test_dgn_update(dev, dt, symm, fnorm):
