@@ -68,7 +68,7 @@ __device__ inline double inv_m(S g, double damp) {
 }
 
 __device__ inline double wg_sum(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v = wave_sum(v);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __syncthreads();
   if (lane == 0) red[w] = v;

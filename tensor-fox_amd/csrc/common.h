@@ -49,11 +49,50 @@ int tfx_make_shape(tfx_ctx* ctx, int L, const int64_t* dims, int R, Shape* s);
 
 static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Sum over the 64 lanes of a wavefront, returned in every lane; all lanes must be active.  DPP
+// row rotations (8, 4, 2, 1) give every lane its 16-lane row's sum, then the four row sums are
+// read from lanes 0, 16, 32, 48 into scalar registers: no LDS traffic and a fixed summation order.
+template <int CTRL>
+__device__ inline unsigned dpp_u32(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ inline double dpp_f64(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = dpp_u32<CTRL>((unsigned)u), hi = dpp_u32<CTRL>((unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ inline double readlane_f64(double x, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ inline double wave_sum(double x) {
+  x += dpp_f64<0x128>(x);   // row_ror:8
+  x += dpp_f64<0x124>(x);   // row_ror:4
+  x += dpp_f64<0x122>(x);   // row_ror:2
+  x += dpp_f64<0x121>(x);   // row_ror:1
+  return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
+}
+__device__ inline float wave_sum(float x) {
+  x += __builtin_bit_cast(float, dpp_u32<0x128>(__builtin_bit_cast(unsigned, x)));
+  x += __builtin_bit_cast(float, dpp_u32<0x124>(__builtin_bit_cast(unsigned, x)));
+  x += __builtin_bit_cast(float, dpp_u32<0x122>(__builtin_bit_cast(unsigned, x)));
+  x += __builtin_bit_cast(float, dpp_u32<0x121>(__builtin_bit_cast(unsigned, x)));
+  const unsigned u = __builtin_bit_cast(unsigned, x);
+  const float r0 = __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_readlane((int)u, 0));
+  const float r1 = __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_readlane((int)u, 16));
+  const float r2 = __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_readlane((int)u, 32));
+  const float r3 = __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_readlane((int)u, 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
 // Sum over a workgroup of blockDim.x threads (multiple of 64).  `red` must hold blockDim.x/64
 // values.  Returns the total in every thread.
 template <typename V>
 __device__ inline V block_sum(V v, V* red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v = wave_sum(v);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   __syncthreads();
   if (lane == 0) red[w] = v;

@@ -165,7 +165,7 @@ __device__ void wg_orthonormalize(double* Z, int n, int p, double* red, double* 
         const double* zi = Z + (int64_t)i * n;
         double a = 0.0;
         for (int e = lane; e < n; e += 64) a += zi[e] * zj[e];
-        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        a = wave_sum(a);
         if (lane == 0) dots[i] = a;
       }
       __syncthreads();
@@ -189,75 +189,108 @@ __device__ void wg_orthonormalize(double* Z, int n, int p, double* red, double* 
 // Parallel cyclic Jacobi (round-robin pairing) on the symmetric m x m matrix H (row-major, in
 // place); V (m x m, col-major) receives the eigenvectors.  Eigenvalues end on the diagonal.
 __device__ void wg_jacobi(double* H, int m, double* V, double* cs, double* red) {
-  const int mm = (m + 1) & ~1;   // even number of players; index m is a dummy when m is odd
-  for (int e = threadIdx.x; e < m * m; e += blockDim.x) V[e] = (e % m) == (e / m) ? 1.0 : 0.0;
+  // Cyclic parallel Jacobi (round-robin ordering: every round rotates mm/2 disjoint pairs).
+  // A round computes its rotations from the current 2 x 2 diagonal blocks, then applies both
+  // sides at once: the 2 x 2 block of rows (a1, b1) and columns (a2, b2) becomes R1^T B R2, each
+  // block read and written by one thread; V's columns follow.  Index mm - 1 is a dummy player
+  // when m is odd.  cs holds 2 * (mm / 2) doubles and 2 * (mm / 2) ints (pair indices).
+  const int mm = (m + 1) & ~1;
+  const int hp = mm / 2;
+  double2* csk = (double2*)cs;       // (cos, sin) per pair
+  int2* pab = (int2*)(cs + 2 * hp);  // (first, second) index per pair (second >= m: dummy)
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wave = t >> 6, nwave = nt >> 6;
+  for (int e = t; e < m * m; e += nt) {
+    const int i = e / m, j = e - (e / m) * m;
+    V[e] = i == j ? 1.0 : 0.0;
+  }
   __syncthreads();
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0.0, dia = 0.0;
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-      const double h = H[e];
-      if ((e % m) == (e / m)) dia += h * h;
-      else off += h * h;
-    }
+    for (int i = wave; i < m; i += nwave)
+      for (int j = lane; j < m; j += 64) {
+        const double h = H[i * m + j];
+        if (i == j) dia += h * h;
+        else off += h * h;
+      }
     off = block_sum(off, red);
     dia = block_sum(dia, red);
     // converged when the off-diagonal norm is below 1e-13 of the diagonal's
     if (off <= 1e-26 * dia || off == 0.0) break;
     for (int round = 0; round < mm - 1; ++round) {
-      // pairs of this round: (round, mm-1) and ((round+k) % (mm-1), (round-k+mm-1) % (mm-1))
-      for (int k = threadIdx.x; k < mm / 2; k += blockDim.x) {
+      for (int k = t; k < hp; k += nt) {
         int a, b;
         if (k == 0) {
           a = round;
           b = mm - 1;
         } else {
-          a = (round + k) % (mm - 1);
-          b = (round - k + mm - 1) % (mm - 1);
+          a = round + k;
+          if (a >= mm - 1) a -= mm - 1;
+          b = round - k;
+          if (b < 0) b += mm - 1;
         }
         if (a > b) {
-          const int tmp = a;
+          const int x = a;
           a = b;
-          b = tmp;
+          b = x;
         }
-        double c = 1.0, s = 0.0;
+        double c = 1.0, sn = 0.0;
         if (b < m) {
           const double hab = H[a * m + b];
           if (hab != 0.0) {
-            const double tau = (H[b * m + b] - H[a * m + a]) / (2.0 * hab);
-            const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + tt * tt);
-            s = tt * c;
+            // t = tan(theta) of the symmetric Schur rotation, as sign(tau) / (|tau| +
+            // sqrt(1 + tau^2)) with tau = d / (2 hab), d = hbb - haa, written with one square
+            // root and one division: t = 2 hab sgn(d) / (|d| + sqrt(d^2 + 4 hab^2)), sgn(0) = 1
+            const double d = H[b * m + b] - H[a * m + a];
+            const double num = d >= 0.0 ? 2.0 * hab : -2.0 * hab;
+            const double tt = d == 0.0 ? 1.0 : num / (fabs(d) + sqrt(d * d + 4.0 * hab * hab));
+            // c = 1 / sqrt(1 + t^2) from the hardware reciprocal square root plus one Newton step
+            const double x = 1.0 + tt * tt;
+            double r = __builtin_amdgcn_rsq(x);
+            r = r * (1.5 - 0.5 * x * r * r);
+            c = r;
+            sn = tt * c;
           }
         }
-        cs[4 * k + 0] = c;
-        cs[4 * k + 1] = s;
-        cs[4 * k + 2] = a;
-        cs[4 * k + 3] = b;
+        csk[k] = make_double2(c, sn);
+        pab[k] = make_int2(a, b);
       }
       __syncthreads();
-      // rows: H <- J^T H
-      for (int e = threadIdx.x; e < (mm / 2) * m; e += blockDim.x) {
-        const int k = e / m, col = e % m;
-        const int a = (int)cs[4 * k + 2], b = (int)cs[4 * k + 3];
-        if (b >= m) continue;
-        const double c = cs[4 * k], s = cs[4 * k + 1];
-        const double ha = H[a * m + col], hb = H[b * m + col];
-        H[a * m + col] = c * ha - s * hb;
-        H[b * m + col] = s * ha + c * hb;
+      // H <- J^T H J by 2 x 2 blocks (rows of pair k1, columns of pair k2), block q = k1 hp + k2
+      for (int q = t; q < hp * hp; q += nt) {
+        const int k1 = q / hp, k2 = q - (q / hp) * hp;
+        {
+          const int2 p1 = pab[k1], p2 = pab[k2];
+          const double2 r1 = csk[k1], r2 = csk[k2];
+          const int a1 = p1.x, b1 = p1.y, a2 = p2.x, b2 = p2.y;
+          const double c1 = r1.x, s1 = r1.y, c2 = r2.x, s2 = r2.y;
+          const bool v1 = b1 < m;
+          const bool v2 = b2 < m;
+          const double h00 = H[a1 * m + a2];
+          const double h01 = v2 ? H[a1 * m + b2] : 0.0;
+          const double h10 = v1 ? H[b1 * m + a2] : 0.0;
+          const double h11 = (v1 && v2) ? H[b1 * m + b2] : 0.0;
+          // rows: x_a = c1 h_a - s1 h_b, x_b = s1 h_a + c1 h_b
+          const double x00 = c1 * h00 - s1 * h10, x01 = c1 * h01 - s1 * h11;
+          const double x10 = s1 * h00 + c1 * h10, x11 = s1 * h01 + c1 * h11;
+          // columns: y_a = c2 x_a - s2 x_b, y_b = s2 x_a + c2 x_b
+          H[a1 * m + a2] = c2 * x00 - s2 * x01;
+          if (v2) H[a1 * m + b2] = s2 * x00 + c2 * x01;
+          if (v1) H[b1 * m + a2] = c2 * x10 - s2 * x11;
+          if (v1 && v2) H[b1 * m + b2] = s2 * x10 + c2 * x11;
+        }
       }
-      __syncthreads();
-      // columns: H <- H J, V <- V J
-      for (int e = threadIdx.x; e < (mm / 2) * m; e += blockDim.x) {
-        const int k = e / m, row = e % m;
-        const int a = (int)cs[4 * k + 2], b = (int)cs[4 * k + 3];
+      // V <- V J (V[col][row] holds column col), item q = k m + row
+      for (int q = t; q < hp * m; q += nt) {
+        const int k = q / m, row = q - (q / m) * m;
+        const int2 pk = pab[k];
+        const int a = pk.x, b = pk.y;
         if (b >= m) continue;
-        const double c = cs[4 * k], s = cs[4 * k + 1];
-        const double ha = H[row * m + a], hb = H[row * m + b];
-        H[row * m + a] = c * ha - s * hb;
-        H[row * m + b] = s * ha + c * hb;
+        const double2 rk = csk[k];
+        const double c = rk.x, sn = rk.y;
         const double va = V[a * m + row], vb = V[b * m + row];
-        V[a * m + row] = c * va - s * vb;
-        V[b * m + row] = s * va + c * vb;
+        V[a * m + row] = c * va - sn * vb;
+        V[b * m + row] = sn * va + c * vb;
       }
       __syncthreads();
     }
@@ -271,14 +304,117 @@ __global__ __launch_bounds__(1024) void k_orth(double* Z, int n, int p) {
   wg_orthonormalize(Z, n, p, red, dots);
 }
 
+// Block classical Gram-Schmidt with reorthogonalization (BCGS2) on the p columns of Z (n x p,
+// column-major), in place, one workgroup.  Columns go in panels of OPB: a panel is staged in LDS,
+// projected twice against all finished columns (each finished column is read once per pass:
+// H = Q^T Z_P, then Z_P -= Q H), then orthonormalized within itself by CGS2 on a single
+// wavefront (each lane owns fixed rows of the panel, inner products are wave reductions, so no
+// workgroup barrier is needed there), and written back.
+constexpr int OPB = 8;
+constexpr int ORTH_MAXN = 2048;   // n * OPB doubles of LDS
+
+__global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* zp = (double*)smem;                 // [OPB][n]
+  double* H = zp + (size_t)OPB * n;           // [p][OPB]
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wave = t >> 6, nwave = nt >> 6;
+  for (int j0 = 0; j0 < p; j0 += OPB) {
+    const int pb = p - j0 < OPB ? p - j0 : OPB;
+    for (int u = t; u < pb * n; u += nt) zp[u] = Z[(int64_t)j0 * n + u];
+    __syncthreads();
+    for (int pass = 0; pass < 2 && j0 > 0; ++pass) {
+      // H[i][c] = Q_i . zp_c: one wavefront per finished column i
+      for (int i = wave; i < j0; i += nwave) {
+        const double* qi = Z + (int64_t)i * n;
+        double acc[OPB];
+#pragma unroll
+        for (int c = 0; c < OPB; ++c) acc[c] = 0.0;
+#pragma unroll 4
+        for (int e = lane; e < n; e += 64) {
+          const double q = qi[e];
+#pragma unroll
+          for (int c = 0; c < OPB; ++c)
+            if (c < pb) acc[c] += q * zp[c * n + e];
+        }
+#pragma unroll
+        for (int c = 0; c < OPB; ++c) {
+          const double v = wave_sum(acc[c]);
+          if (lane == 0 && c < pb) H[i * OPB + c] = v;
+        }
+      }
+      __syncthreads();
+      // zp_c -= sum_i Q_i H[i][c] (eight finished columns' loads in flight at a time)
+      for (int e = t; e < n; e += nt) {
+        double acc[OPB];
+#pragma unroll
+        for (int c = 0; c < OPB; ++c) acc[c] = c < pb ? zp[c * n + e] : 0.0;
+        int i = 0;
+        for (; i + 8 <= j0; i += 8) {
+          double q[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) q[u] = Z[(int64_t)(i + u) * n + e];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int c = 0; c < OPB; ++c) acc[c] -= q[u] * H[(i + u) * OPB + c];
+        }
+        for (; i < j0; ++i) {
+          const double q = Z[(int64_t)i * n + e];
+#pragma unroll
+          for (int c = 0; c < OPB; ++c) acc[c] -= q * H[i * OPB + c];
+        }
+#pragma unroll
+        for (int c = 0; c < OPB; ++c)
+          if (c < pb) zp[c * n + e] = acc[c];
+      }
+      __syncthreads();
+    }
+    // CGS2 inside the panel on wavefront 0 (lane-owned rows e = lane + 64 u)
+    if (wave == 0) {
+      for (int c = 0; c < pb; ++c) {
+        double* zc = zp + c * n;
+        for (int pass = 0; pass < 2 && c > 0; ++pass) {
+          double d[OPB];
+#pragma unroll
+          for (int i = 0; i < OPB; ++i) d[i] = 0.0;
+          for (int e = lane; e < n; e += 64) {
+            const double v = zc[e];
+#pragma unroll
+            for (int i = 0; i < OPB; ++i)
+              if (i < c) d[i] += zp[i * n + e] * v;
+          }
+#pragma unroll
+          for (int i = 0; i < OPB; ++i) d[i] = i < c ? wave_sum(d[i]) : 0.0;
+          for (int e = lane; e < n; e += 64) {
+            double v = zc[e];
+#pragma unroll
+            for (int i = 0; i < OPB; ++i)
+              if (i < c) v -= d[i] * zp[i * n + e];
+            zc[e] = v;
+          }
+        }
+        double a = 0.0;
+        for (int e = lane; e < n; e += 64) a += zc[e] * zc[e];
+        a = wave_sum(a);
+        const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+        for (int e = lane; e < n; e += 64) zc[e] *= inv;
+      }
+    }
+    __syncthreads();
+    for (int u = t; u < pb * n; u += nt) Z[(int64_t)j0 * n + u] = zp[u];
+    __syncthreads();
+  }
+}
+
 // Jacobi eigensolver of the symmetric m x m matrix H (row-major, global, destroyed), one
 // workgroup with H and V staged in LDS when they fit.  Writes the eigenvectors of the k largest
 // eigenvalues, in descending order, to Vk (m x k col-major) and the eigenvalues to lam.
-__global__ __launch_bounds__(256) void k_jacobi_top(double* H, int m, int k, double* Vg,
+__global__ __launch_bounds__(1024) void k_jacobi_top(double* H, int m, int k, double* Vg,
                                                      double* Vk, double* lam, int in_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[4];
-  __shared__ double cs[4 * 260];
+  __shared__ double red[16];
+  __shared__ __align__(16) double cs[4 * 260];
   __shared__ int order[512];
   double* Hs = in_lds ? (double*)smem : H;
   double* Vs = in_lds ? (double*)smem + m * m : Vg;
@@ -441,6 +577,18 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
     hipLaunchKernelGGL(k_gemm_small<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
                        transA, mm_, nn_, kk_, A_, B_, C_);
   };
+  // orthonormalization: block CGS2 with its panel in LDS when it fits, else column CGS2
+  const size_t orth_lds = sizeof(double) * ((size_t)OPB * n + (size_t)p * OPB);
+  const bool orth_block = n <= ORTH_MAXN && orth_lds <= 150 * 1024;
+  if (orth_block)
+    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_orth_block,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)orth_lds));
+  auto orth = [&](double* Zo) {
+    if (orth_block)
+      hipLaunchKernelGGL(k_orth_block, dim3(1), dim3(1024), orth_lds, stream, Zo, n, p);
+    else
+      hipLaunchKernelGGL(k_orth, dim3(1), dim3(1024), 0, stream, Zo, n, p);
+  };
   if (direct) {
     TFX_HIP(ctx, hipMemcpyAsync(H, G, sizeof(double) * n * n, hipMemcpyDeviceToDevice, stream));
   } else {
@@ -451,12 +599,12 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
     // scale them by (lambda_k / lambda_1)^iters before the single QR)
     for (int it = 0; it < iters; ++it) {
       gemm(0, n, p, n, G, Q, Z);
-      hipLaunchKernelGGL(k_orth, dim3(1), dim3(1024), 0, stream, Z, n, p);
+      orth(Z);
       double* t = Q;
       Q = Z;
       Z = t;
     }
-    if (iters == 0) hipLaunchKernelGGL(k_orth, dim3(1), dim3(1024), 0, stream, Q, n, p);
+    if (iters == 0) orth(Q);
     // Rayleigh-Ritz: H = Q^T (G Q)
     gemm(0, n, p, n, G, Q, Z);
     gemm(1, p, p, n, Q, Z, H);
@@ -466,7 +614,7 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   const int in_lds = lds <= 140 * 1024;
   TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_jacobi_top, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(in_lds ? lds : 1)));
-  hipLaunchKernelGGL(k_jacobi_top, dim3(1), dim3(256), in_lds ? lds : 0, stream, H, m, k, V, Vk, lam, in_lds);
+  hipLaunchKernelGGL(k_jacobi_top, dim3(1), dim3(1024), in_lds ? lds : 0, stream, H, m, k, V, Vk, lam, in_lds);
   TFX_CHECK_LAUNCH(ctx);
   if (direct) {
     TFX_HIP(ctx, hipMemcpyAsync(U, Vk, sizeof(double) * n * k, hipMemcpyDeviceToDevice, stream));

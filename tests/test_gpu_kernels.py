@@ -314,7 +314,8 @@ def test_gram_mode(dev, dims, dt):
 
 
 @pytest.mark.parametrize('n,k,p,kind', [(40, 5, 15, 'gap'), (400, 50, 60, 'gap'), (12, 3, 13, 'full'),
-                                        (100, 10, 20, 'gap'), (64, 16, 26, 'gap')])
+                                        (100, 10, 20, 'gap'), (64, 16, 26, 'gap'), (13, 4, 13, 'full'),
+                                        (300, 31, 41, 'gap'), (1500, 20, 30, 'gap')])
 def test_eig_top(dev, n, k, p, kind):
     print('eig', n, k, p, flush=True)
     rs = np.random.RandomState(9)
