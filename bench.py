@@ -49,6 +49,7 @@ def parse():
     ap.add_argument('--no-colscale', action='store_true', help='skip the column-scaled instance')
     ap.add_argument('--no-fp64', action='store_true', help='skip the FP64 GPU form')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--no-loss', action='store_true', help='skip the CPDLoss forward/backward timing')
     ap.add_argument('--cpu-sample-iters', type=int, default=1)
     ap.add_argument('--kernel-reps', type=int, default=20)
     ap.add_argument('--kernel-only', action='store_true',
@@ -124,6 +125,43 @@ def kernel_only(args):
            'frac': round(alg_bytes / kt / 1e9 / HBM_PEAK_GBS, 4),
            'copy_avg_s': ct, 'copy_GBs': round(2 * 4 * N / ct / 1e9, 1)}
     print(json.dumps(res), flush=True)
+
+
+def cpdloss_time(tfx, dev, reps):
+    """CPDLoss on a 256^3 FP32 target from seeded rank-32 factors plus 1e-3 relative noise, with
+    A, B, C (256 x 32 FP32) as leaf tensors: forward (one FP64-accumulated pass over T) and
+    forward + backward (adds the all-mode bf16x3 MFMA MTTKRP pass), timed with HIP events."""
+    import numpy as np
+    import torch
+    dims, R = (256, 256, 256), 32
+    T, Ft = tfx.make_lowrank(dims, R, 1e-3, seed=0, dtype=np.float32)
+    rs = np.random.RandomState(1)
+    Td = torch.from_numpy(T).to(dev.dev)
+    Fd = [torch.tensor(f + 0.1 * rs.standard_normal(f.shape), dtype=torch.float32, device=dev.dev,
+                       requires_grad=True) for f in Ft]
+    st = torch.cuda.current_stream()
+
+    def run(backward):
+        for f in Fd:
+            f.grad = None
+        loss = tfx.cpd_loss(Td, *Fd)
+        if backward:
+            loss.backward()
+        return loss
+
+    res = {'workload': 'CPDLoss 256x256x256 FP32, rank 32, noise 1e-3, factors as FP32 leaf tensors'}
+    for name, bw in (('forward_s', False), ('forward_backward_s', True)):
+        for _ in range(3):
+            run(bw)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            run(bw)
+        e1.record(st)
+        e1.synchronize()
+        res[name] = e0.elapsed_time(e1) / 1e3 / reps
+    res['tensor_passes_GBs'] = round(2 * 4 * T.size / res['forward_backward_s'] / 1e9, 1)
+    return res
 
 
 def cpu_baseline(T32, R, sample_iters, seed):
@@ -250,6 +288,10 @@ def main():
         log('colscale instance:', res)
         result['colscale_instance'] = res
         del Tcd
+
+    if not args.no_loss:
+        result['cpdloss'] = cpdloss_time(tfx, dev, args.kernel_reps)
+        log('cpdloss:', result['cpdloss'])
 
     if not args.no_cpu:
         cores = os.cpu_count()

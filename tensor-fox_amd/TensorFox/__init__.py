@@ -3,6 +3,7 @@ modules/TensorFox/__init__.py) on hand-written gfx950 HIP kernels.
 
     import TensorFox as tfx
     factors, output = tfx.cpd(T, R, options)
+    loss = tfx.cpd_loss(T, A, B, C)      # differentiable 0.5 ||T - [[A, B, C]]||^2
 """
 
 __version__ = '1.2.1+mi355x'
@@ -15,3 +16,4 @@ def version():
 from .auxiliar import (Output, consistency, gen_rand_tensor, make_lowrank, make_options,  # noqa: E402
                        sort_dims, unsort_dims, cpd2tens)
 from .tensorfox import cpd, tricpd  # noqa: E402
+from .loss import CPDLoss, cpd_loss  # noqa: E402
