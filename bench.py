@@ -50,6 +50,7 @@ def parse():
     ap.add_argument('--no-fp64', action='store_true', help='skip the FP64 GPU form')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--no-loss', action='store_true', help='skip the CPDLoss forward/backward timing')
+    ap.add_argument('--no-order4', action='store_true', help='skip the order-4 96^4 problem')
     ap.add_argument('--cpu-sample-iters', type=int, default=1)
     ap.add_argument('--kernel-reps', type=int, default=20)
     ap.add_argument('--kernel-only', action='store_true',
@@ -161,6 +162,38 @@ def cpdloss_time(tfx, dev, reps):
         e1.synchronize()
         res[name] = e0.elapsed_time(e1) / 1e3 / reps
     res['tensor_passes_GBs'] = round(2 * 4 * T.size / res['forward_backward_s'] / 1e9, 1)
+    return res
+
+
+def order4_time(tfx, dev):
+    """96^4 FP32 tensor, rank 24, first two columns of the first factor with cosine 0.9, relative
+    noise 1e-5, trunc_dims [24]*4, no refinement (the error is the FP64 residual on the full
+    tensor): the mixed form (warm) and the FP64 form."""
+    import numpy as np
+    import torch
+    dims, R = (96, 96, 96, 96), 24
+    T, _ = tfx.make_lowrank(dims, R, 1e-5, seed=0, collinear=0.9, dtype=np.float32)
+    Td = torch.from_numpy(T).to(dev.dev)
+    del T
+
+    class O:
+        trunc_dims = [24] * 4
+    res = {'workload': 'dense CPD 96^4 FP32 rank 24, collinear pair (cos 0.9), noise 1e-5, '
+                       'trunc_dims [24]*4, seed 0'}
+    for prec in ('mixed', 'mixed', 'fp64'):
+        o = O()
+        o.precision = prec
+        X = Td if prec == 'mixed' else Td.to(torch.float64)
+        torch.cuda.synchronize()
+        np.random.seed(0)
+        t0 = time.perf_counter()
+        _, out = tfx.cpd(X, R, o)
+        torch.cuda.synchronize()
+        res[prec + '_seconds'] = round(time.perf_counter() - t0, 5)
+        res[prec + '_rel_error'] = out.rel_error
+        res[prec + '_steps'] = int(out.num_steps)
+        res[prec + '_stage_seconds'] = {a: round(b, 5) for a, b in out.timings.items()}
+        del X
     return res
 
 
@@ -292,6 +325,10 @@ def main():
     if not args.no_loss:
         result['cpdloss'] = cpdloss_time(tfx, dev, args.kernel_reps)
         log('cpdloss:', result['cpdloss'])
+
+    if not args.no_order4:
+        result['order4'] = order4_time(tfx, dev)
+        log('order4:', result['order4'])
 
     if not args.no_cpu:
         cores = os.cpu_count()

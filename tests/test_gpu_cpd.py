@@ -161,3 +161,56 @@ def test_dogleg_path_runs():
     _, og = tfx.cpd(T, 3, make_opts(dict(tol_jump=0.9, maxiter=30)))
     assert og.dogleg_calls > 0
     assert np.isfinite(og.rel_error) and og.rel_error < 1.0
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_order4_collinear_truncated_vs_oracle(precision):
+    """Order-4 problem with a collinear pair (cos 0.9) and trunc_dims, against the oracle."""
+    import TensorFox as tfx
+    print('cpd order4 collinear', precision, flush=True)
+    dims, R = (20, 18, 16, 14), 5
+    T, Ftrue = tfx.make_lowrank(dims, R, 1e-5, seed=3, collinear=0.9)
+    if precision == 'mixed':
+        T = T.astype(np.float32)
+    fo, oo, fg, og = _both(T, R, dict(trunc_dims=[R] * 4), 3, precision)
+    assert og.compressed_dims == (R,) * 4
+    assert abs(og.rel_error - oo.rel_error) < 1e-6, (og.rel_error, oo.rel_error)
+    assert congruence(fg, fo) > 0.999
+
+
+def _fp64_rel_residual(T, F):
+    """||T - [[F]]|| / ||T|| on the host in FP64, for an order-4 tensor (row blocks of mode 1)."""
+    A, B, C, D = F
+    CD = (C[:, None, :] * D[None, :, :]).reshape(-1, A.shape[1])
+    T2 = T.reshape(A.shape[0] * B.shape[0], -1)
+    num, den = 0.0, 0.0
+    for i in range(A.shape[0]):
+        AB = A[i][None, :] * B
+        blk = T2[i * B.shape[0]:(i + 1) * B.shape[0]].astype(np.float64)
+        num += float(np.sum((blk - AB @ CD.T) ** 2))
+        den += float(np.sum(blk ** 2))
+    return np.sqrt(num / den)
+
+
+def test_order4_96_rank24_collinear_mixed():
+    """96^4 FP32 tensor, rank 24, first two columns of the first factor with cosine 0.9, relative
+    noise 1e-5; trunc_dims [24]*4, dGN on the 24^4 core in mixed precision, FP64 residual on the
+    full tensor.  At this size the oracle is not run: the reported error must equal an FP64 host
+    recomputation from the returned factors, reach the noise floor, agree with the FP64 GPU form,
+    and the factors must match the generating ones."""
+    import TensorFox as tfx
+    print('cpd 96^4 R=24 collinear mixed', flush=True)
+    dims, R, nu = (96, 96, 96, 96), 24, 1e-5
+    T, Ftrue = tfx.make_lowrank(dims, R, nu, seed=0, collinear=0.9, dtype=np.float32)
+    res = {}
+    for precision in ('mixed', 'fp64'):
+        np.random.seed(0)
+        f, out = tfx.cpd(T, R, make_opts(dict(trunc_dims=[24] * 4, precision=precision)))
+        res[precision] = (f, out)
+    f, out = res['mixed']
+    assert out.compressed_dims == (24,) * 4
+    host = _fp64_rel_residual(T, f)
+    assert abs(out.rel_error - host) < 1e-9 * max(1.0, host / 1e-5), (out.rel_error, host)
+    assert out.rel_error < 2 * nu
+    assert abs(out.rel_error - res['fp64'][1].rel_error) < 1e-6
+    assert congruence(f, Ftrue) > 0.999
