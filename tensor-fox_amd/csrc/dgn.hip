@@ -6,11 +6,18 @@
 // Flat layout (x, grad, y, factors): element (i, r) of mode l at off_l + r*I_l + i.
 #include "common.h"
 
+#include <type_traits>
+
 // multi-kernel CG (cg_multi.hip) for problems whose operands do not fit one workgroup's LDS
 template <typename S>
 int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, const S* gamma,
                double damp, const S* grad, int maxiter, double tol, S* y, S* jtjg, double* scal,
                hipStream_t st);
+
+// single-workgroup FP64 MFMA CG (cg_lds.hip): 1 if launched, 0 if the problem does not fit
+int cg_lds_try(tfx_ctx* ctx, const Shape& s, const double* F, const double* P1, const double* P2,
+               const double* gamma, double damp, const double* grad, int maxiter, double tol,
+               double* y, double* jtjg, double* scal, hipStream_t st);
 
 namespace {
 
@@ -499,6 +506,10 @@ int cg_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, con
          double damp, const S* grad, int maxiter, double tol, S* y, S* jtjg, double* scal,
          hipStream_t st) {
   const int64_t LRR = (int64_t)s.L * s.R * s.R;
+  if constexpr (std::is_same<S, double>::value) {
+    const int rc = cg_lds_try(ctx, s, F, P1, P2, gamma, damp, grad, maxiter, tol, y, jtjg, scal, st);
+    if (rc != 0) return rc < 0 ? rc : TFX_OK;
+  }
   // small problems: register-tiled single-workgroup kernel
   TileMap tm;
   tm.rb = (s.R + 3) / 4;

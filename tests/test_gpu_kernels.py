@@ -273,6 +273,38 @@ def test_cg_multi_matches_oracle(dev, dims, R, maxiter, tol, dtn):
         assert _rel(jtjg.cpu().double().numpy(), rj) < 1e-5
 
 
+@pytest.mark.parametrize('dims,R,maxiter,tol', [
+    ((24, 24, 24, 24), 24, 30, 1e-16),   # the 96^4 rank-24 problem's 24^4 core (16 output tiles)
+    ((10, 10, 10), 10, 12, 1e-16),       # the 100^3 rank-10 problem's core
+    ((16, 9, 13), 5, 8, 1e-16),          # rows not a multiple of 4, one rank tile
+    ((33, 17, 40), 30, 10, 1e-16),       # two rank tiles, ragged rows, 131 KB of LDS
+    ((20, 30), 7, 9, 1e-16),             # order 2 (P2 = ones)
+    ((6, 5, 4, 3), 3, 9, 1e-16),         # n <= 100: tolerance ignored
+    ((30, 25, 20), 8, 60, 1e-2)])        # stops on the tolerance
+def test_cg_lds_matches_oracle(dev, dims, R, maxiter, tol):
+    """Single-workgroup FP64 MFMA CG (all operands in LDS) against the oracle's CG body."""
+    print('cg lds', dims, R, maxiter, tol, flush=True)
+    rs = np.random.RandomState(23)
+    F = [rs.randn(d, R) / np.sqrt(d) for d in dims]
+    g = rs.randn(R * sum(dims))
+    ry, _, rj, ritn, rrn = o.cg_from_grad([f.copy() for f in F], g, 0.3, maxiter, tol)
+    dt = torch.float64
+    L = len(dims)
+    Fd = dev.upload(_flat(F), dt)
+    Gr, P1, P2 = dev.empty((L, R, R), dt), dev.empty((L, R, R), dt), dev.empty((L, L, R, R), dt)
+    gamma = dev.empty((L, R), dt)
+    dev.gramians(dims, R, Fd, Gr, P1, P2, gamma)
+    gd = dev.upload(g, dt)
+    y, jtjg = dev.empty(g.size, dt), dev.empty(g.size, dt)
+    scal = dev.zeros(2, torch.float64)
+    dev.cg(dims, R, Fd, P1, P2, gamma, 0.3, gd, maxiter, tol, y, jtjg, scal)
+    sc = scal.cpu().numpy()
+    assert int(sc[0]) == ritn
+    assert sc[1] == pytest.approx(rrn, rel=1e-8)
+    assert _rel(y.cpu().numpy(), ry) < 1e-9
+    assert _rel(jtjg.cpu().numpy(), rj) < 1e-11
+
+
 @pytest.mark.parametrize('dt', [torch.float64, torch.float32])
 @pytest.mark.parametrize('symm,fnorm', [(False, 0.0), (True, 0.0), (False, 2.5)])
 def test_dgn_update(dev, dt, symm, fnorm):
