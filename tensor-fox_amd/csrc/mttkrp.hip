@@ -11,6 +11,10 @@
 // the MFMA split-precision path is in mttkrp_mfma.hip.
 #include "common.h"
 
+// FP64 MFMA residual (residual_mfma.hip): 1 when it ran, 0 when the shape is not covered
+int residual_mfma_try(tfx_ctx* ctx, const void* T, int t_dtype, const Shape& s, const double* F,
+                      double* out, hipStream_t st);
+
 namespace {
 
 constexpr int RB = 32;   // rows per block step
@@ -446,6 +450,10 @@ extern "C" int tfx_cpd_residual_mixed(tfx_ctx* ctx, const void* T, int t_dtype, 
   int rc = tfx_make_shape(ctx, L, dims, R, &s);
   if (rc) return rc;
   if (R > 256) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "residual supports R <= 256");
+  if (f_dtype == TFX_F64) {
+    const int m = residual_mfma_try(ctx, T, t_dtype, s, (const double*)F_flat, out_sqnorm, stream);
+    if (m != 0) return m < 0 ? m : TFX_OK;
+  }
   if (t_dtype == TFX_F32 && f_dtype == TFX_F32) return residual_dispatch<float, float>(ctx, T, s, F_flat, out_sqnorm, stream);
   if (t_dtype == TFX_F32 && f_dtype == TFX_F64) return residual_dispatch<float, double>(ctx, T, s, F_flat, out_sqnorm, stream);
   if (t_dtype == TFX_F64 && f_dtype == TFX_F64) return residual_dispatch<double, double>(ctx, T, s, F_flat, out_sqnorm, stream);

@@ -114,6 +114,26 @@ def test_residual(dev, dims, R, dt):
     assert abs(out.item() - ref) / ref < tol
 
 
+@pytest.mark.parametrize('tdt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('dims,R', [((37, 23, 70), 50),      # rows and columns not tile multiples
+                                    ((100, 130), 20),        # order 2
+                                    ((13, 9, 11, 67), 64),   # order 4, R at the limit
+                                    ((64, 48, 128), 1),      # rank 1
+                                    ((200, 150, 130), 50)])  # many workgroups
+def test_residual_mfma_fp64_factors(dev, dims, R, tdt):
+    """FP64 MFMA residual (FP64 factors, R <= 64, order 2..4) against the oracle's reconstruction."""
+    print('residual mfma', dims, R, tdt, flush=True)
+    rs = np.random.RandomState(8)
+    F = [rs.randn(d, R) for d in dims]
+    T = o.cpd_tensor(F) + 1e-4 * rs.randn(*dims)
+    if tdt == torch.float32:
+        T = T.astype(np.float32)
+    out = dev.zeros(1, torch.float64)
+    dev.residual_sq(dev.upload(T.ravel(), tdt), dims, dev.upload(_flat(F), torch.float64), R, out)
+    ref = np.sum((T.astype(np.float64) - o.cpd_tensor(F)) ** 2)
+    assert abs(out.item() - ref) / ref < 1e-9
+
+
 def test_residual_fp32_tensor_fp64_factors(dev):
     dims, R = (30, 20, 25), 6
     rs = np.random.RandomState(4)
