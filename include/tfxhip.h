@@ -159,6 +159,46 @@ int tfx_gemm_small(tfx_ctx* ctx, int dtype, int transA, int m, int n, int kk, co
 int tfx_convert(tfx_ctx* ctx, int64_t n, const void* src, int src_dtype, void* dst, int dst_dtype,
                 hipStream_t stream);
 
+/* ---- batches of small CPDs --------------------------------------------------------------
+ * A batch runs many independent problems of uneven shape and rank, each taken through the
+ * reference pipeline with default options (TensorFox.py:424-655: sorted modes, MLSVD 'seq',
+ * random start on the core, dGN with CG, uncompression, error on the full tensor).  The
+ * reference has no batch call: its users loop over `cpd` (TensorFox.py:69).  Members are
+ * described by device int64 records of 16 fields, built by the Python layer
+ * (TensorFox/batched.py): [0] order L, [1] rank R (<= 16), [2..5] sorted dims (<= 64),
+ * [6] tensor offset (elements), [7] U offset, [8] core offset, [9] work-vector offset,
+ * [10] work-vector stride, [11] full-factor offset, [12] history offset, [13] MLSVD scratch
+ * offset, [14] scratch half size (all in doubles unless stated).  `order` lists member indices
+ * in launch order.  Per-member results and dGN state: 16 doubles each (`res`). */
+typedef struct tfx_batch_opts {
+  int maxiter;
+  int symm;
+  double tol, tol_step, tol_improv, tol_grad, tol_jump, init_damp, cg_tol, factors_norm;
+} tfx_batch_opts;
+
+/* MLSVD of every member (Compression.py:141-156, 192-301 with the Gram eigensolver of
+ * tfx_eig_top): one workgroup per member.  Writes the truncated core dims (nbatch x 4 int32),
+ * the bases U_l and the cores. */
+int tfx_batch_mlsvd(tfx_ctx* ctx, const void* T, int dtype, int nbatch, const int64_t* meta,
+                    const int32_t* order, double tol_mlsvd, int32_t* kdims, double* U, double* S,
+                    double* W, hipStream_t stream);
+
+/* dGN iterations [it_begin, it_end) of every member on its core (GaussNewton.py:29-312), one
+ * wavefront per member; the member's start point is work vector 0 when it_begin == 0, its
+ * state is kept in `res` / `work` between calls.  caps: nbatch x maxiter CG iteration caps.
+ * lds_bytes: dynamic LDS per member (the largest member's need). */
+int tfx_batch_dgn(tfx_ctx* ctx, int L, int nbatch, const int64_t* meta, const int32_t* order,
+                  const int32_t* kdims, const double* S, double* work, double* hist, double* res,
+                  const int32_t* caps, const tfx_batch_opts* opts, int it_begin, int it_end,
+                  int lds_bytes, hipStream_t stream);
+
+/* Uncompression F_l = U_l W_l (TensorFox.py:573-574) of the best core factors and the FP64
+ * squared error / squared norm of every member's full tensor (res fields 11, 12). */
+int tfx_batch_finish(tfx_ctx* ctx, const void* T, int dtype, int L, int nbatch,
+                     const int64_t* meta, const int32_t* order, const int32_t* kdims,
+                     const double* U, const double* work, double* Ffull, double* res,
+                     hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

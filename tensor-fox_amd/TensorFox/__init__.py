@@ -4,6 +4,7 @@ modules/TensorFox/__init__.py) on hand-written gfx950 HIP kernels.
     import TensorFox as tfx
     factors, output = tfx.cpd(T, R, options)
     loss = tfx.cpd_loss(T, A, B, C)      # differentiable 0.5 ||T - [[A, B, C]]||^2
+    results = tfx.cpd_batched(tensors, ranks, options, seeds)   # many small CPDs at once
 """
 
 __version__ = '1.2.1+mi355x'
@@ -17,3 +18,4 @@ from .auxiliar import (Output, consistency, gen_rand_tensor, make_lowrank, make_
                        sort_dims, unsort_dims, cpd2tens)
 from .tensorfox import cpd, tricpd  # noqa: E402
 from .loss import CPDLoss, cpd_loss  # noqa: E402
+from .batched import autotune_batched, cpd_batched  # noqa: E402

@@ -42,7 +42,20 @@ SIGNATURES = {
     'tfx_ttm': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_vp],
     'tfx_gemm_small': [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_convert': [_c_vp, _c_i64, _c_vp, _c_int, _c_vp, _c_int, _c_vp],
+    'tfx_batch_mlsvd': [_c_vp, _c_vp, _c_int, _c_int, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp, _c_vp,
+                        _c_vp, _c_vp],
+    'tfx_batch_dgn': [_c_vp, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                      _c_vp, _c_vp, _c_int, _c_int, _c_int, _c_vp],
+    'tfx_batch_finish': [_c_vp, _c_vp, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                         _c_vp, _c_vp, _c_vp],
 }
+
+
+class BatchOpts(ctypes.Structure):
+    """tfx_batch_opts (include/tfxhip.h)."""
+    _fields_ = [('maxiter', _c_int), ('symm', _c_int), ('tol', _c_dbl), ('tol_step', _c_dbl),
+                ('tol_improv', _c_dbl), ('tol_grad', _c_dbl), ('tol_jump', _c_dbl),
+                ('init_damp', _c_dbl), ('cg_tol', _c_dbl), ('factors_norm', _c_dbl)]
 
 _lib = None
 _lock = threading.Lock()

@@ -10,6 +10,7 @@ Precision forms (option `precision`):
              (core dGN, CG, eigensolve, factors) run in FP64 (default for float32 input).
 """
 
+import math
 import sys
 import time
 from copy import deepcopy
@@ -27,14 +28,25 @@ _CLEAN_RNG = np.random.RandomState(98765)
 
 
 def _equalize(F, R):
-    """Conversion.py:361-387 on host factors."""
+    """Conversion.py:361-387 on host factors: every rank-one term's columns rescaled to the
+    geometric mean of their norms.  Column norms are sqrt(dot) of contiguous copies and the
+    products run left to right, which is what np.linalg.norm and np.prod compute for these
+    sizes, so the result is bitwise that of the per-column loop; the scaling is applied to all
+    columns at once."""
     L = len(F)
+    Ft = [np.ascontiguousarray(f.T) for f in F]
+    scale = np.ones((L, R))
     for r in range(R):
-        nr = np.array([np.linalg.norm(F[l][:, r]) for l in range(L)])
-        if np.prod(nr) != 0.0:
-            num = np.prod(nr) ** (1 / L)
+        nr = [math.sqrt(Ft[l][r].dot(Ft[l][r])) for l in range(L)]
+        p = nr[0]
+        for l in range(1, L):
+            p = p * nr[l]
+        if p != 0.0:
+            num = p ** (1 / L)
             for l in range(L):
-                F[l][:, r] = (num / nr[l]) * F[l][:, r]
+                scale[l, r] = num / nr[l]
+    for l in range(L):
+        F[l] = F[l] * scale[l][None, :]
     return F
 
 

@@ -105,3 +105,41 @@ def test_product_does_not_import_oracle():
         if f.endswith('.py'):
             src = open(os.path.join(pkg, f)).read()
             assert 'oracle' not in src, f
+
+
+def test_batched_draws_follow_the_oracle_stream():
+    """The batched driver replays a member's random draws up front (MLSVD sketches, starting
+    factors, the CG caps of all maxiter iterations).  With every iteration run (tolerances 0),
+    the oracle's cpd consumes exactly the same stream: the generator states must agree, and the
+    starting point must be the oracle's."""
+    import TensorFox as tfx
+    from TensorFox.batched import _draw_member
+    from tests.golden.cases import make_opts
+    rs = np.random.RandomState(4)
+    dims, R = (12, 10, 9), 3
+    F = [rs.randn(d, R) for d in dims]
+    T = o.cpd_tensor(F) + 1e-3 * rs.randn(*dims)
+    kw = dict(maxiter=6, tol=0, tol_step=0, tol_improv=0, tol_grad=0)
+    np.random.seed(11)
+    o.cpd(T, R, make_opts(kw))
+    s1 = np.random.get_state()
+    x, caps = _draw_member(11, dims, (3, 3, 3), R, tfx.make_options(make_opts(kw)), 6)
+    s2 = np.random.get_state()
+    assert s1[2] == s2[2] and np.array_equal(s1[1], s2[1])
+    assert caps.shape == (6,) and caps[0] == 2      # 1 + (L - 2) * randint(1, 2)
+    assert x.shape == (R * 9,)
+
+
+def test_host_equalize_bitwise_equals_oracle():
+    """The host equalization of starting points (vectorised scaling) gives bitwise the oracle's
+    per-column result (Conversion.py:361-387), zero columns included."""
+    from TensorFox.tensorfox import _equalize
+    rs = np.random.RandomState(2)
+    for trial in range(400):
+        L, R = rs.randint(2, 5), rs.randint(1, 17)
+        F = [rs.randn(rs.randint(1, 65), R) * 10 ** rs.uniform(-3, 3) for _ in range(L)]
+        if trial % 7 == 0:
+            F[1][:, R - 1] = 0
+        A = o.equalize([f.copy() for f in F], R)
+        B = _equalize([f.copy() for f in F], R)
+        assert all(np.array_equal(a, b) for a, b in zip(A, B))
