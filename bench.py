@@ -51,6 +51,8 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--no-loss', action='store_true', help='skip the CPDLoss forward/backward timing')
     ap.add_argument('--no-order4', action='store_true', help='skip the order-4 96^4 problem')
+    ap.add_argument('--no-batched', action='store_true', help='skip the 4096-member batch')
+    ap.add_argument('--batch-size', type=int, default=4096)
     ap.add_argument('--cpu-sample-iters', type=int, default=1)
     ap.add_argument('--kernel-reps', type=int, default=20)
     ap.add_argument('--kernel-only', action='store_true',
@@ -197,6 +199,66 @@ def order4_time(tfx, dev):
     return res
 
 
+def make_batch(nb, seed=0):
+    """nb dense 3rd-order FP32 tensors, each mode size uniform in 8..64, rank uniform in 2..16,
+    factors N(0, 1), relative noise 1e-4 (legacy RandomState(seed) draws)."""
+    import numpy as np
+    rs = np.random.RandomState(seed)
+    dims = rs.randint(8, 65, (nb, 3))
+    ranks = rs.randint(2, 17, nb)
+    tensors = []
+    for m in range(nb):
+        I, J, K = (int(v) for v in dims[m])
+        R = int(ranks[m])
+        A, B, C = (rs.standard_normal((d, R)) for d in (I, J, K))
+        T0 = ((A[:, None, :] * B[None, :, :]).reshape(I * J, R) @ C.T).reshape(I, J, K)
+        E = rs.standard_normal((I, J, K))
+        T = T0 + 1e-4 * np.linalg.norm(T0) * E / np.linalg.norm(E)
+        tensors.append(T.astype(np.float32))
+    return tensors, [int(r) for r in ranks]
+
+
+def batched_time(tfx, nb, steps):
+    """The batch resident on the device, autotuned launch configuration, then `steps` timed
+    solves of the whole batch (member m seeded with m); plus the oracle on a bounded sample of
+    members on the host cores."""
+    import numpy as np
+    import oracle.tfx_oracle as o
+    tensors, ranks = make_batch(nb)
+    seeds = list(range(nb))
+    best, tune_times = tfx.autotune_batched(tensors, ranks, None, seeds, sample=256)
+    prob = tfx.BatchedCPD(tensors, ranks, None, config=best)
+    prob.run(seeds)
+    times, res = [], None
+    for _ in range(max(1, steps)):
+        t0 = time.perf_counter()
+        res = prob.run(seeds)
+        times.append(time.perf_counter() - t0)
+    t = float(np.mean(times))
+    errs = np.array([out.rel_error for _, out in res])
+    nsteps = np.array([out.num_steps for _, out in res])
+    out = {'workload': '%d dense FP32 tensors, dims U{8..64}^3, rank U{2..16}, noise 1e-4, '
+                       'default options (MLSVD, dGN on the core), member m seeded with m' % nb,
+           'seconds': round(t, 5), 'members_per_s': round(nb / t, 1),
+           'stage_seconds': {a: round(b, 5) for a, b in res[0][1].timings.items()},
+           'config': best, 'tune_seconds': {a: round(b, 4) for a, b in tune_times.items()},
+           'rel_error_median': float(np.median(errs)),
+           'rel_error_below_1e-3': float(np.mean(errs < 1e-3)),
+           'dgn_steps_mean': float(nsteps.mean())}
+    # oracle on a bounded sample (every 512th member), scaled to the batch
+    idx = list(range(0, nb, max(1, nb // 8)))
+    t0 = time.perf_counter()
+    for i in idx:
+        np.random.seed(seeds[i])
+        o.cpd(tensors[i].astype(np.float64), ranks[i])
+    tc = time.perf_counter() - t0
+    out['cpu_baseline'] = {'value': round(tc / len(idx) * nb, 2), 'unit': 's', 'cores': os.cpu_count(),
+                           'kind': 'port',
+                           'sample': 'oracle (FP64 NumPy restatement of the reference) on %d members '
+                                     '(%.2f s), scaled to %d members' % (len(idx), tc, nb)}
+    return out
+
+
 def cpu_baseline(T32, R, sample_iters, seed):
     """FP64 NumPy restatement of the reference algorithm (the oracle) on the host cores: the
     MLSVD, `sample_iters` main-stage and `sample_iters` refinement dGN iterations are timed, then
@@ -325,6 +387,10 @@ def main():
     if not args.no_loss:
         result['cpdloss'] = cpdloss_time(tfx, dev, args.kernel_reps)
         log('cpdloss:', result['cpdloss'])
+
+    if not args.no_batched:
+        result['batched'] = batched_time(tfx, args.batch_size, args.steps)
+        log('batched:', {k: v for k, v in result['batched'].items() if k != 'tune_seconds'})
 
     if not args.no_order4:
         result['order4'] = order4_time(tfx, dev)

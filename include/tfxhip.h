@@ -192,6 +192,21 @@ int tfx_batch_dgn(tfx_ctx* ctx, int L, int nbatch, const int64_t* meta, const in
                   const int32_t* caps, const tfx_batch_opts* opts, int it_begin, int it_end,
                   int lds_bytes, hipStream_t stream);
 
+/* Host-side (no GPU, no context): every member's random draws from its seed, as NumPy's legacy
+ * global generator produces them for one reference `cpd` call -- the MLSVD sketches
+ * (Compression.py:337-409, discarded), the starting factors randn(k_l, R) (Initialization.py:
+ * 20-94), equalized (Conversion.py:361-387) and scaled to factors_norm when positive, written
+ * flat at x0 + x_off[m], and the CG cap of each iteration from the integer bounds
+ * randint(cap_low[it], cap_low[it] + cap_rng[it] + 1) (GaussNewton.py:263-266), or cg_maxiter
+ * when cg_static.  dims: nbatch x L sorted dims; kdims: nbatch x 4 core dims.  has_zero[m] is
+ * set when a starting entry is exactly 0 (the reference then redraws it; the caller handles
+ * that member).  Runs on up to nthreads host threads (0: all cores). */
+int tfx_batch_draws(int nbatch, int L, const uint32_t* seeds, const int32_t* dims,
+                    const int32_t* kdims, const int32_t* ranks, int maxiter, const int64_t* cap_low,
+                    const int64_t* cap_rng, double cg_factor, int cg_static, int cg_maxiter,
+                    double factors_norm, const int64_t* x_off, double* x0, int32_t* caps,
+                    int32_t* has_zero, int nthreads);
+
 /* Uncompression F_l = U_l W_l (TensorFox.py:573-574) of the best core factors and the FP64
  * squared error / squared norm of every member's full tensor (res fields 11, 12). */
 int tfx_batch_finish(tfx_ctx* ctx, const void* T, int dtype, int L, int nbatch,

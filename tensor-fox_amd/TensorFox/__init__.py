@@ -18,4 +18,4 @@ from .auxiliar import (Output, consistency, gen_rand_tensor, make_lowrank, make_
                        sort_dims, unsort_dims, cpd2tens)
 from .tensorfox import cpd, tricpd  # noqa: E402
 from .loss import CPDLoss, cpd_loss  # noqa: E402
-from .batched import autotune_batched, cpd_batched  # noqa: E402
+from .batched import BatchedCPD, autotune_batched, cpd_batched  # noqa: E402

@@ -46,6 +46,8 @@ SIGNATURES = {
                         _c_vp, _c_vp],
     'tfx_batch_dgn': [_c_vp, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
                       _c_vp, _c_vp, _c_int, _c_int, _c_int, _c_vp],
+    'tfx_batch_draws': [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_dbl,
+                        _c_int, _c_int, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_int],
     'tfx_batch_finish': [_c_vp, _c_vp, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
                          _c_vp, _c_vp, _c_vp],
 }

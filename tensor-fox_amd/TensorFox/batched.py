@@ -14,13 +14,17 @@ FP64 MFMA with its operands in LDS) -- or, in graph mode, one captured launch pe
 replayed as one HIP graph; one launch uncompresses and measures the error.  The host only
 replays each member's random draws: the MLSVD sketches (drawn and discarded, in the reference's
 shapes, because the device computes exact eigenvectors), the starting factors and the CG caps
-of every iteration (GaussNewton.py:263-266), all from the member's seed.
+of every iteration (GaussNewton.py:263-266), all from the member's seed -- with seeds, in native
+code on host threads (tfx_batch_draws, the same stream as NumPy's legacy generator); without
+seeds, from NumPy's global generator member after member (pre-drawing every iteration's cap, so
+the global stream then differs from a loop of reference calls).
 
 Supported options: method 'dGN', inner_method 'cg' or 'cg_static', initialization 'random',
 mlsvd_method 'seq', no trunc_dims / svd_n_components, no refinement, no symm; order 2-4,
 dims <= 64, rank <= 16.  Anything else raises NotImplementedError.
 """
 
+import ctypes
 import json
 import os
 import time
@@ -107,6 +111,15 @@ class _Plan:
             self.order = np.arange(nb, dtype=np.int32)
 
 
+def _cap_bounds(maxiter):
+    """Integer bounds of the CG cap draws randint(1 + it**0.4, 2 + it**0.9), it = 0..maxiter-1,
+    with the powers taken on Python floats as the reference's loop takes them (randint truncates
+    float bounds)."""
+    low = np.array([int(1 + it ** 0.4) for it in range(maxiter)], dtype=np.int64)
+    high = np.array([int(2 + it ** 0.9) for it in range(maxiter)], dtype=np.int64)
+    return low, high
+
+
 def _draw_member(seed, dims, kd, R, options, maxiter):
     """The member's random draws in the reference's order: MLSVD sketches (Compression.py:
     337-409 draws one Gaussian sketch per mode), starting factors (Initialization.py:20-94) and
@@ -126,8 +139,8 @@ def _draw_member(seed, dims, kd, R, options, maxiter):
     F = _equalize(F, R)
     F = _transform(F, options.symm, options.factors_norm)
     if options.inner_method == 'cg':
-        its = np.arange(maxiter)
-        draws = np.random.randint(1 + its ** 0.4, 2 + its ** 0.9)
+        low, high = _cap_bounds(maxiter)
+        draws = np.random.randint(low, high)
         caps = 1 + (L - 2) * np.trunc(options.cg_factor * draws).astype(np.int64)
     else:
         caps = np.full(maxiter, int(options.cg_maxiter), dtype=np.int64)
@@ -174,102 +187,119 @@ def _store_tuned(key, cfg):
         pass
 
 
+class BatchedCPD:
+    """A batch resident on the device: validated, sorted, packed and uploaded once.  `run`
+    solves it (every member from its seed) and returns the (factors, output) pairs."""
+
+    def __init__(self, tensors, ranks, options=False, config=None, device=None):
+        t0 = time.perf_counter()
+        options = make_options(options)
+        _check_options(options)
+        tensors = [T.detach().cpu().numpy() if isinstance(T, torch.Tensor) else np.asarray(T)
+                   for T in tensors]
+        nb = len(tensors)
+        if nb < 1 or nb != len(ranks):
+            raise ValueError('one rank per tensor expected')
+        dims_sorted, orderings = [], []
+        for T, R in zip(tensors, ranks):
+            consistency(R, tuple(T.shape), options)
+            d, ordering = sort_dims(T.shape)
+            if len(d) < 2 or len(d) > MAX_ORDER or max(d) > MAX_DIM or R > MAX_RANK:
+                raise NotImplementedError('cpd_batched covers orders 2-%d, dims <= %d and rank <= %d; '
+                                          'got dims %s, rank %d' % (MAX_ORDER, MAX_DIM, MAX_RANK,
+                                                                   tuple(T.shape), R))
+            dims_sorted.append(d)
+            orderings.append(ordering)
+        L = len(dims_sorted[0])
+        if any(len(d) != L for d in dims_sorted):
+            raise NotImplementedError('cpd_batched needs members of one order')
+        self.dev = dev = get_device(options.device if device is None else device)
+        precision = options.precision
+        if precision is None:
+            precision = 'mixed' if all(T.dtype == np.float32 for T in tensors) else 'fp64'
+        self.tdt = torch.float32 if precision == 'mixed' else torch.float64
+        npdt = np.float32 if precision == 'mixed' else np.float64
+        maxiter = int(options.maxiter)
+        if config is None:
+            config = _load_tuned(_tune_key(dev, L, dims_sorted, ranks, maxiter)) or DEFAULT_CONFIG
+        self.config = config
+        self.options, self.precision, self.nb, self.L = options, precision, nb, L
+        self.ranks = [int(r) for r in ranks]
+        self.dims_sorted, self.orderings = dims_sorted, orderings
+        plan = self.plan = _Plan(dims_sorted, self.ranks, maxiter, config.get('order', 'size'))
+        sz = plan.sizes
+        host = np.empty(sz['T'], dtype=npdt)
+        for m, (T, ordering) in enumerate(zip(tensors, orderings)):
+            A = T
+            if any(int(ordering[l]) != l for l in range(L)):
+                A = np.moveaxis(A, ordering, np.arange(L))
+            o = int(plan.meta[m, 6])
+            host[o:o + A.size] = np.ascontiguousarray(A, dtype=npdt).ravel()
+        e = dev.empty
+        self.Tbuf = torch.from_numpy(host).to(dev.dev)
+        self.meta = torch.from_numpy(plan.meta).to(dev.dev)
+        self.order = torch.from_numpy(plan.order).to(dev.dev)
+        self.kdims = dev.zeros((nb, 4), torch.int32)
+        self.U = e(max(sz['U'], 1), torch.float64)
+        self.S = e(max(sz['S'], 1), torch.float64)
+        self.W = e(max(sz['W'], 1), torch.float64)
+        self.work = dev.zeros(max(sz['X'], 1), torch.float64)
+        self.hist = dev.zeros(max(sz['H'], 1), torch.float64)
+        self.res = dev.zeros(nb * RS_STRIDE, torch.float64)
+        self.Ffull = e(max(sz['F'], 1), torch.float64)
+        torch.cuda.synchronize()
+        self.upload_seconds = time.perf_counter() - t0
+
+    def run(self, seeds=None):
+        t_all = time.perf_counter()
+        nb, L, plan, options = self.nb, self.L, self.plan, self.options
+        if seeds is not None and len(seeds) != nb:
+            raise ValueError('one seed per tensor expected')
+        maxiter = int(options.maxiter)
+        self.hist.zero_()
+        self.res.zero_()
+        timings = {'upload': self.upload_seconds}
+        kd, resh, hh, Fh = _solve(self.dev, plan, options, seeds, self.config, self.precision,
+                                  self.tdt, self.Tbuf, self.meta, self.order, self.kdims, self.U,
+                                  self.S, self.W, self.work, self.hist, self.res, self.Ffull,
+                                  timings)
+        t0 = time.perf_counter()
+        results = []
+        z = np.array([0])
+        for m in range(nb):
+            R = self.ranks[m]
+            d = self.dims_sorted[m]
+            fo = int(plan.meta[m, 11])
+            Fm = []
+            for l in range(L):
+                Fm.append(Fh[fo:fo + R * d[l]].reshape(R, d[l]).T.copy())
+                fo += R * d[l]
+            Fm = unsort_dims(Fm, self.orderings[m])
+            r = resh[m]
+            k = int(r[RS_STEPS])
+            h0 = int(plan.meta[m, 12])
+            hm = hh[h0:h0 + 4 * maxiter].reshape(4, maxiter)[:, :k]
+            final_err = float(np.sqrt(r[RS_E2]) / np.sqrt(r[RS_T2]))
+            o = Output(k, final_err, [hm[1].copy(), z], [hm[0].copy(), z], [hm[2].copy(), z],
+                       [hm[3].copy(), z], [int(r[RS_STOP]), 8], options,
+                       timings=timings if m == 0 else {}, precision=self.precision)
+            o.final_rel_error = final_err
+            o.compressed_dims = tuple(int(v) for v in kd[m, :L])
+            o.dogleg_calls = int(r[RS_DOGLEG])
+            results.append((Fm, o))
+        timings['outputs'] = time.perf_counter() - t0
+        timings['total'] = time.perf_counter() - t_all
+        return results
+
+
 def cpd_batched(tensors, ranks, options=False, seeds=None, config=None, device=None):
     """CPD of every tensor in `tensors` with the rank in `ranks` (Python ints).  Returns a list
-    of (factors, output) pairs.  seeds[m] (optional) seeds the global NumPy RNG before member m's
-    draws, as `np.random.seed(seeds[m])` before a call of the reference's cpd would.  `config`
-    ({'mode': 'persistent'|'graph', 'order': 'size'|'natural'}) overrides the tuned launch
-    configuration.  output.timings (of the first member) holds the batch's stage times."""
-    t_all = time.perf_counter()
-    options = make_options(options)
-    _check_options(options)
-    tensors = [T.detach().cpu().numpy() if isinstance(T, torch.Tensor) else np.asarray(T)
-               for T in tensors]
-    nb = len(tensors)
-    if nb != len(ranks):
-        raise ValueError('one rank per tensor expected')
-    if seeds is not None and len(seeds) != nb:
-        raise ValueError('one seed per tensor expected')
-    dims_sorted, orderings = [], []
-    for T, R in zip(tensors, ranks):
-        consistency(R, tuple(T.shape), options)
-        d, ordering = sort_dims(T.shape)
-        if len(d) < 2 or len(d) > MAX_ORDER or max(d) > MAX_DIM or R > MAX_RANK:
-            raise NotImplementedError('cpd_batched covers orders 2-%d, dims <= %d and rank <= %d; got '
-                                      'dims %s, rank %d' % (MAX_ORDER, MAX_DIM, MAX_RANK, tuple(T.shape), R))
-        dims_sorted.append(d)
-        orderings.append(ordering)
-    L = len(dims_sorted[0])
-    if any(len(d) != L for d in dims_sorted):
-        raise NotImplementedError('cpd_batched needs members of one order')
-    dev = get_device(options.device if device is None else device)
-    precision = options.precision
-    if precision is None:
-        precision = 'mixed' if all(T.dtype == np.float32 for T in tensors) else 'fp64'
-    tdt = torch.float32 if precision == 'mixed' else torch.float64
-    npdt = np.float32 if precision == 'mixed' else np.float64
-    maxiter = int(options.maxiter)
-    if config is None:
-        key = _tune_key(dev, L, dims_sorted, ranks, maxiter)
-        config = _load_tuned(key) or DEFAULT_CONFIG
-    plan = _Plan(dims_sorted, list(ranks), maxiter, config.get('order', 'size'))
-    sz = plan.sizes
-
-    # tensors in sorted mode order, packed, resident on the device
-    host = np.empty(sz['T'], dtype=npdt)
-    for m, (T, ordering) in enumerate(zip(tensors, orderings)):
-        A = np.asarray(T)
-        if any(int(ordering[l]) != l for l in range(L)):
-            A = np.moveaxis(A, ordering, np.arange(L))
-        o = int(plan.meta[m, 6])
-        host[o:o + A.size] = np.ascontiguousarray(A, dtype=npdt).ravel()
-    e = dev.empty
-    Tbuf = torch.from_numpy(host).to(dev.dev)
-    meta = torch.from_numpy(plan.meta).to(dev.dev)
-    order = torch.from_numpy(plan.order).to(dev.dev)
-    kdims = dev.zeros((nb, 4), torch.int32)
-    U = e(max(sz['U'], 1), torch.float64)
-    S = e(max(sz['S'], 1), torch.float64)
-    W = e(max(sz['W'], 1), torch.float64)
-    work = dev.zeros(max(sz['X'], 1), torch.float64)
-    hist = dev.zeros(max(sz['H'], 1), torch.float64)
-    res = dev.zeros(nb * RS_STRIDE, torch.float64)
-    Ffull = e(max(sz['F'], 1), torch.float64)
-    torch.cuda.synchronize()
-    timings = {'upload': time.perf_counter() - t_all}
-    out = _solve(dev, plan, options, seeds, config, precision, tdt, Tbuf, meta, order, kdims, U,
-                 S, W, work, hist, res, Ffull, timings)
-    kd, resh, hh, Fh = out
-    results = []
-    ssum = 0
-    for m in range(nb):
-        R = int(ranks[m])
-        d = dims_sorted[m]
-        fo = int(plan.meta[m, 11])
-        Fm = []
-        for l in range(L):
-            Fm.append(Fh[fo:fo + R * d[l]].reshape(R, d[l]).T.copy())
-            fo += R * d[l]
-        Fm = unsort_dims(Fm, orderings[m])
-        r = resh[m]
-        k = int(r[RS_STEPS])
-        h0 = int(plan.meta[m, 12])
-        errs = hh[h0:h0 + maxiter][:k].copy()
-        steps = hh[h0 + maxiter:h0 + 2 * maxiter][:k].copy()
-        imp = hh[h0 + 2 * maxiter:h0 + 3 * maxiter][:k].copy()
-        grads = hh[h0 + 3 * maxiter:h0 + 4 * maxiter][:k].copy()
-        final_err = float(np.sqrt(r[RS_E2]) / np.sqrt(r[RS_T2]))
-        z = np.array([0])
-        o = Output(k, final_err, [steps, z], [errs, z], [imp, z], [grads, z],
-                   [int(r[RS_STOP]), 8], options, timings=timings if m == 0 else {},
-                   precision=precision)
-        o.final_rel_error = final_err
-        o.compressed_dims = tuple(int(v) for v in kd[m, :L])
-        o.dogleg_calls = int(r[RS_DOGLEG])
-        results.append((Fm, o))
-        ssum += k
-    timings['total'] = time.perf_counter() - t_all
-    return results
+    of (factors, output) pairs.  seeds[m] (optional, 0 .. 2**32 - 1) seeds the global NumPy
+    generator for member m's draws, as `np.random.seed(seeds[m])` before a call of the
+    reference's cpd would.  `config` ({'mode': 'persistent'|'graph', 'order': 'size'|'natural'})
+    overrides the tuned launch configuration.  output.timings of the first member holds the
+    batch's stage times (seconds; 'total' excludes packing and upload, which are 'upload')."""
+    return BatchedCPD(tensors, ranks, options, config, device).run(seeds)
 
 
 def _lds_bytes(plan, kd, maxiter):
@@ -303,7 +333,29 @@ def _solve(dev, plan, options, seeds, config, precision, tdt, Tbuf, meta, order,
     maxiter = int(options.maxiter)
     x0 = np.zeros(plan.sizes['X'])
     caps = np.empty((nb, maxiter), dtype=np.int32)
-    for m in range(nb):
+    redo = range(nb)
+    if seeds is not None:
+        # every member's stream from its own seed, on host threads (libtfxhip, batch_draws.cpp)
+        low, high = _cap_bounds(maxiter)
+        rng = (high - 1 - low).astype(np.int64)
+        sd = np.asarray(seeds, dtype=np.int64)
+        if sd.min() < 0 or sd.max() > 2 ** 32 - 1:
+            raise ValueError('seeds must be between 0 and 2**32 - 1')
+        sd = sd.astype(np.uint32)
+        dims = np.ascontiguousarray(plan.meta[:, 2:2 + L], dtype=np.int32)
+        rk = np.ascontiguousarray(plan.meta[:, 1], dtype=np.int32)
+        kdc = np.ascontiguousarray(kd, dtype=np.int32)
+        xoff = np.ascontiguousarray(plan.meta[:, 9], dtype=np.int64)
+        hz = np.zeros(nb, dtype=np.int32)
+        cp = lambda a: ctypes.c_void_p(a.ctypes.data)
+        rc = dev.lib.tfx_batch_draws(int(nb), int(L), cp(sd), cp(dims), cp(kdc), cp(rk), maxiter,
+                                     cp(low), cp(rng), float(options.cg_factor),
+                                     int(options.inner_method == 'cg_static'),
+                                     int(options.cg_maxiter), float(options.factors_norm),
+                                     cp(xoff), cp(x0), cp(caps), cp(hz), 0)
+        _lib.check(rc)
+        redo = np.nonzero(hz)[0]     # a start entry exactly 0: the reference redraws it
+    for m in redo:
         R = int(plan.meta[m, 1])
         d = tuple(int(v) for v in plan.meta[m, 2:2 + L])
         x, c = _draw_member(None if seeds is None else seeds[m], d, kd[m, :L], R, options, maxiter)
@@ -321,7 +373,6 @@ def _solve(dev, plan, options, seeds, config, precision, tdt, Tbuf, meta, order,
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     timings['init'] = t2 - t1
-    import ctypes
 
     def launch(a, b):
         dev._call('tfx_batch_dgn', int(L), int(nb), _p(meta), _p(order), _p(kdims), _p(S), _p(work),

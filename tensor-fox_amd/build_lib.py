@@ -1,4 +1,5 @@
-"""Builds libtfxhip.so (all HIP sources under csrc/) for gfx950 with hipcc, in-tree.
+"""Builds libtfxhip.so (the HIP sources and host C++ sources under csrc/) for gfx950 with hipcc,
+in-tree.
 
     python tensor-fox_amd/build_lib.py            # -> tensor-fox_amd/TensorFox/libtfxhip.so
 """
@@ -33,7 +34,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.hip'))
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(('.hip', '.cpp')))
     workers = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with concurrent.futures.ThreadPoolExecutor(workers) as ex:
         objs = list(ex.map(_compile, srcs))

@@ -143,3 +143,43 @@ def test_host_equalize_bitwise_equals_oracle():
         A = o.equalize([f.copy() for f in F], R)
         B = _equalize([f.copy() for f in F], R)
         assert all(np.array_equal(a, b) for a, b in zip(A, B))
+
+
+def test_native_batch_draws_match_numpy_stream():
+    """tfx_batch_draws (host C++, MT19937 + polar Gaussians + masked bounded integers) against
+    NumPy's legacy global generator driven by the Python replay: identical CG caps (drawn after
+    every Gaussian, so any drift in the stream would show) and the same equalized start point up
+    to rounding (the equalization sums differ from BLAS's)."""
+    import ctypes
+    import TensorFox as tfx
+    from TensorFox import _lib
+    from TensorFox.batched import _cap_bounds, _draw_member
+    lib = _lib.load()
+    rs = np.random.RandomState(9)
+    nb, maxiter = 40, 60
+    L = 3
+    dims = np.sort(rs.randint(8, 65, (nb, L)), axis=1)[:, ::-1].astype(np.int32).copy()
+    ranks = rs.randint(2, 17, nb).astype(np.int32)
+    kd = np.ones((nb, 4), dtype=np.int32)
+    for m in range(nb):
+        kd[m, :L] = np.minimum(ranks[m], dims[m])
+    kd[3, 1] = max(1, kd[3, 1] - 1)                  # a truncated mode changes later sketch shapes
+    seeds = (rs.randint(0, 2 ** 31, nb)).astype(np.uint32)
+    nx = np.array([ranks[m] * kd[m, :L].sum() for m in range(nb)], dtype=np.int64)
+    xoff = np.concatenate([[0], np.cumsum(nx)[:-1]]).astype(np.int64)
+    x0 = np.zeros(int(nx.sum()))
+    caps = np.zeros((nb, maxiter), dtype=np.int32)
+    hz = np.zeros(nb, dtype=np.int32)
+    low, high = _cap_bounds(maxiter)
+    rng = (high - 1 - low).astype(np.int64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = lib.tfx_batch_draws(nb, L, p(seeds), p(dims), p(kd), p(ranks), maxiter, p(low), p(rng),
+                             1.0, 0, 100, 0.0, p(xoff), p(x0), p(caps), p(hz), 4)
+    assert rc == 0
+    opts = tfx.make_options(False)
+    for m in range(nb):
+        x, c = _draw_member(int(seeds[m]), tuple(dims[m]), kd[m, :L], int(ranks[m]), opts, maxiter)
+        assert np.array_equal(c, caps[m]), m
+        got = x0[xoff[m]:xoff[m] + nx[m]]
+        assert np.abs(got - x).max() <= 1e-13 * np.abs(x).max(), m
+    assert not hz.any()

@@ -106,3 +106,26 @@ def test_batched_rejects_unsupported():
         tfx.cpd_batched([T], [3])
     with pytest.raises(NotImplementedError):
         tfx.cpd_batched([T[:10]], [3], make_opts(dict(refine=True)))
+
+
+def test_batched_4096_members_sampled_against_oracle():
+    """The full batch: 4096 FP32 tensors, dims U{8..64}, rank U{2..16}, noise 1e-4, member m
+    seeded with m.  Size-independent checks over every member (finite errors, histories of the
+    reported length, stop codes 0-7) and oracle parity on sampled members, the largest included."""
+    import TensorFox as tfx
+    from bench import make_batch
+    print('batched 4096', flush=True)
+    tensors, ranks = make_batch(4096)
+    seeds = list(range(4096))
+    res = tfx.cpd_batched(tensors, ranks, None, seeds)
+    errs = np.array([out.rel_error for _, out in res])
+    assert np.all(np.isfinite(errs))
+    assert np.median(errs) < 2e-4
+    for f, out in res:
+        assert out.stop[0] in range(8) and len(out.errors[0]) == out.num_steps
+    sizes = [np.prod(T.shape) * R for T, R in zip(tensors, ranks)]
+    sample = sorted(set([int(np.argmax(sizes))] + list(range(0, 4096, 683))))
+    for m in sample:
+        np.random.seed(seeds[m])
+        fo, oo = o.cpd(tensors[m].astype(np.float64), ranks[m])
+        assert abs(res[m][1].rel_error - oo.rel_error) < 1e-6, (m, tensors[m].shape, ranks[m])
