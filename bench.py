@@ -53,6 +53,9 @@ def parse():
     ap.add_argument('--no-order4', action='store_true', help='skip the order-4 96^4 problem')
     ap.add_argument('--no-batched', action='store_true', help='skip the 4096-member batch')
     ap.add_argument('--batch-size', type=int, default=4096)
+    ap.add_argument('--no-pipeline', action='store_true',
+                    help='skip the 400^3 pipeline (and its FP64, column-scaled and CPU legs): time only '
+                         'the other sections, e.g. to profile one of them')
     ap.add_argument('--cpu-sample-iters', type=int, default=1)
     ap.add_argument('--kernel-reps', type=int, default=20)
     ap.add_argument('--kernel-only', action='store_true',
@@ -300,89 +303,93 @@ def main():
     from TensorFox import _lib
     from TensorFox.device import get_device
 
-    dims, R, nu = tuple(args.dims), args.rank, args.noise
-    log('generating', dims, R, nu)
-    T, Ftrue = tfx.make_lowrank(dims, R, nu, seed=args.seed, dtype=np.float32)
     dev = get_device(0)
-    Tdev = torch.from_numpy(T).to(dev.dev)
-    torch.cuda.synchronize()
-    T0 = tfx.cpd2tens(Ftrue)
-    T0n = float(np.linalg.norm(T0))
+    result = {'metric': 'time to rel.err 1e-6, 400^3 R=50: mixed vs FP64 GPU dGN; MTTKRP GB/s',
+              'value': None, 'unit': 's', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+              'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None}
+    if not args.no_pipeline:
+        dims, R, nu = tuple(args.dims), args.rank, args.noise
+        log('generating', dims, R, nu)
+        T, Ftrue = tfx.make_lowrank(dims, R, nu, seed=args.seed, dtype=np.float32)
+        Tdev = torch.from_numpy(T).to(dev.dev)
+        torch.cuda.synchronize()
+        T0 = tfx.cpd2tens(Ftrue)
+        T0n = float(np.linalg.norm(T0))
 
-    for w in range(args.warmup):
-        t, _, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
-        log('warmup %d: %.3f s rel_error %.3e' % (w, t, out.rel_error))
-    times, outs, facs = [], [], []
-    torch.cuda.synchronize()
-    for k in range(args.steps):
-        t, f, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
-        times.append(t)
-        outs.append(out)
-        facs.append(f)
-        log('step %d: %.3f s rel_error %.6e final %.6e steps %s stop %s timings %s' % (
-            k, t, out.rel_error, out.final_rel_error, out.num_steps, out.stop,
-            {a: round(b, 4) for a, b in out.timings.items()}))
-    ms = 1e3 * float(np.mean(times))
-    out = outs[-1]
-    model_err = float(np.linalg.norm(tfx.cpd2tens(facs[-1]) - T0)) / T0n
+        for w in range(args.warmup):
+            t, _, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
+            log('warmup %d: %.3f s rel_error %.3e' % (w, t, out.rel_error))
+        times, outs, facs = [], [], []
+        torch.cuda.synchronize()
+        for k in range(args.steps):
+            t, f, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
+            times.append(t)
+            outs.append(out)
+            facs.append(f)
+            log('step %d: %.3f s rel_error %.6e final %.6e steps %s stop %s timings %s' % (
+                k, t, out.rel_error, out.final_rel_error, out.num_steps, out.stop,
+                {a: round(b, 4) for a, b in out.timings.items()}))
+        ms = 1e3 * float(np.mean(times))
+        out = outs[-1]
+        model_err = float(np.linalg.norm(tfx.cpd2tens(facs[-1]) - T0)) / T0n
 
-    result = {
-        'metric': 'time to rel.err 1e-6, 400^3 R=50: mixed vs FP64 GPU dGN; MTTKRP GB/s',
-        'value': round(ms / 1e3, 5),
-        'unit': 's',
-        'n_gpus': 1,
-        'steps': args.steps,
-        'warmup': args.warmup,
-        'ms_per_step': round(ms, 3),
-        'higher_is_better': False,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'f32+f64',
-        'data': 'synthetic (seeded low-rank + 1e-5 relative noise)',
-        'config': {'workload': 'dense CPD %s rank %d, noise %g, seed %d, refine, mixed precision' % (
-                       'x'.join(map(str, dims)), R, nu, args.seed),
-                   'dims': list(dims), 'rank': R, 'noise': nu, 'seed': args.seed, 'refine': True,
-                   'precision': 'mixed'},
-        'rel_error': out.rel_error,
-        'final_rel_error': out.final_rel_error,
-        'model_error_vs_noise_free': model_err,
-        'num_steps': int(out.num_steps),
-        'stage_seconds': {a: round(b, 5) for a, b in out.timings.items()},
-    }
+        result = {
+            'metric': 'time to rel.err 1e-6, 400^3 R=50: mixed vs FP64 GPU dGN; MTTKRP GB/s',
+            'value': round(ms / 1e3, 5),
+            'unit': 's',
+            'n_gpus': 1,
+            'steps': args.steps,
+            'warmup': args.warmup,
+            'ms_per_step': round(ms, 3),
+            'higher_is_better': False,
+            'scaling': 'weak',
+            'vs_baseline': None,
+            'dtype': 'f32+f64',
+            'data': 'synthetic (seeded low-rank + 1e-5 relative noise)',
+            'config': {'workload': 'dense CPD %s rank %d, noise %g, seed %d, refine, mixed precision' % (
+                           'x'.join(map(str, dims)), R, nu, args.seed),
+                       'dims': list(dims), 'rank': R, 'noise': nu, 'seed': args.seed, 'refine': True,
+                       'precision': 'mixed'},
+            'rel_error': out.rel_error,
+            'final_rel_error': out.final_rel_error,
+            'model_error_vs_noise_free': model_err,
+            'num_steps': int(out.num_steps),
+            'stage_seconds': {a: round(b, 5) for a, b in out.timings.items()},
+        }
 
-    # dominant kernel: fused MTTKRP over the full tensor
-    kt = mttkrp_kernel_time(dev, Tdev, dims, R, _lib.TFX_PREC_SPLIT16, args.kernel_reps)
-    N = int(np.prod(dims))
-    alg_bytes = 4 * N + 8 * R * sum(dims) * 2
-    gbs = alg_bytes / kt / 1e9
-    result['roofline'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                          'frac': round(gbs / HBM_PEAK_GBS, 4),
-                          'traffic': MEASURED_TRAFFIC.get((dims, R)),
-                          'kernel': 'tfx_mttkrp split16 (all modes, FP32 tensor, bf16x3 MFMA)', 'avg_launch_s': kt}
+        # dominant kernel: fused MTTKRP over the full tensor
+        kt = mttkrp_kernel_time(dev, Tdev, dims, R, _lib.TFX_PREC_SPLIT16, args.kernel_reps)
+        N = int(np.prod(dims))
+        alg_bytes = 4 * N + 8 * R * sum(dims) * 2
+        gbs = alg_bytes / kt / 1e9
+        result['roofline'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                              'frac': round(gbs / HBM_PEAK_GBS, 4),
+                              'traffic': MEASURED_TRAFFIC.get((dims, R)),
+                              'kernel': 'tfx_mttkrp split16 (all modes, FP32 tensor, bf16x3 MFMA)', 'avg_launch_s': kt}
 
-    if not args.no_fp64:
-        T64 = Tdev.to(torch.float64)          # the FP64 form's input, resident before timing
-        time_cpd(tfx, T64, R, 'fp64', args.seed)
-        t64, f64, out64 = time_cpd(tfx, T64, R, 'fp64', args.seed)
-        del T64
-        log('fp64: %.3f s rel_error %.6e timings %s' % (t64, out64.rel_error, out64.timings))
-        result['fp64_seconds'] = round(t64, 5)
-        result['fp64_rel_error'] = out64.rel_error
-        result['speedup_mixed_vs_fp64'] = round(t64 / (ms / 1e3), 3)
-        result['abs_err_diff_vs_fp64'] = abs(out.rel_error - out64.rel_error)
-
-    if not args.no_colscale:
-        Tc, Fc = tfx.make_lowrank(dims, R, nu, seed=0, col_scale=True, dtype=np.float32)
-        Tcd = torch.from_numpy(Tc).to(dev.dev)
-        tm, _, om = time_cpd(tfx, Tcd, R, 'mixed', 0)
-        res = {'workload': 'same, first factor columns x 10^U(-1,1), seed 0', 'mixed_seconds': round(tm, 5),
-               'mixed_rel_error': om.rel_error}
         if not args.no_fp64:
-            t6, _, o6 = time_cpd(tfx, Tcd.to(torch.float64), R, 'fp64', 0)
-            res.update(fp64_seconds=round(t6, 5), fp64_rel_error=o6.rel_error)
-        log('colscale instance:', res)
-        result['colscale_instance'] = res
-        del Tcd
+            T64 = Tdev.to(torch.float64)          # the FP64 form's input, resident before timing
+            time_cpd(tfx, T64, R, 'fp64', args.seed)
+            t64, f64, out64 = time_cpd(tfx, T64, R, 'fp64', args.seed)
+            del T64
+            log('fp64: %.3f s rel_error %.6e timings %s' % (t64, out64.rel_error, out64.timings))
+            result['fp64_seconds'] = round(t64, 5)
+            result['fp64_rel_error'] = out64.rel_error
+            result['speedup_mixed_vs_fp64'] = round(t64 / (ms / 1e3), 3)
+            result['abs_err_diff_vs_fp64'] = abs(out.rel_error - out64.rel_error)
+
+        if not args.no_colscale:
+            Tc, Fc = tfx.make_lowrank(dims, R, nu, seed=0, col_scale=True, dtype=np.float32)
+            Tcd = torch.from_numpy(Tc).to(dev.dev)
+            tm, _, om = time_cpd(tfx, Tcd, R, 'mixed', 0)
+            res = {'workload': 'same, first factor columns x 10^U(-1,1), seed 0', 'mixed_seconds': round(tm, 5),
+                   'mixed_rel_error': om.rel_error}
+            if not args.no_fp64:
+                t6, _, o6 = time_cpd(tfx, Tcd.to(torch.float64), R, 'fp64', 0)
+                res.update(fp64_seconds=round(t6, 5), fp64_rel_error=o6.rel_error)
+            log('colscale instance:', res)
+            result['colscale_instance'] = res
+            del Tcd
 
     if not args.no_loss:
         result['cpdloss'] = cpdloss_time(tfx, dev, args.kernel_reps)
@@ -396,7 +403,7 @@ def main():
         result['order4'] = order4_time(tfx, dev)
         log('order4:', result['order4'])
 
-    if not args.no_cpu:
+    if not args.no_cpu and not args.no_pipeline:
         cores = os.cpu_count()
         log('cpu baseline on %d cores' % cores)
         c = cpu_baseline(T, R, args.cpu_sample_iters, args.seed)
