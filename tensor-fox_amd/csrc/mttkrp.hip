@@ -132,68 +132,109 @@ __global__ __launch_bounds__(256) void k_mttkrp_rows(const TS* __restrict__ T, S
   }
 }
 
-// Pass 2: N_{L-1}[k, rc0 + r] = sum_w part[w][k][r].
+// Pass 2: N_{L-1}[k, rc0 + r] = sum_w part[w][k][r].  One workgroup per k: thread (g, r) sums
+// w = g, g + 8, .. (the loads of a step are contiguous over r), then the 8 sums are added in
+// order of g.
 template <typename S>
-__global__ void k_mttkrp_reduce_last(Shape s, int rc0, int nwg, const S* __restrict__ part,
-                                     S* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_mttkrp_reduce_last(Shape s, int rc0, int nwg,
+                                                             const S* __restrict__ part,
+                                                             S* __restrict__ out) {
   constexpr int RC = RCOf<S>::v;
+  constexpr int NG = 256 / RC;
+  __shared__ S red[NG][RC];
   const int64_t K = s.d[s.L - 1];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= K * RC) return;
-  const int64_t k = e / RC;
-  const int r = (int)(e % RC);
-  if (rc0 + r >= s.R) return;
+  const int64_t k = blockIdx.x;
+  const int r = threadIdx.x % RC, g = threadIdx.x / RC;
   S acc = S(0);
-  for (int w = 0; w < nwg; ++w) acc += part[(int64_t)w * K * RC + e];
-  out[s.off[s.L - 1] + (int64_t)(rc0 + r) * K + k] = acc;
+  const S* p = part + k * RC + r;
+  for (int w = g; w < nwg; w += NG) acc += p[(int64_t)w * K * RC];
+  red[g][r] = acc;
+  __syncthreads();
+  if (g == 0 && rc0 + r < s.R) {
+    S tot = red[0][r];
+    for (int q = 1; q < NG; ++q) tot += red[q][r];
+    out[s.off[s.L - 1] + (int64_t)(rc0 + r) * K + k] = tot;
+  }
 }
 
-// Pass 3: N_m for m < L-1 from Y.  One workgroup per (m, i_m); mode m is skipped unless its
-// bit is set in mask.
+// Pass 3: N_m for m < L-1 from Y (NR x R, row-major).  One workgroup per (m, i_m); lane = rank
+// (chunks of 64), the eight waves split the rows with i_m = i.  The rows are walked as an outer
+// index over the row modes other than m and the innermost varying mode qi (the last row mode,
+// or the one before it when m is the last), decomposed once per outer step with 32-bit
+// arithmetic, around an inner loop over qi; the product of the outer modes' factor entries is
+// formed once per outer step.
 template <typename S>
-__global__ __launch_bounds__(256) void k_mttkrp_rowreduce(Shape s, const S* __restrict__ F,
+__global__ __launch_bounds__(512) void k_mttkrp_rowreduce(Shape s, const S* __restrict__ F,
                                                            const S* __restrict__ Y,
                                                            unsigned mask, S* __restrict__ out) {
-  __shared__ S red[4][64];
+  constexpr int NWV = 8;
+  __shared__ S red[NWV][64];
   const int L = s.L, R = s.R;
-  int64_t b = blockIdx.x;
+  const int LR = L - 1;                       // row modes 0..LR-1
+  int b = blockIdx.x;
   int m = 0;
-  while (m < L - 1 && b >= s.d[m]) {
-    b -= s.d[m];
+  while (m < LR && b >= s.d[m]) {
+    b -= (int)s.d[m];
     ++m;
   }
-  if (m >= L - 1 || !((mask >> m) & 1u)) return;
-  const int64_t i = b;
-  const int64_t NR = s.N / s.d[L - 1];
-  const int64_t cnt = NR / s.d[m];
-  // row strides over modes 0..L-2
-  int64_t rs[TFX_MAXL];
-  rs[L - 2] = 1;
-  for (int q = L - 3; q >= 0; --q) rs[q] = rs[q + 1] * s.d[q + 1];
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  if (m >= LR || !((mask >> m) & 1u)) return;
+  const int i = b;
+  int rs[TFX_MAXL];                           // row strides over the row modes
+  rs[LR - 1] = 1;
+  for (int q = LR - 2; q >= 0; --q) rs[q] = rs[q + 1] * (int)s.d[q + 1];
+  const int qi = LR == 1 ? -1 : (m == LR - 1 ? LR - 2 : LR - 1);
+  int n_outer = 1;
+  for (int q = 0; q < LR; ++q)
+    if (q != m && q != qi) n_outer *= (int)s.d[q];
+  const int ni = qi >= 0 ? (int)s.d[qi] : 1;
+  const int si = qi >= 0 ? rs[qi] : 0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int r0 = 0; r0 < R; r0 += 64) {
     const int r = r0 + lane;
+    const bool rok = r < R;
+    const int rc = rok ? r : 0;
     S acc = S(0);
-    if (r < R) {
-      for (int64_t e = grp; e < cnt; e += 4) {
-        // decompose e over the row modes other than m (last fastest)
-        int64_t rem = e, rho = i * rs[m];
-        S prod = S(1);
-        for (int q = L - 2; q >= 0; --q) {
-          if (q == m) continue;
-          const int64_t iq = rem % s.d[q];
-          rem /= s.d[q];
-          rho += iq * rs[q];
-          prod *= F[s.off[q] + (int64_t)r * s.d[q] + iq];
-        }
-        acc += Y[rho * R + r] * prod;
+    for (int ot = wv; ot < n_outer; ot += NWV) {
+      int rem = ot, base = i * rs[m];
+      S pre = S(1);
+      for (int q = LR - 1; q >= 0; --q) {
+        if (q == m || q == qi) continue;
+        const int dq = (int)s.d[q];
+        const int iq = rem % dq;
+        rem /= dq;
+        base += iq * rs[q];
+        pre *= F[s.off[q] + (int64_t)rc * dq + iq];
       }
+      S inner = S(0);
+      if (qi >= 0) {
+        // four independent partial sums: four loads of Y and F in flight per step
+        const S* fq = F + s.off[qi] + (int64_t)rc * ni;
+        const S* yp = Y + (int64_t)base * R + rc;
+        const int64_t ys = (int64_t)si * R;
+        S p0 = S(0), p1 = S(0), p2 = S(0), p3 = S(0);
+        int iq = 0;
+        for (; iq + 4 <= ni; iq += 4) {
+          p0 += yp[iq * ys] * fq[iq];
+          p1 += yp[(iq + 1) * ys] * fq[iq + 1];
+          p2 += yp[(iq + 2) * ys] * fq[iq + 2];
+          p3 += yp[(iq + 3) * ys] * fq[iq + 3];
+        }
+        for (; iq < ni; ++iq) p0 += yp[iq * ys] * fq[iq];
+        inner = (p0 + p1) + (p2 + p3);
+      } else {
+        inner = Y[(int64_t)base * R + rc];
+      }
+      acc += pre * inner;
     }
     __syncthreads();
-    red[grp][lane] = acc;
+    red[wv][lane] = rok ? acc : S(0);
     __syncthreads();
-    if (grp == 0 && r < R)
-      out[s.off[m] + (int64_t)r * s.d[m] + i] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    if (wv == 0 && rok) {
+      S tot = red[0][lane];
+#pragma unroll
+      for (int q = 1; q < NWV; ++q) tot += red[q][lane];
+      out[s.off[m] + (int64_t)r * s.d[m] + i] = tot;
+    }
   }
 }
 
@@ -275,6 +316,8 @@ int mttkrp_native(tfx_ctx* ctx, const TS* T, const Shape& s, const S* F, unsigne
   const int L = s.L, R = s.R;
   const int64_t K = s.d[L - 1];
   const int64_t NR = s.N / K;
+  if (NR > 0x7fffffff)
+    return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "MTTKRP (FMA form) needs fewer than 2^31 rows");
   const int64_t nblocks = (NR + RB - 1) / RB;
   int nwg = (int)std::min<int64_t>(nblocks, (int64_t)ctx->cu_count);
   if (nwg < 1) nwg = 1;
@@ -300,16 +343,15 @@ int mttkrp_native(tfx_ctx* ctx, const TS* T, const Shape& s, const S* F, unsigne
                        rows_per_wg, Y, part, np_in_lds);
     TFX_CHECK_LAUNCH(ctx);
     if ((mask >> (L - 1)) & 1u) {
-      const int64_t tot = K * RC;
-      hipLaunchKernelGGL(k_mttkrp_reduce_last<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                         0, stream, s, rc0, nwg, part, out);
+      hipLaunchKernelGGL(k_mttkrp_reduce_last<S>, dim3((unsigned)K), dim3(256), 0, stream, s, rc0,
+                         nwg, part, out);
       TFX_CHECK_LAUNCH(ctx);
     }
   }
   if (need_rows) {
     int64_t nb = 0;
     for (int m = 0; m < L - 1; ++m) nb += s.d[m];
-    hipLaunchKernelGGL(k_mttkrp_rowreduce<S>, dim3((unsigned)nb), dim3(256), 0, stream, s, F, Y,
+    hipLaunchKernelGGL(k_mttkrp_rowreduce<S>, dim3((unsigned)nb), dim3(512), 0, stream, s, F, Y,
                        mask, out);
     TFX_CHECK_LAUNCH(ctx);
   }
