@@ -33,13 +33,18 @@ def clean_compression(sigma, tol_mlsvd, L):
     return len(sigma)
 
 
+def _data_prec(X, prec):
+    """The split precision applies to FP32 data only; FP64 intermediates run FP64 MFMA."""
+    return prec if X.dtype == torch.float32 else _lib.TFX_PREC_NATIVE
+
+
 def _top_eigvecs(dev, X, dims, l, n_comp, prec, draw):
     """U_l (I_l x n_comp, FP64 device, column-major) and eigenvalues of the mode-l Gram matrix.
     `draw` is the sketch drawn from the global RNG for this mode (shape as the reference's)."""
     I = dims[l]
     p = n_comp + 10
     G = dev.empty((I, I), torch.float64)
-    dev.gram_mode(X, dims, l, G, prec=prec)
+    dev.gram_mode(X, dims, l, G, prec=_data_prec(X, prec))
     U = dev.empty((n_comp, I), torch.float64)       # column-major I x n_comp
     lam = dev.empty(n_comp, torch.float64)
     if I <= p:
@@ -81,14 +86,14 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
             new_dims[l] = n_comp
             Snew = dev.empty(int(np.prod(new_dims)), wdt)
             sig2 = dev.empty(n_comp, torch.float64)
-            dev.ttm(S, S_dims, l, U, n_comp, Snew, sig2, prec=prec)
+            dev.ttm(S, S_dims, l, U, n_comp, Snew, sig2, prec=_data_prec(S, prec))
             sigma = np.sqrt(np.maximum(sig2.cpu().numpy(), 0.0))
             dim = clean_compression(sigma, tol_mlsvd, L)
             if dim < n_comp:
                 U = U[:dim].contiguous()
                 new_dims[l] = dim
                 Snew = dev.empty(int(np.prod(new_dims)), wdt)
-                dev.ttm(S, S_dims, l, U, dim, Snew, None, prec=prec)
+                dev.ttm(S, S_dims, l, U, dim, Snew, None, prec=_data_prec(S, prec))
                 sigma = sigma[:dim]
             U_list.append(U)
             sig_list.append(sigma)
@@ -111,7 +116,7 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
             new_dims = list(S_dims)
             new_dims[l] = k
             Snew = dev.empty(int(np.prod(new_dims)), wdt)
-            dev.ttm(S, S_dims, l, U_list[l], k, Snew, None, prec=prec)
+            dev.ttm(S, S_dims, l, U_list[l], k, Snew, None, prec=_data_prec(S, prec))
             S, S_dims = Snew, new_dims
     else:
         raise ValueError("mlsvd_method must be 'seq', 'classic' or 'gpu' on the dense path")

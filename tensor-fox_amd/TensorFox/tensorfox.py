@@ -127,6 +127,8 @@ def tricpd(T, R, options):
     wdt = torch.float64                                   # factors, CG, core
     tdt = torch.float32 if precision == 'mixed' else torch.float64   # the tensor in HBM
     prec = _lib.TFX_PREC_NATIVE
+    # compression of an FP32 tensor: bf16x3 MFMA Gram, FP32 MFMA mode product
+    mprec = _lib.TFX_PREC_SPLIT16 if precision == 'mixed' else _lib.TFX_PREC_NATIVE
     L = T.ndim
     dims, ordering = sort_dims(T.shape)
     Tdev = _to_device(dev, T, ordering, tdt)
@@ -141,7 +143,7 @@ def tricpd(T, R, options):
     timings['upload'] = t0 - t_all
 
     # compression
-    S, S_dims, U, sigmas = mlsvd(dev, Tdev, dims, R, options, wdt, prec)
+    S, S_dims, U, sigmas = mlsvd(dev, Tdev, dims, R, options, wdt, mprec)
     if options.symm:
         R_min = min(S_dims)
         view = S.view(*S_dims)

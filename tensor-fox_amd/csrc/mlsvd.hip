@@ -1,7 +1,8 @@
 // MLSVD building blocks (Compression.py:48-237): Gram matrix of a mode unfolding, top-k FP64
 // eigenpairs by subspace iteration + Rayleigh-Ritz (parallel cyclic Jacobi), the mode product
 // Y = X x_mode U^T with the squared row norms of its unfolding, and a small GEMM for the
-// uncompression U_l W_l.
+// uncompression U_l W_l.  The Gram and mode-product kernels here are the FMA forms for FP32 data
+// at native precision; FP64 data and the split precision run the MFMA kernels of mlsvd_mfma.hip.
 //
 // A tensor X with dims d[0..L-1] is viewed around `mode` as (P, I, Q):
 //   P = prod_{m < mode} d[m], I = d[mode], Q = prod_{m > mode} d[m],
@@ -421,9 +422,15 @@ int ttm_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, const double* U, i
 
 }  // namespace
 
+// MFMA kernels (mlsvd_mfma.hip): bf16x3 Gram and FP32-MFMA product on FP32 data, FP64 MFMA on
+// FP64 data.
 int tfx_gram_split16(tfx_ctx* ctx, const float* X, const Shape& s, int mode, double* G,
                      hipStream_t st);
 int tfx_ttm_split16(tfx_ctx* ctx, const float* X, const Shape& s, int mode, const double* U,
+                    int k, double* Y, double* sig2, hipStream_t st);
+int tfx_gram_f64mfma(tfx_ctx* ctx, const double* X, const Shape& s, int mode, double* G,
+                     hipStream_t st);
+int tfx_ttm_f64mfma(tfx_ctx* ctx, const double* X, const Shape& s, int mode, const double* U,
                     int k, double* Y, double* sig2, hipStream_t st);
 
 extern "C" int tfx_gram_mode(tfx_ctx* ctx, const void* X, int dtype, int L, const int64_t* dims,
@@ -438,7 +445,7 @@ extern "C" int tfx_gram_mode(tfx_ctx* ctx, const void* X, int dtype, int L, cons
     return tfx_gram_split16(ctx, (const float*)X, s, mode, G, stream);
   }
   if (dtype == TFX_F32) return gram_t<float>(ctx, (const float*)X, s, mode, G, stream);
-  if (dtype == TFX_F64) return gram_t<double>(ctx, (const double*)X, s, mode, G, stream);
+  if (dtype == TFX_F64) return tfx_gram_f64mfma(ctx, (const double*)X, s, mode, G, stream);
   return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
 }
 
@@ -532,7 +539,7 @@ extern "C" int tfx_ttm_mixed(tfx_ctx* ctx, const void* X, int x_dtype, int L, co
   }
   if (x_dtype == TFX_F32 && y_dtype == TFX_F32) return ttm_t<float, float>(ctx, (const float*)X, s, mode, U, k, (float*)Y, sig2, stream);
   if (x_dtype == TFX_F32 && y_dtype == TFX_F64) return ttm_t<float, double>(ctx, (const float*)X, s, mode, U, k, (double*)Y, sig2, stream);
-  if (x_dtype == TFX_F64 && y_dtype == TFX_F64) return ttm_t<double, double>(ctx, (const double*)X, s, mode, U, k, (double*)Y, sig2, stream);
+  if (x_dtype == TFX_F64 && y_dtype == TFX_F64) return tfx_ttm_f64mfma(ctx, (const double*)X, s, mode, U, k, (double*)Y, sig2, stream);
   return tfx_set_error(ctx, TFX_E_ARG, "unsupported dtype pair %d/%d", x_dtype, y_dtype);
 }
 

@@ -352,8 +352,12 @@ def test_dgn_update(dev, dt, symm, fnorm):
     assert s[3] == pytest.approx(np.abs(g).max(), rel=tol * 10)
 
 
+GRAM_CASES = [(40, 30, 20), (5, 6, 7, 8), (64, 64, 1), (3, 100, 2), (300, 40, 24), (130, 7, 9),
+              (37, 45, 29), (96, 24, 24, 8)]
+
+
 @pytest.mark.parametrize('dt', [torch.float64, torch.float32])
-@pytest.mark.parametrize('dims', [(40, 30, 20), (5, 6, 7, 8), (64, 64, 1), (3, 100, 2)])
+@pytest.mark.parametrize('dims', GRAM_CASES)
 def test_gram_mode(dev, dims, dt):
     print('gram', dims, dt, flush=True)
     X = np.random.RandomState(8).randn(*dims)
@@ -363,6 +367,23 @@ def test_gram_mode(dev, dims, dt):
         dev.gram_mode(Xd, dims, l, G)
         U = o.unfold(X, l + 1)
         assert _rel(G.cpu().numpy(), U @ U.T) < TOL[dt] * 5
+
+
+@pytest.mark.parametrize('dims', GRAM_CASES)
+def test_gram_mode_split16(dev, dims):
+    """bf16x3 MFMA Gram of FP32 data against the FP64 Gram of the same FP32 values: every
+    product keeps hi*hi + hi*lo + lo*hi (about 2^-16 relative each), FP32 sums per slice."""
+    from TensorFox import _lib
+    print('gram split16', dims, flush=True)
+    X = np.random.RandomState(12).randn(*dims).astype(np.float32)
+    Xd = dev.upload(X.ravel(), torch.float32)
+    for l in range(len(dims)):
+        G = dev.empty((dims[l], dims[l]), torch.float64)
+        dev.gram_mode(Xd, dims, l, G, prec=_lib.TFX_PREC_SPLIT16)
+        U = o.unfold(X.astype(np.float64), l + 1)
+        g = G.cpu().numpy()
+        assert np.array_equal(g, g.T)
+        assert _rel(g, U @ U.T) < 1e-5
 
 
 @pytest.mark.parametrize('n,k,p,kind', [(40, 5, 15, 'gap'), (400, 50, 60, 'gap'), (12, 3, 13, 'full'),
@@ -393,9 +414,13 @@ def test_eig_top(dev, n, k, p, kind):
         assert Ug[np.argmax(np.abs(Ug[:, c])), c] > 0
 
 
+TTM_CASES = [((20, 30, 40), 0, 5), ((20, 30, 40), 1, 7), ((20, 30, 40), 2, 3), ((4, 5, 6, 7), 2, 4),
+             ((400, 50, 8), 0, 50), ((33, 70, 6), 1, 70), ((50, 50, 37), 2, 20), ((9, 130), 1, 17),
+             ((20, 30, 41), 1, 12), ((24, 24, 24, 96), 3, 24), ((3, 2), 0, 1)]
+
+
 @pytest.mark.parametrize('dt', [torch.float64, torch.float32])
-@pytest.mark.parametrize('dims,mode,k', [((20, 30, 40), 0, 5), ((20, 30, 40), 1, 7), ((20, 30, 40), 2, 3),
-                                         ((4, 5, 6, 7), 2, 4)])
+@pytest.mark.parametrize('dims,mode,k', TTM_CASES)
 def test_ttm_and_row_norms(dev, dims, mode, k, dt):
     rs = np.random.RandomState(10)
     X = rs.randn(*dims)
@@ -410,6 +435,29 @@ def test_ttm_and_row_norms(dev, dims, mode, k, dt):
     ref = np.moveaxis(np.tensordot(U.T, np.moveaxis(X, mode, 0), axes=1), 0, mode)
     got = Y.cpu().double().numpy().reshape(ydims)
     assert _rel(got, ref) < TOL[dt] * 5
+    assert _rel(sig2.cpu().numpy(), np.sum(o.unfold(got, mode + 1) ** 2, axis=1)) < 1e-10
+
+
+@pytest.mark.parametrize('dims,mode,k', TTM_CASES)
+def test_ttm_split16_and_row_norms(dev, dims, mode, k):
+    """FP32-MFMA mode product of FP32 data (U rounded to FP32, FP32 sums) into FP64, against the
+    FP64 product of the same FP32 values; sig2 equals the row norms of the returned Y."""
+    from TensorFox import _lib
+    print('ttm split16', dims, mode, k, flush=True)
+    rs = np.random.RandomState(13)
+    X = rs.randn(*dims).astype(np.float32)
+    U = rs.randn(dims[mode], k)
+    Xd = dev.upload(X.ravel(), torch.float32)
+    Ud = dev.upload(U.ravel(order='F'), torch.float64)
+    ydims = list(dims)
+    ydims[mode] = k
+    Y = dev.empty(int(np.prod(ydims)), torch.float64)
+    sig2 = dev.empty(k, torch.float64)
+    dev.ttm(Xd, dims, mode, Ud, k, Y, sig2, prec=_lib.TFX_PREC_SPLIT16)
+    X64 = X.astype(np.float64)
+    ref = np.moveaxis(np.tensordot(U.T, np.moveaxis(X64, mode, 0), axes=1), 0, mode)
+    got = Y.cpu().numpy().reshape(ydims)
+    assert _rel(got, ref) < 1e-5
     assert _rel(sig2.cpu().numpy(), np.sum(o.unfold(got, mode + 1) ** 2, axis=1)) < 1e-10
 
 
