@@ -61,6 +61,9 @@ def parse():
     ap.add_argument('--kernel-only', action='store_true',
                     help='time only the fused MTTKRP on a random resident tensor of the same shape, '
                          'and a plain device copy of the tensor (the reference for HBM counters)')
+    ap.add_argument('--compression-only', action='store_true',
+                    help='time only the MLSVD mode Gram and mode product on a random resident tensor '
+                         'of the same shape')
     return ap.parse_args()
 
 
@@ -131,6 +134,74 @@ def kernel_only(args):
            'frac': round(alg_bytes / kt / 1e9 / HBM_PEAK_GBS, 4),
            'copy_avg_s': ct, 'copy_GBs': round(2 * 4 * N / ct / 1e9, 1)}
     print(json.dumps(res), flush=True)
+
+
+BF16_MFMA_PEAK_TFLOPS = 2500.0    # MI355X dense bf16 MFMA
+FP32_MFMA_PEAK_TFLOPS = 157.3     # MI355X dense FP32 MFMA
+
+
+def _event_time(fn, reps, warm=3):
+    import torch
+    for _ in range(warm):
+        fn()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def compression_time(dev, Tdev, dims, R, reps):
+    """The MLSVD's two passes over the FP32 tensor (mode 0, as in the first ST-HOSVD step), timed
+    with HIP events on the stream the kernels run on:
+    - the bf16x3 MFMA mode Gram: useful flops I (I + 1) K (the distinct entries of the symmetric
+      product, K = N / I columns); the MFMA pipe executes 3x that (hi*hi + hi*lo + lo*hi);
+    - the FP32-MFMA mode product with fused squared row norms, k = min(R, I) rows: HBM bound,
+      4 N + 8 k K bytes (tensor read, FP64 result written)."""
+    import numpy as np
+    import torch
+    from TensorFox import _lib
+    I = dims[0]
+    N = int(np.prod(dims))
+    K = N // I
+    k = min(R, I)
+    G = dev.empty((I, I), torch.float64)
+    rs = np.random.RandomState(3)
+    U = dev.upload(np.linalg.qr(rs.standard_normal((I, k)))[0].ravel(order='F'), torch.float64)
+    Y = dev.empty(k * K, torch.float64)
+    sig2 = dev.empty(k, torch.float64)
+    sp = _lib.TFX_PREC_SPLIT16
+    tg = _event_time(lambda: dev.gram_mode(Tdev, dims, 0, G, prec=sp), reps)
+    tt = _event_time(lambda: dev.ttm(Tdev, dims, 0, U, k, Y, sig2, prec=sp), reps)
+    useful = I * (I + 1) * K / tg / 1e12
+    ttm_bytes = 4 * N + 8 * k * K
+    return {
+        'gram': {'kernel': 'tfx_gram_mode split16 mode 0 (k_gram_mfma<float> + k_gram_reduce)',
+                 'bound': 'mfma', 'avg_launch_s': tg, 'useful_TFLOPs': round(useful, 1),
+                 'achieved': round(3 * useful, 1), 'peak': BF16_MFMA_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                 'frac': round(3 * useful / BF16_MFMA_PEAK_TFLOPS, 4),
+                 'input_read_GBs': round(4 * N / tg / 1e9, 1)},
+        'ttm': {'kernel': 'tfx_ttm split16 mode 0, k=%d (k_ttm_prep + k_ttm_mfma<float> + k_ttm_sig2)' % k,
+                'bound': 'hbm', 'avg_launch_s': tt, 'achieved': round(ttm_bytes / tt / 1e9, 1),
+                'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ttm_bytes / tt / 1e9 / HBM_PEAK_GBS, 4),
+                'fp32_mfma_TFLOPs': round(2 * k * I * K / tt / 1e12, 1),
+                'fp32_mfma_frac': round(2 * k * I * K / tt / 1e12 / FP32_MFMA_PEAK_TFLOPS, 4)},
+    }
+
+
+def compression_only(args):
+    """compression_time alone on a random FP32 tensor of the workload's shape (for counters)."""
+    import torch
+    from TensorFox.device import get_device
+    dims = tuple(args.dims)
+    dev = get_device(0)
+    g = torch.Generator(device=dev.dev)
+    g.manual_seed(0)
+    Tdev = torch.randn(dims, generator=g, device=dev.dev, dtype=torch.float32)
+    print(json.dumps(compression_time(dev, Tdev, dims, args.rank, args.kernel_reps)), flush=True)
 
 
 def cpdloss_time(tfx, dev, reps):
@@ -297,6 +368,9 @@ def main():
     if args.kernel_only:
         kernel_only(args)
         return
+    if args.compression_only:
+        compression_only(args)
+        return
     import numpy as np
     import torch
     import TensorFox as tfx
@@ -366,6 +440,8 @@ def main():
                               'frac': round(gbs / HBM_PEAK_GBS, 4),
                               'traffic': MEASURED_TRAFFIC.get((dims, R)),
                               'kernel': 'tfx_mttkrp split16 (all modes, FP32 tensor, bf16x3 MFMA)', 'avg_launch_s': kt}
+        result['compression_roofline'] = compression_time(dev, Tdev, dims, R, args.kernel_reps)
+        log('compression:', result['compression_roofline'])
 
         if not args.no_fp64:
             T64 = Tdev.to(torch.float64)          # the FP64 form's input, resident before timing

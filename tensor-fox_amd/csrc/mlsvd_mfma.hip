@@ -99,9 +99,10 @@ __device__ inline ColTrack col_init(const GPlan& G, int64_t n) {
   return c;
 }
 
+template <int LAY>
 __device__ inline void col_advance(const GPlan& G, ColTrack& c) {
   c.n += GKS;
-  if (G.lay == LAY_ACONT) {
+  if constexpr (LAY == LAY_ACONT) {
     c.base += (int64_t)GKS * G.I;
     return;
   }
@@ -113,39 +114,46 @@ __device__ inline void col_advance(const GPlan& G, ColTrack& c) {
   }
 }
 
-// The 8 values (a, n..n+7) of the unfolding; zeros outside.
-template <typename S>
-__device__ inline void load8(const S* __restrict__ X, const GPlan& G, const ColTrack& c, int64_t a,
-                             S v[8]) {
+// The 8 values (a, n..n+7) of the unfolding.  Branch-free: an element outside the tensor loads
+// X[0] instead, and its bit in the returned mask is clear (the value is zeroed where it is used,
+// so the loads stay in flight until then).
+template <typename S, int LAY>
+__device__ inline unsigned load8(const S* __restrict__ X, const GPlan& G, const ColTrack& c,
+                                 int64_t a, S v[8]) {
   const bool aok = a < G.I;
-  if (G.lay == LAY_VEC) {
-    if (aok && c.n < G.K) {   // K % 8 == 0 here: the 8 columns are all in range
-      const S* src = X + c.base + a * G.Q;
-      if constexpr (sizeof(S) == 4) {
-        const float4 x0 = *(const float4*)src, x1 = *(const float4*)(src + 4);
-        v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
-        v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; e += 2) {
-          const double2 x = *(const double2*)(src + e);
-          v[e] = x.x;
-          v[e + 1] = x.y;
-        }
-      }
+  if constexpr (LAY == LAY_VEC) {
+    const bool ok = aok && c.n < G.K;   // K % 8 == 0 here: the 8 columns are all in range
+    const S* src = X + (ok ? c.base + a * G.Q : 0);
+    if constexpr (sizeof(S) == 4) {
+      const float4 x0 = *(const float4*)src, x1 = *(const float4*)(src + 4);
+      v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
+      v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
     } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = S(0);
+      for (int e = 0; e < 8; e += 2) {
+        const double2 x = *(const double2*)(src + e);
+        v[e] = x.x;
+        v[e + 1] = x.y;
+      }
     }
-  } else if (G.lay == LAY_ACONT) {
+    return ok ? 0xffu : 0u;
+  } else if constexpr (LAY == LAY_ACONT) {
+    unsigned m = 0;
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-      v[e] = (aok && c.n + e < G.K) ? X[c.base + (int64_t)e * G.I + a] : S(0);
+    for (int e = 0; e < 8; ++e) {
+      const bool ok = aok && c.n + e < G.K;
+      v[e] = X[ok ? c.base + (int64_t)e * G.I + a : 0];
+      m |= (unsigned)ok << e;
+    }
+    return m;
   } else {
+    unsigned m = 0;
     int64_t qq = c.q, o = c.base + a * G.Q;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      v[e] = (aok && c.n + e < G.K) ? X[o] : S(0);
+      const bool ok = aok && c.n + e < G.K;
+      v[e] = X[ok ? o : 0];
+      m |= (unsigned)ok << e;
       ++qq;
       ++o;
       if (qq == G.Q) {
@@ -153,6 +161,7 @@ __device__ inline void load8(const S* __restrict__ X, const GPlan& G, const ColT
         o += (G.I - 1) * G.Q;
       }
     }
+    return m;
   }
 }
 
@@ -174,10 +183,11 @@ __device__ inline f32x4 mfma_bf16x3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, 
 
 // Staging items of thread t: 4 (block, row, column group) triples.  LAY_ACONT puts consecutive
 // threads on consecutive rows (the contiguous direction); the others on consecutive columns.
-__device__ inline void item_of(const GPlan& G, int t, int i, int& blk, int& row, int& cg, int& ng) {
+template <int LAY>
+__device__ inline void item_of(int t, int i, int& blk, int& row, int& cg, int& ng) {
   blk = i >> 1;
   const int u = i & 1;
-  if (G.lay == LAY_ACONT) {
+  if constexpr (LAY == LAY_ACONT) {
     row = t & (GB - 1);
     ng = (t >> 7) + 2 * u;   // column groups of 8: 0..3
     cg = u;                  // two trackers (the two column groups)
@@ -188,7 +198,7 @@ __device__ inline void item_of(const GPlan& G, int t, int i, int& blk, int& row,
   }
 }
 
-template <typename S>
+template <typename S, int LAY>
 __global__ __launch_bounds__(GNT) void k_gram_mfma(const S* __restrict__ X, GPlan G,
                                                    typename GramT<S>::Part* __restrict__ part) {
   typedef typename GramT<S>::Acc Acc;
@@ -213,7 +223,7 @@ __global__ __launch_bounds__(GNT) void k_gram_mfma(const S* __restrict__ X, GPla
   ColTrack ct[2];
   int irow[4], ing[4], icg[4], iblk[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) item_of(G, t, i, iblk[i], irow[i], icg[i], ing[i]);
+  for (int i = 0; i < 4; ++i) item_of<LAY>(t, i, iblk[i], irow[i], icg[i], ing[i]);
   ct[0] = col_init(G, s0 * GKS + 8 * ing[0]);
   ct[1] = col_init(G, s0 * GKS + 8 * ing[1]);
   int64_t arow[4];
@@ -221,10 +231,11 @@ __global__ __launch_bounds__(GNT) void k_gram_mfma(const S* __restrict__ X, GPla
   for (int i = 0; i < 4; ++i) arow[i] = (int64_t)(iblk[i] ? tb : ta) * GB + irow[i];
 
   S v[4][8];
+  unsigned vm[4] = {0u, 0u, 0u, 0u};
   auto load_all = [&]() {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      if (i < nitems) load8<S>(X, G, ct[icg[i]], arow[i], v[i]);
+      if (i < nitems) vm[i] = load8<S, LAY>(X, G, ct[icg[i]], arow[i], v[i]);
   };
   load_all();
 
@@ -247,7 +258,7 @@ __global__ __launch_bounds__(GNT) void k_gram_mfma(const S* __restrict__ X, GPla
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
             __bf16 a, b;
-            split(v[i][e], a, b);
+            split(((vm[i] >> e) & 1u) ? v[i][e] : 0.f, a, b);
             h[e] = a;
             l[e] = b;
           }
@@ -263,13 +274,13 @@ __global__ __launch_bounds__(GNT) void k_gram_mfma(const S* __restrict__ X, GPla
         if (i < nitems) {
           double* rowp = L + ((size_t)iblk[i] * GB + irow[i]) * DP + 8 * ing[i];
 #pragma unroll
-          for (int e = 0; e < 8; ++e) rowp[e] = v[i][e];
+          for (int e = 0; e < 8; ++e) rowp[e] = ((vm[i] >> e) & 1u) ? v[i][e] : 0.0;
         }
     }
     __syncthreads();
     if (st + 1 < s1) {   // next stage's loads overlap this stage's MFMAs
-      col_advance(G, ct[0]);
-      if (G.lay == LAY_ACONT) col_advance(G, ct[1]);
+      col_advance<LAY>(G, ct[0]);
+      if (LAY == LAY_ACONT) col_advance<LAY>(G, ct[1]);
       load_all();
     }
     if (active) {
@@ -378,9 +389,17 @@ int gram_mfma(tfx_ctx* ctx, const S* X, const Shape& s, int mode, double* out, h
   if (rc) return rc;
   PT* part = (PT*)wsp;
   const size_t lds = GramT<S>::lds;
-  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_gram_mfma<S>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_gram_mfma<S>, dim3((unsigned)G.nlaunch), dim3(GNT), lds, st, X, G, part);
+  const void* kern = G.lay == LAY_VEC     ? (const void*)k_gram_mfma<S, LAY_VEC>
+                     : G.lay == LAY_ACONT ? (const void*)k_gram_mfma<S, LAY_ACONT>
+                                          : (const void*)k_gram_mfma<S, LAY_GEN>;
+  TFX_HIP(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const dim3 grid((unsigned)G.nlaunch), block(GNT);
+  if (G.lay == LAY_VEC)
+    hipLaunchKernelGGL((k_gram_mfma<S, LAY_VEC>), grid, block, lds, st, X, G, part);
+  else if (G.lay == LAY_ACONT)
+    hipLaunchKernelGGL((k_gram_mfma<S, LAY_ACONT>), grid, block, lds, st, X, G, part);
+  else
+    hipLaunchKernelGGL((k_gram_mfma<S, LAY_GEN>), grid, block, lds, st, X, G, part);
   TFX_CHECK_LAUNCH(ctx);
   const int64_t tot = G.I * G.I;
   hipLaunchKernelGGL(k_gram_reduce<PT>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
@@ -400,11 +419,13 @@ constexpr int TLAY_GEN = 2;
 
 struct TPlan {
   int64_t I, Q, Nc;   // Nc = P Q columns
-  int k, ngrp, nchunk, nsb, lay;
+  int k, ngrp, nchunk, lay;
+  int nsb, sbw;       // blocks of sbw = 16 x (s tiles per wave) rows of Y
 };
 
 // U fragments in MFMA operand order: UF[((sb 4 + st) ngrp + grp) 4 + m][lane] = U[s][a] with
-// s = 64 sb + 16 st + lane % 16 and a = 16 grp + 4 (lane / 16) + m (zero outside).
+// s = sbw sb + 16 st + lane % 16 and a = 16 grp + 4 (lane / 16) + m (zero outside the block,
+// the rank or the mode).
 template <typename S>
 __global__ __launch_bounds__(256) void k_ttm_prep(const double* __restrict__ U, TPlan T,
                                                   S* __restrict__ UF) {
@@ -416,9 +437,9 @@ __global__ __launch_bounds__(256) void k_ttm_prep(const double* __restrict__ U, 
   const int grp = (int)(rest % T.ngrp);
   rest /= T.ngrp;
   const int stile = (int)(rest & 3), sb = (int)(rest >> 2);
-  const int64_t s = 64 * sb + 16 * stile + (lane & 15);
+  const int64_t s = (int64_t)T.sbw * sb + 16 * stile + (lane & 15);
   const int64_t a = 16 * (int64_t)grp + 4 * (lane >> 4) + m;
-  UF[e] = (s < T.k && a < T.I) ? (S)U[s * T.I + a] : S(0);
+  UF[e] = (16 * stile < T.sbw && s < T.k && a < T.I) ? (S)U[s * T.I + a] : S(0);
 }
 
 template <typename S>
@@ -451,7 +472,43 @@ __device__ inline void load4(const S* p, S o[4]) {
   }
 }
 
-template <typename S, int ST>
+// The lane's values of one group of 16 a (K-step m: a = a0 + m) for its 4 columns, [t][m], and
+// the mask of the ones inside the tensor (bit 4 t + m); outside ones load X[0].
+template <typename S, int LAY>
+__device__ inline unsigned load_group(const S* __restrict__ X, const TPlan& T, const int64_t xb[4],
+                                      const bool cv[4], int64_t a0, S v[4][4]) {
+  unsigned msk = 0;
+  if constexpr (LAY == TLAY_QVEC) {   // the 4 columns are contiguous: one vector per a
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const bool ok = cv[0] && a0 + m < T.I;
+      S o[4];
+      load4<S>(X + (ok ? xb[0] + (a0 + m) * T.Q : 0), o);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v[t][m] = o[t];
+      msk |= ok ? (0x1111u << m) : 0u;
+    }
+  } else if constexpr (LAY == TLAY_ACONT) {   // I % 4 == 0: a0 < I means all 4 a are inside
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bool ok = cv[t] && a0 < T.I;
+      load4<S>(X + (ok ? xb[t] + a0 : 0), v[t]);
+      msk |= ok ? (0xfu << (4 * t)) : 0u;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const bool ok = cv[t] && a0 + m < T.I;
+        v[t][m] = X[ok ? xb[t] + (a0 + m) * T.Q : 0];
+        msk |= (unsigned)ok << (4 * t + m);
+      }
+  }
+  return msk;
+}
+
+template <typename S, int ST, int LAY>
 __global__ __launch_bounds__(TNT) void k_ttm_mfma(const S* __restrict__ X, TPlan T,
                                                   const S* __restrict__ UF, double* __restrict__ Y,
                                                   double* __restrict__ part) {
@@ -479,45 +536,27 @@ __global__ __launch_bounds__(TNT) void k_ttm_mfma(const S* __restrict__ X, TPlan
       for (int j = 0; j < 4; ++j) acc[i][t][j] = 0;
 
   const S* uf = UF + ((size_t)(sb * 4) * T.ngrp * 4) * 64 + lane;
-  for (int grp = 0; grp < T.ngrp; ++grp) {
-    const int64_t a0 = 16 * (int64_t)grp + 4 * lg;   // this lane's a for K-step m: a0 + m
-    S v[4][4];                                        // [t][m]
-    if (T.lay == TLAY_QVEC) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        S o[4] = {S(0), S(0), S(0), S(0)};
-        if (cv[0] && a0 + m < T.I) load4<S>(X + xb[0] + (a0 + m) * T.Q, o);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) v[t][m] = o[t];
-      }
-    } else if (T.lay == TLAY_ACONT) {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        if (cv[t] && a0 + 3 < T.I) {
-          load4<S>(X + xb[t] + a0, v[t]);
-        } else {
-#pragma unroll
-          for (int m = 0; m < 4; ++m) v[t][m] = (cv[t] && a0 + m < T.I) ? X[xb[t] + a0 + m] : S(0);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-          v[t][m] = (cv[t] && a0 + m < T.I) ? X[xb[t] + (a0 + m) * T.Q] : S(0);
-    }
-    S u[ST][4];
+  auto load_u = [&](int grp, S u[ST][4]) {
 #pragma unroll
     for (int i = 0; i < ST; ++i)
 #pragma unroll
       for (int m = 0; m < 4; ++m) u[i][m] = uf[(((size_t)i * T.ngrp + grp) * 4 + m) * 64];
+  };
+  // One group at a time: loads (branch-free) then MFMAs; the other waves on the SIMD (3 at 4
+  // tiles of FP32) cover the load latency.
+  for (int grp = 0; grp < T.ngrp; ++grp) {
+    S v[4][4];
+    S u[ST][4];
+    const unsigned msk = load_group<S, LAY>(X, T, xb, cv, 16 * (int64_t)grp + 4 * lg, v);
+    load_u(grp, u);
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int i = 0; i < ST; ++i)
+      for (int t = 0; t < 4; ++t) {
+        const S b = ((msk >> (4 * t + m)) & 1u) ? v[t][m] : S(0);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[i][t] = TtmT<S>::mfma(u[i][m], v[t][m], acc[i][t]);
+        for (int i = 0; i < ST; ++i) acc[i][t] = TtmT<S>::mfma(u[i][m], b, acc[i][t]);
+      }
   }
 
   // Y and the columns' share of sig2
@@ -527,7 +566,7 @@ __global__ __launch_bounds__(TNT) void k_ttm_mfma(const S* __restrict__ X, TPlan
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int sl = 16 * i + TtmT<S>::row(lg, j);
-      const int64_t s = 64 * (int64_t)sb + sl;
+      const int64_t s = (int64_t)T.sbw * sb + sl;
       double sq = 0.0;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -535,7 +574,7 @@ __global__ __launch_bounds__(TNT) void k_ttm_mfma(const S* __restrict__ X, TPlan
         sq += y * y;
       }
       if (s < T.k) {
-        if (T.lay == TLAY_QVEC) {
+        if constexpr (LAY == TLAY_QVEC) {
           if (cv[0]) {
             double* dst = Y + yb[0] + s * T.Q;
             *(double2*)dst = make_double2((double)acc[i][0][j], (double)acc[i][1][j]);
@@ -563,7 +602,7 @@ __global__ __launch_bounds__(256) void k_ttm_sig2(const double* __restrict__ par
                                                   int st_tiles, double* __restrict__ sig2) {
   __shared__ double red[4];
   const int s = blockIdx.x;
-  const int sb = s / 64, sl = s % 64;
+  const int sb = s / T.sbw, sl = s % T.sbw;
   double acc = 0.0;
   if (sl < 16 * st_tiles)
     for (int c = threadIdx.x; c < T.nchunk; c += 256) acc += part[((size_t)sb * T.nchunk + c) * 64 + sl];
@@ -574,7 +613,13 @@ __global__ __launch_bounds__(256) void k_ttm_sig2(const double* __restrict__ par
 template <typename S, int ST>
 void launch_ttm(const S* X, const TPlan& T, const S* UF, double* Y, double* part, hipStream_t st) {
   const dim3 grid((unsigned)((T.nchunk + 3) / 4), (unsigned)T.nsb);
-  hipLaunchKernelGGL((k_ttm_mfma<S, ST>), grid, dim3(TNT), 0, st, X, T, UF, Y, part);
+  if (T.lay == TLAY_QVEC) {
+    hipLaunchKernelGGL((k_ttm_mfma<S, ST, TLAY_QVEC>), grid, dim3(TNT), 0, st, X, T, UF, Y, part);
+  } else if (T.lay == TLAY_ACONT) {
+    hipLaunchKernelGGL((k_ttm_mfma<S, ST, TLAY_ACONT>), grid, dim3(TNT), 0, st, X, T, UF, Y, part);
+  } else {
+    hipLaunchKernelGGL((k_ttm_mfma<S, ST, TLAY_GEN>), grid, dim3(TNT), 0, st, X, T, UF, Y, part);
+  }
 }
 
 template <typename S>
@@ -591,8 +636,6 @@ int ttm_mfma(tfx_ctx* ctx, const S* X, const Shape& s, int mode, const double* U
   if (nchunk > 0x7ffffff0 / 4 * 4 || v.I > (1 << 24))
     return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "MFMA TTM: tensor too large");
   T.nchunk = (int)nchunk;
-  const int st_tiles = k > 64 ? 4 : (k + 15) / 16;
-  T.nsb = (k + 16 * st_tiles - 1) / (16 * st_tiles);
   const bool xal = ((uintptr_t)X % 16) == 0, yal = ((uintptr_t)Y % 16) == 0;
   if (v.Q % 4 == 0 && xal && yal)
     T.lay = TLAY_QVEC;
@@ -600,6 +643,10 @@ int ttm_mfma(tfx_ctx* ctx, const S* X, const Shape& s, int mode, const double* U
     T.lay = TLAY_ACONT;
   else
     T.lay = TLAY_GEN;
+  const int st_max = 4;   // s tiles per wave
+  const int st_tiles = k > 16 * st_max ? st_max : (k + 15) / 16;
+  T.sbw = 16 * st_tiles;
+  T.nsb = (k + T.sbw - 1) / T.sbw;
   const size_t nuf = (size_t)T.nsb * 4 * T.ngrp * 4 * 64;
   const size_t uf_bytes = align256(sizeof(S) * nuf);
   const size_t part_bytes = sig2 ? sizeof(double) * (size_t)T.nsb * T.nchunk * 64 : 0;
