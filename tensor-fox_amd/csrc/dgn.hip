@@ -374,25 +374,39 @@ __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restric
                                                       S* __restrict__ xn, S* __restrict__ F,
                                                       double* __restrict__ scal) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double* cn = (double*)smem;   // L x R column norms
+  double* cn = (double*)smem;   // L x R column norms, then L x R equalization scales
   __shared__ double red[16];
   __shared__ double fnorm[TFX_MAXL];
   const int64_t n = s.n;
   const int L = s.L, R = s.R;
   const int t = threadIdx.x, nt = blockDim.x;
+  const int lane = t & 63, wave = t >> 6, nwave = nt >> 6;
   double gain = 0.0, dy2 = 0.0, x2 = 0.0, gmax = 0.0;
-  for (int64_t e = t; e < n; e += nt) {
-    const double xo = (double)x[e];
-    const S dyS = (S)alpha * y[e];
-    const S v = x[e] + dyS;
-    xn[e] = v;
-    F[e] = v;
-    const double dy = (double)(v - x[e]);   // x_new - x_old as the reference forms it
-    gain += dy * ((double)g[e] + damp * dy);
-    dy2 += dy * dy;
-    x2 += xo * xo;
-    const double ag = fabs((double)g[e]);
-    gmax = ag > gmax ? ag : gmax;
+  // one wave per factor column (l, r): the update, the step statistics and the column's norm
+  for (int c = wave; c < L * R; c += nwave) {
+    const int l = c / R, r = c - (c / R) * R;
+    const int64_t I = s.d[l];
+    const int64_t base = s.off[l] + (int64_t)r * I;
+    double ss = 0.0;
+    for (int64_t i = lane; i < I; i += 64) {
+      const int64_t e = base + i;
+      const double xo = (double)x[e];
+      const S dyS = (S)alpha * y[e];
+      const S v = x[e] + dyS;
+      xn[e] = v;
+      F[e] = v;
+      const double dy = (double)(v - x[e]);   // x_new - x_old as the reference forms it
+      gain += dy * ((double)g[e] + damp * dy);
+      dy2 += dy * dy;
+      x2 += xo * xo;
+      const double ag = fabs((double)g[e]);
+      gmax = ag > gmax ? ag : gmax;
+      ss += (double)v * (double)v;
+    }
+    if (eq) {
+      ss = wave_sum(ss);
+      if (lane == 0) cn[c] = sqrt(ss);
+    }
   }
   gain = block_sum(gain, red);
   dy2 = block_sum(dy2, red);
@@ -406,22 +420,23 @@ __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restric
   }
   __syncthreads();
   if (eq) {
-    for (int e = t; e < L * R; e += nt) {
-      const int l = e / R, r = e % R;
-      const int64_t I = s.d[l];
-      const S* col = F + s.off[l] + (int64_t)r * I;
-      double a = 0.0;
-      for (int64_t i = 0; i < I; ++i) a += (double)col[i] * (double)col[i];
-      cn[e] = sqrt(a);
-    }
-    __syncthreads();
-    for (int64_t e = t; e < n; e += nt) {
-      int64_t w;
-      const int l = mode_of(s, e, &w);
-      const int r = (int)(w / s.d[l]);
+    // column r of mode l scaled by (prod_k ||f_k^r||)^(1/L) / ||f_l^r|| (Conversion.py:361-387);
+    // a column whose norm product is zero is left as it is (scale 0 marks it)
+    double* sc = cn + L * R;
+    for (int c = t; c < L * R; c += nt) {
+      const int l = c / R, r = c - (c / R) * R;
       double prod = 1.0;
       for (int k = 0; k < L; ++k) prod *= cn[k * R + r];
-      if (prod != 0.0) F[e] = (S)(pow(prod, 1.0 / L) / cn[l * R + r] * (double)F[e]);
+      sc[c] = prod != 0.0 ? pow(prod, 1.0 / L) / cn[l * R + r] : 0.0;
+    }
+    __syncthreads();
+    for (int c = wave; c < L * R; c += nwave) {
+      const double f = sc[c];
+      if (f == 0.0) continue;
+      const int l = c / R, r = c - (c / R) * R;
+      const int64_t I = s.d[l];
+      S* col = F + s.off[l] + (int64_t)r * I;
+      for (int64_t i = lane; i < I; i += 64) col[i] = (S)(f * (double)col[i]);
     }
     __syncthreads();
   }
@@ -614,8 +629,14 @@ extern "C" int tfx_dgn_update(tfx_ctx* ctx, int dtype, int L, const int64_t* dim
   if (symm)
     for (int l = 1; l < L; ++l)
       if (dims[l] != dims[0]) return tfx_set_error(ctx, TFX_E_ARG, "symm needs equal dims");
-  const size_t lds = sizeof(double) * (size_t)L * R;
-  if (lds > 60 * 1024) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "L*R too large");
+  const size_t lds = sizeof(double) * 2 * (size_t)L * R;
+  if (lds > 120 * 1024) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "L*R too large");
+  if (lds > 64 * 1024) {
+    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_dgn_update<float>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_dgn_update<double>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   if (dtype == TFX_F32)
     hipLaunchKernelGGL(k_dgn_update<float>, dim3(1), dim3(1024), lds, stream, s,
                        (const float*)x_flat, (const float*)y_flat, alpha, (const float*)grad_flat,

@@ -385,6 +385,46 @@ __global__ void k_gemm_small(int transA, int m, int n, int kk, const double* __r
   C[e] = (S)acc;
 }
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// C (m x n col-major) = op(A) B in FP64 on the matrix cores: one workgroup per 16 x 16 tile of
+// C; its 4 waves take the K-steps of 4 in turn (v_mfma_f64_16x16x4f64), and the 4 partial tiles
+// are summed in LDS in a fixed order.  Loads are branch-free (indices clamped, values masked).
+__global__ __launch_bounds__(256) void k_gemm_mfma64(int transA, int m, int n, int kk,
+                                                     const double* __restrict__ A,
+                                                     const double* __restrict__ B,
+                                                     double* __restrict__ C) {
+  __shared__ double part[4][256];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, l16 = lane & 15, lg = lane >> 4;
+  const int i = blockIdx.x * 16 + l16, j = blockIdx.y * 16 + l16;
+  const bool iok = i < m, jok = j < n;
+  const int ic = iok ? i : 0, jc = jok ? j : 0;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 4 * w; k0 < kk; k0 += 16) {
+    const int k = k0 + lg;
+    const bool kok = k < kk;
+    const int kc = kok ? k : 0;
+    const double av = transA ? A[(int64_t)ic * kk + kc] : A[(int64_t)kc * m + ic];
+    const double bv = B[(int64_t)jc * kk + kc];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64((kok && iok) ? av : 0.0, (kok && jok) ? bv : 0.0,
+                                               acc, 0, 0, 0);
+  }
+  // register r of lane l holds C row 4 r + l / 16, column l % 16 of the tile
+#pragma unroll
+  for (int r = 0; r < 4; ++r) part[w][(4 * r + lg) * 16 + l16] = acc[r];
+  __syncthreads();
+  const int e = threadIdx.x, row = e >> 4, col = e & 15;
+  const int ci = blockIdx.x * 16 + row, cj = blockIdx.y * 16 + col;
+  if (ci < m && cj < n)
+    C[(int64_t)cj * m + ci] = ((part[0][e] + part[1][e]) + part[2][e]) + part[3][e];
+}
+
+void launch_gemm64(int transA, int m, int n, int kk, const double* A, const double* B, double* C,
+                   hipStream_t st) {
+  const dim3 grid((unsigned)((m + 15) / 16), (unsigned)((n + 15) / 16));
+  hipLaunchKernelGGL(k_gemm_mfma64, grid, dim3(256), 0, st, transA, m, n, kk, A, B, C);
+}
+
 template <typename S>
 int gram_t(tfx_ctx* ctx, const S* X, const Shape& s, int mode, double* G, hipStream_t st) {
   const PIQ v = piq_of(s, mode);
@@ -470,9 +510,7 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   double* V = H + (size_t)m * m;
   double* Vk = V + (size_t)m * m;
   auto gemm = [&](int transA, int mm_, int nn_, int kk_, const double* A_, const double* B_, double* C_) {
-    const int64_t tot = (int64_t)mm_ * nn_;
-    hipLaunchKernelGGL(k_gemm_small<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
-                       transA, mm_, nn_, kk_, A_, B_, C_);
+    launch_gemm64(transA, mm_, nn_, kk_, A_, B_, C_, stream);
   };
   // orthonormalization: block CGS2 with its panel in LDS when it fits, else column CGS2
   const size_t orth_lds = sizeof(double) * ((size_t)OPB * n + (size_t)p * OPB);
@@ -559,8 +597,7 @@ extern "C" int tfx_gemm_small(tfx_ctx* ctx, int dtype, int transA, int m, int n,
     hipLaunchKernelGGL(k_gemm_small<float>, dim3(nb), dim3(256), 0, stream, transA, m, n, kk, A,
                        (const float*)B, (float*)C);
   else if (dtype == TFX_F64)
-    hipLaunchKernelGGL(k_gemm_small<double>, dim3(nb), dim3(256), 0, stream, transA, m, n, kk, A,
-                       (const double*)B, (double*)C);
+    launch_gemm64(transA, m, n, kk, A, (const double*)B, (double*)C, stream);
   else
     return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
   TFX_CHECK_LAUNCH(ctx);
