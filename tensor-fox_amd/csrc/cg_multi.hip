@@ -49,6 +49,7 @@ struct CgBufs {
   S* z;
   S* yc;
   double* A;        // A_part
+  double* Mv;       // preconditioner M[l][c] = 1 / sqrt(gamma^2 + damp^2 gamma^2), L x R
   double* part;     // per-workgroup partial sums
   double* sc;       // scalar block
   unsigned* cnt;    // counters
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
     int64_t w;
     const int l = mode_of_e(s, e, &w);
     const int rk = (int)(w / s.d[l]);
-    const double M = inv_m(B.gamma[l * s.R + rk], P.damp);
+    const double M = B.Mv[l * s.R + rk];
     const S rr = (S)(M * (double)B.grad[e]);
     B.r[e] = rr;
     B.p[0][e] = rr;
@@ -140,6 +141,8 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
     B.sc[SC_DONE] = 0.0;
     B.sc[SC_ITN] = 0.0;
   }
+  if (raw && blockIdx.x == 0)
+    for (int e = threadIdx.x; e < s.L * s.R; e += NTH) B.Mv[e] = inv_m(B.gamma[e], P.damp);
   if (!raw && B.sc[SC_DONE] != 0.0) return;
   int m = 0;
   while (m + 1 < s.L && (int)blockIdx.x >= P.k1first[m + 1]) ++m;
@@ -158,38 +161,51 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   const S* pold = B.p[(j + 1) & 1];
   S* pnew = B.p[j & 1];
   const bool writer = !raw && j >= 1 && rbk == 0;
+  const S* src1 = raw ? B.grad : (j == 0 ? B.p[0] : B.r);
   const int t = threadIdx.x;
   const int tc = t >> 4, tr = t & 15;
   double acc = 0.0;
   for (int64_t i0 = ib; i0 < ie; i0 += KI) {
-    // stage q[i0..i0+31][c0..c0+15] and F[i0..i0+31][r0..r0+15] (coalesced along i)
-    for (int u = t; u < KI * CT; u += NTH) {
+    // stage q[i0..i0+31][c0..c0+15] and F[i0..i0+31][r0..r0+15] (coalesced along i); loads
+    // first (clamped indices), values after
+    constexpr int NK1 = KI * CT / NTH;
+    S v1[NK1], vo[NK1], vf[NK1];
+#pragma unroll
+    for (int k = 0; k < NK1; ++k) {
+      const int u = t + NTH * k;
       const int cc = u / KI, ii = u % KI;
       const int64_t i = i0 + ii;
-      double qv = 0.0, fv = 0.0;
-      if (i < ie) {
-        const int c = c0 + cc;
-        if (c < R) {
-          const int64_t e = off + (int64_t)c * I + i;
-          if (raw) {
-            qv = (double)B.grad[e];
+      const int c = c0 + cc, rr = r0 + cc;
+      const int64_t e = (i < ie && c < R) ? off + (int64_t)c * I + i : 0;
+      v1[k] = src1[e];
+      vo[k] = pold[e];
+      vf[k] = Fm[(i < ie && rr < R) ? (int64_t)rr * I + i : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < NK1; ++k) {
+      const int u = t + NTH * k;
+      const int cc = u / KI, ii = u % KI;
+      const int64_t i = i0 + ii;
+      const int c = c0 + cc, rr = r0 + cc;
+      const bool okq = i < ie && c < R;
+      double qv = 0.0;
+      if (okq) {
+        if (raw) {
+          qv = (double)v1[k];
+        } else {
+          S pv;
+          if (j == 0) {
+            pv = v1[k];
           } else {
-            S pv;
-            if (j == 0) {
-              pv = B.p[0][e];
-            } else {
-              const S rv = B.r[e];
-              pv = (S)((double)rv + beta * (double)(j == 1 ? rv : pold[e]));
-              if (writer) pnew[e] = pv;
-            }
-            qv = inv_m(B.gamma[m * R + c], P.damp) * (double)pv;
+            const S rv = v1[k];
+            pv = (S)((double)rv + beta * (double)(j == 1 ? rv : vo[k]));
+            if (writer) pnew[off + (int64_t)c * I + i] = pv;
           }
+          qv = B.Mv[m * R + c] * (double)pv;
         }
-        const int rr = r0 + cc;
-        if (rr < R) fv = (double)Fm[(int64_t)rr * I + i];
       }
       qs[ii][cc] = qv;
-      fs[ii][cc] = fv;
+      fs[ii][cc] = (i < ie && rr < R) ? (double)vf[k] : 0.0;
     }
     __syncthreads();
 #pragma unroll 8
@@ -226,37 +242,74 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   const int t = threadIdx.x;
   const int tr = t & 15, ig = t >> 4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const S* qsrc = raw ? B.grad : pj;
   for (int cc0 = 0; cc0 < R; cc0 += ZC) {
-    // tmp_l[c][r] = sum_{m != l} P2[l][m][c][r] sum_split A_m[split][c][r], and P1_l
-    for (int u = t; u < ZC * CT; u += NTH) {
-      const int cc = u / CT, rq = u % CT;
-      const int c = cc0 + cc, rr = r0 + rq;
-      double tv = 0.0, pv = 0.0;
-      if (c < R && rr < R) {
-        for (int m = 0; m < L; ++m) {
-          if (m == l) continue;
-          double a = 0.0;
-          for (int sp = 0; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + ((int64_t)sp * R + c) * R + rr];
-          tv += (double)B.P2[((int64_t)l * L + m) * RR + (int64_t)c * R + rr] * a;
-        }
-        pv = (double)B.P1[(int64_t)l * RR + (int64_t)c * R + rr];
-      }
-      Ts[cc][rq] = tv;
-      P1s[cc][rq] = pv;
-    }
+    // Loads first, values after: every loop below has a compile-time trip count and
+    // clamped indices, so all of a thread's loads are in flight together.
     // F_l and q_l rows i0..i0+63, columns cc0..cc0+63 (coalesced along i)
-    for (int u = t; u < ZI * ZC; u += NTH) {
+    constexpr int NFQ = ZI * ZC / NTH;
+    S fv[NFQ], qv[NFQ];
+#pragma unroll
+    for (int k = 0; k < NFQ; ++k) {
+      const int u = t + NTH * k;
       const int cc = u / ZI, ii = u % ZI;
       const int64_t i = i0 + ii;
       const int c = cc0 + cc;
-      double fv = 0.0, qv = 0.0;
-      if (i < I && c < R) {
-        const int64_t e = off + (int64_t)c * I + i;
-        fv = (double)B.F[e];
-        qv = raw ? (double)B.grad[e] : inv_m(B.gamma[l * R + c], P.damp) * (double)pj[e];
+      const int64_t e = (i < I && c < R) ? off + (int64_t)c * I + i : 0;
+      fv[k] = B.F[e];
+      qv[k] = qsrc[e];
+    }
+    // tmp_l[c][r] = sum_{m != l} P2[l][m][c][r] sum_split A_m[split][c][r], and P1_l
+    constexpr int NT = ZC * CT / NTH;
+    double tv[NT], pv[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const int u = t + NTH * k;
+      const int cc = u / CT, rq = u % CT;
+      const int c = cc0 + cc, rr = r0 + rq;
+      const bool ok = c < R && rr < R;
+      const int64_t cr = ok ? (int64_t)c * R + rr : 0;
+      // modes 0..3: split 0 of A_m and P2[l][m] loaded together (mode 0 stands in for the
+      // inactive ones); further splits and modes >= 4 follow in the same summation order
+      double a0[4], p2[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int mc = (m < L && m != l) ? m : 0;
+        a0[m] = B.A[P.aoff[mc] + cr];
+        p2[m] = (double)B.P2[((int64_t)l * L + mc) * RR + cr];
       }
-      Fs[ii][cc] = fv;
-      Qs[ii][cc] = qv;
+      double acc2 = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        if (m >= L || m == l) continue;   // uniform in the workgroup
+        double a = a0[m];
+        for (int sp = 1; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + (int64_t)sp * RR + cr];
+        acc2 += p2[m] * a;
+      }
+      for (int m = 4; m < L; ++m) {
+        if (m == l) continue;
+        double a = 0.0;
+        for (int sp = 0; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + (int64_t)sp * RR + cr];
+        acc2 += (double)B.P2[((int64_t)l * L + m) * RR + cr] * a;
+      }
+      tv[k] = ok ? acc2 : 0.0;
+      pv[k] = ok ? (double)B.P1[(int64_t)l * RR + cr] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const int u = t + NTH * k;
+      Ts[u / CT][u % CT] = tv[k];
+      P1s[u / CT][u % CT] = pv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NFQ; ++k) {
+      const int u = t + NTH * k;
+      const int cc = u / ZI, ii = u % ZI;
+      const int64_t i = i0 + ii;
+      const int c = cc0 + cc;
+      const bool ok = i < I && c < R;
+      Fs[ii][cc] = ok ? (double)fv[k] : 0.0;
+      Qs[ii][cc] = ok ? (raw ? (double)qv[k] : B.Mv[l * R + c] * (double)qv[k]) : 0.0;
     }
     __syncthreads();
     const int cn = R - cc0 < ZC ? R - cc0 : ZC;
@@ -278,7 +331,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
         out[e] = (S)acc[a];
       } else {
         const double g = (double)B.gamma[l * R + rr];
-        const double M = inv_m(B.gamma[l * R + rr], P.damp);
+        const double M = B.Mv[l * R + rr];
         const double q = M * (double)pj[e];
         const S zv = (S)(M * (acc[a] + P.damp * g * q));
         out[e] = zv;
@@ -334,7 +387,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_end(CgPlan P, CgBufs<S> B, int maxi
     int64_t w;
     const int l = mode_of_e(s, e, &w);
     const int rk = (int)(w / s.d[l]);
-    y[e] = (S)(inv_m(B.gamma[l * s.R + rk], P.damp) * (double)B.yc[e]);
+    y[e] = (S)(B.Mv[l * s.R + rk] * (double)B.yc[e]);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const int itn = B.sc[SC_DONE] != 0.0 ? (int)B.sc[SC_ITN] : maxiter - 1;
@@ -344,7 +397,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_end(CgPlan P, CgBufs<S> B, int maxi
 }
 
 inline unsigned nblocks_vec(int64_t n) {
-  int64_t b = (n + NTH * 8 - 1) / (NTH * 8);
+  int64_t b = (n + NTH - 1) / NTH;
   if (b < 1) b = 1;
   if (b > 1024) b = 1024;
   return (unsigned)b;
@@ -379,6 +432,7 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   const int npart = std::max<int>(k2, (int)nbv);
   const size_t vec = align256(sizeof(S) * (size_t)s.n);
   const size_t bytes = 5 * vec + align256(sizeof(double) * (size_t)aoff) +
+                       align256(sizeof(double) * (size_t)s.L * s.R) +
                        align256(sizeof(double) * (size_t)npart) + align256(sizeof(double) * SC_N) +
                        align256(sizeof(unsigned) * CN_N);
   void* wsp;
@@ -397,11 +451,12 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   B.z = (S*)w; w += vec;
   B.yc = (S*)w; w += vec;
   B.A = (double*)w; w += align256(sizeof(double) * (size_t)aoff);
+  B.Mv = (double*)w; w += align256(sizeof(double) * (size_t)s.L * s.R);
   B.part = (double*)w; w += align256(sizeof(double) * (size_t)npart);
   B.sc = (double*)w; w += align256(sizeof(double) * SC_N);
   B.cnt = (unsigned*)w;
 
-  // J^T J grad for the dogleg (also clears counters and the stop flag)
+  // J^T J grad for the dogleg (also clears counters and the stop flag, and fills Mv)
   hipLaunchKernelGGL(k_cgm_A<S>, dim3(k1), dim3(NTH), 0, st, P, B, 1, 0);
   TFX_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_cgm_Z<S>, dim3(k2), dim3(NTH), 0, st, P, B, 1, 0, jtjg);
