@@ -7,10 +7,13 @@
 //       and the dot p_j . z, whose last arriving workgroup forms alpha = rn / (p . z)
 //   K3  y += alpha p_j, r -= alpha z, and r . r, whose last workgroup forms beta, rn and the
 //       stop flag (rn <= tol)
-// Scalars pass between kernels through a small device block; a dot product is finished by the
-// last workgroup to arrive (atomic counter after a release fence; no workgroup ever waits for
-// another).  Kernels launched after the stop flag is set return at once, so the host launches
-// maxiter iterations without reading anything back.  Dots and norms are FP64; vectors are in
+// A dot product is left as one partial per workgroup; every workgroup of the next kernel that
+// needs it sums the partials itself in a fixed order (the kernel boundary has published them),
+// so no atomics, fences or waits are involved and all workgroups get the same value.  Partials
+// are kept per iteration parity, so a kernel reads the previous iteration's while writing its
+// own.  K1 detects convergence (rn <= tol after the previous update) and sets a stop flag;
+// kernels launched after it return at once, so the host launches maxiter iterations without
+// reading anything back.  Dots and norms are FP64; vectors are in
 // the working type S.  J^T J grad (for the dogleg) is the same K1/K2 pair without M and damping.
 #include "common.h"
 
@@ -24,8 +27,7 @@ constexpr int ZC = 64;        // c per LDS chunk in K2
 constexpr int NTH = 256;
 
 // device scalar block (double) and counters (unsigned)
-enum { SC_RN = 0, SC_ALPHA, SC_BETA, SC_DONE, SC_ITN, SC_N };
-enum { CN_K0 = 0, CN_K2, CN_K3, CN_N };
+enum { SC_DONE = 0, SC_ITN, SC_N };
 
 struct CgPlan {
   Shape s;
@@ -34,6 +36,7 @@ struct CgPlan {
   int k1first[TFX_MAXL + 1];     // first K1 block of each mode
   int k2first[TFX_MAXL + 1];     // first K2 block of each mode
   int64_t aoff[TFX_MAXL];        // offset of mode m's partials in A_part: [split][c][r]
+  int nbv, nk2;                  // workgroups of the vector kernels (K0, K3) and of K2
   double damp, tol;
 };
 
@@ -50,9 +53,9 @@ struct CgBufs {
   S* yc;
   double* A;        // A_part
   double* Mv;       // preconditioner M[l][c] = 1 / sqrt(gamma^2 + damp^2 gamma^2), L x R
-  double* part;     // per-workgroup partial sums
-  double* sc;       // scalar block
-  unsigned* cnt;    // counters
+  double* pz[2];    // K2 partials of p . z, by iteration parity
+  double* rq[2];    // K3 partials of r . r by iteration parity (K0's in rr[1])
+  double* sc;       // stop flag and the iteration it was set at
 };
 
 __device__ inline int mode_of_e(const Shape& s, int64_t e, int64_t* within) {
@@ -81,31 +84,31 @@ __device__ inline double wg_sum(double v, double* red) {
 
 // Publish this workgroup's partial and report whether it is the last to arrive; the last one
 // sees every partial (release fence before the counter, acquire fence after it).
-__device__ inline bool arrive_last(double v, double* part, unsigned* counter, bool* flag) {
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&part[blockIdx.x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    *flag = prev == gridDim.x - 1;
+// Sum of n per-workgroup partials written by an earlier kernel: wave 0 sums them in a fixed
+// lane order and shares the result through `slot` (one LDS double per call site).
+__device__ inline double sum_parts(const double* part, int n, double* slot) {
+  if (threadIdx.x < 64) {
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) t += part[i];
+    t = wave_sum(t);
+    if (threadIdx.x == 0) *slot = t;
   }
   __syncthreads();
-  const bool last = *flag;
-  if (last) __threadfence();
-  return last;
+  return *slot;
 }
 
-// sum of the partials in index order (same result whichever workgroup came last)
-__device__ inline double sum_parts(const double* part, int n) {
-  double t = 0.0;
-  for (int i = 0; i < n; ++i) t += __hip_atomic_load(&part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return t;
+// r . r after the update of iteration jj (jj = -1: after K0, where 0 becomes 1e-6)
+template <typename S>
+__device__ inline double rn_after(const CgPlan& P, const CgBufs<S>& B, int jj, double* slot) {
+  double v = sum_parts(B.rq[jj & 1], P.nbv, slot);
+  if (jj < 0 && v == 0.0) v = 1e-6;
+  return v;
 }
 
 // K0: r = M grad, p_0 = r, y = 0, rn = r . r (1e-6 when zero).
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
   __shared__ double red[NTH / 64];
-  __shared__ bool last;
   const Shape& s = P.s;
   double acc = 0.0;
   for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
@@ -120,30 +123,45 @@ __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
     acc += (double)rr * (double)rr;
   }
   const double v = wg_sum(acc, red);
-  if (arrive_last(v, B.part, &B.cnt[CN_K0], &last) && threadIdx.x == 0) {
-    double rn = sum_parts(B.part, gridDim.x);
-    if (rn == 0.0) rn = 1e-6;
-    B.sc[SC_RN] = rn;
-    B.cnt[CN_K0] = 0;
-  }
+  if (threadIdx.x == 0) B.rq[1][blockIdx.x] = v;
 }
 
 // K1: A_part[m][split][c][r] = sum_{i in split} q_m[i][c] F_m[i][r].
 //   raw: q = grad (J^T J grad); else q = M p_j with p_j formed here (j >= 1) and written by the
-//   r-block-0 workgroups.  Block 0 of the raw pass also clears the counters and the stop flag.
+//   r-block-0 workgroups.  Block 0 of the raw pass also clears the stop flag and fills Mv.
+//   For j >= 1 every workgroup first forms rn_j = r_j . r_j: rn_j <= tol stops the CG (the
+//   previous iteration was the last), else beta = rn_j / rn_{j-1}.
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, int j) {
   __shared__ double qs[KI][CT + 1];
   __shared__ double fs[KI][CT + 1];
   const Shape& s = P.s;
-  if (raw && blockIdx.x == 0 && threadIdx.x < CN_N) B.cnt[threadIdx.x] = 0u;
+  __shared__ double slots[2];
+  __shared__ int stop_s;
   if (raw && blockIdx.x == 0 && threadIdx.x == 0) {
     B.sc[SC_DONE] = 0.0;
     B.sc[SC_ITN] = 0.0;
   }
   if (raw && blockIdx.x == 0)
     for (int e = threadIdx.x; e < s.L * s.R; e += NTH) B.Mv[e] = inv_m(B.gamma[e], P.damp);
-  if (!raw && B.sc[SC_DONE] != 0.0) return;
+  if (!raw) {
+    // workgroups of this kernel may set the flag: read it once for the whole workgroup
+    if (threadIdx.x == 0) stop_s = B.sc[SC_DONE] != 0.0;
+    __syncthreads();
+    if (stop_s) return;
+  }
+  double beta = 0.0;
+  if (!raw && j >= 1) {
+    const double rn_j = rn_after(P, B, j - 1, &slots[0]);
+    if (rn_j <= P.tol) {
+      if (threadIdx.x == 0) {
+        B.sc[SC_DONE] = 1.0;
+        B.sc[SC_ITN] = (double)(j - 1);
+      }
+      return;
+    }
+    beta = rn_j / rn_after(P, B, j - 2, &slots[1]);
+  }
   int m = 0;
   while (m + 1 < s.L && (int)blockIdx.x >= P.k1first[m + 1]) ++m;
   const int nrb = P.nrb;
@@ -157,7 +175,6 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   const int R = s.R;
   const S* Fm = B.F + s.off[m];
   const int64_t off = s.off[m];
-  const double beta = (!raw && j >= 1) ? B.sc[SC_BETA] : 0.0;
   const S* pold = B.p[(j + 1) & 1];
   S* pnew = B.p[j & 1];
   const bool writer = !raw && j >= 1 && rbk == 0;
@@ -224,7 +241,6 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   __shared__ double Ts[ZC][CT];
   __shared__ double P1s[ZC][CT];
   __shared__ double red[NTH / 64];
-  __shared__ bool last;
   const Shape& s = P.s;
   if (!raw && B.sc[SC_DONE] != 0.0) return;
   int l = 0;
@@ -341,22 +357,21 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   }
   if (raw) return;
   const double v = wg_sum(dot, red);
-  if (arrive_last(v, B.part, &B.cnt[CN_K2], &last) && threadIdx.x == 0) {
-    double den = sum_parts(B.part, gridDim.x);
-    if (den == 0.0) den = 1e-6;
-    B.sc[SC_ALPHA] = B.sc[SC_RN] / den;
-    B.cnt[CN_K2] = 0;
-  }
+  if (threadIdx.x == 0) B.pz[j & 1][blockIdx.x] = v;
 }
 
-// K3: y += alpha p_j, r -= alpha z, rn_new = r . r; beta = rn_new / rn; stop when rn <= tol.
+// K3: alpha = rn_j / (p_j . z) (0 becomes 1e-6), y += alpha p_j, r -= alpha z, partials of
+// the new r . r.
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_upd(CgPlan P, CgBufs<S> B, int j) {
   __shared__ double red[NTH / 64];
-  __shared__ bool last;
+  __shared__ double slots[2];
   const Shape& s = P.s;
   if (B.sc[SC_DONE] != 0.0) return;
-  const S aS = (S)B.sc[SC_ALPHA];
+  const double rn_j = rn_after(P, B, j - 1, &slots[0]);
+  double den = sum_parts(B.pz[j & 1], P.nk2, &slots[1]);
+  if (den == 0.0) den = 1e-6;
+  const S aS = (S)(rn_j / den);
   const S* pj = B.p[j & 1];
   double acc = 0.0;
   for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
@@ -366,16 +381,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_upd(CgPlan P, CgBufs<S> B, int j) {
     acc += (double)rv * (double)rv;
   }
   const double v = wg_sum(acc, red);
-  if (arrive_last(v, B.part, &B.cnt[CN_K3], &last) && threadIdx.x == 0) {
-    const double rn_new = sum_parts(B.part, gridDim.x);
-    B.sc[SC_BETA] = rn_new / B.sc[SC_RN];
-    B.sc[SC_RN] = rn_new;
-    if (rn_new <= P.tol) {
-      B.sc[SC_DONE] = 1.0;
-      B.sc[SC_ITN] = (double)j;
-    }
-    B.cnt[CN_K3] = 0;
-  }
+  if (threadIdx.x == 0) B.rq[j & 1][blockIdx.x] = v;
 }
 
 // y_out = M y; scal = [iterations done, final residual norm^2]
@@ -389,10 +395,14 @@ __global__ __launch_bounds__(NTH) void k_cgm_end(CgPlan P, CgBufs<S> B, int maxi
     const int rk = (int)(w / s.d[l]);
     y[e] = (S)(B.Mv[l * s.R + rk] * (double)B.yc[e]);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  __shared__ double slot;
+  if (blockIdx.x == 0) {
     const int itn = B.sc[SC_DONE] != 0.0 ? (int)B.sc[SC_ITN] : maxiter - 1;
-    scal[0] = (double)(itn + 1);
-    scal[1] = B.sc[SC_RN];
+    const double rn = rn_after(P, B, itn, &slot);
+    if (threadIdx.x == 0) {
+      scal[0] = (double)(itn + 1);
+      scal[1] = rn;
+    }
   }
 }
 
@@ -429,12 +439,13 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   P.k1first[s.L] = k1;
   P.k2first[s.L] = k2;
   const unsigned nbv = nblocks_vec(s.n);
-  const int npart = std::max<int>(k2, (int)nbv);
+  P.nbv = (int)nbv;
+  P.nk2 = k2;
   const size_t vec = align256(sizeof(S) * (size_t)s.n);
   const size_t bytes = 5 * vec + align256(sizeof(double) * (size_t)aoff) +
                        align256(sizeof(double) * (size_t)s.L * s.R) +
-                       align256(sizeof(double) * (size_t)npart) + align256(sizeof(double) * SC_N) +
-                       align256(sizeof(unsigned) * CN_N);
+                       2 * align256(sizeof(double) * (size_t)k2) +
+                       2 * align256(sizeof(double) * (size_t)nbv) + align256(sizeof(double) * SC_N);
   void* wsp;
   int rc = tfx_workspace(ctx, bytes + 256, st, &wsp);
   if (rc) return rc;
@@ -452,11 +463,17 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   B.yc = (S*)w; w += vec;
   B.A = (double*)w; w += align256(sizeof(double) * (size_t)aoff);
   B.Mv = (double*)w; w += align256(sizeof(double) * (size_t)s.L * s.R);
-  B.part = (double*)w; w += align256(sizeof(double) * (size_t)npart);
-  B.sc = (double*)w; w += align256(sizeof(double) * SC_N);
-  B.cnt = (unsigned*)w;
+  for (int q = 0; q < 2; ++q) {
+    B.pz[q] = (double*)w;
+    w += align256(sizeof(double) * (size_t)k2);
+  }
+  for (int q = 0; q < 2; ++q) {
+    B.rq[q] = (double*)w;
+    w += align256(sizeof(double) * (size_t)nbv);
+  }
+  B.sc = (double*)w;
 
-  // J^T J grad for the dogleg (also clears counters and the stop flag, and fills Mv)
+  // J^T J grad for the dogleg (also clears the stop flag and fills Mv)
   hipLaunchKernelGGL(k_cgm_A<S>, dim3(k1), dim3(NTH), 0, st, P, B, 1, 0);
   TFX_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_cgm_Z<S>, dim3(k2), dim3(NTH), 0, st, P, B, 1, 0, jtjg);
