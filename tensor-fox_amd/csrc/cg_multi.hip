@@ -19,6 +19,8 @@
 
 namespace {
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 constexpr int CT = 16;        // c / r tile of the R x R products
 constexpr int ICH = 128;      // i per workgroup in K1 (split of the reduction over I_m)
 constexpr int KI = 32;        // i staged per step in K1
@@ -256,8 +258,10 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   const int64_t off = s.off[l];
   const S* pj = B.p[j & 1];
   const int t = threadIdx.x;
-  const int tr = t & 15, ig = t >> 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // wave w owns rows i0 + 16 w .. + 15 of the 64 x 16 output block, as one FP64 MFMA tile:
+  // register j of lane (li, lg) holds row 16 w + 4 j + lg, column r0 + li
+  const int wv = t >> 6, li = t & 15, lg = (t >> 4) & 3;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   const S* qsrc = raw ? B.grad : pj;
   for (int cc0 = 0; cc0 < R; cc0 += ZC) {
     // Loads first, values after: every loop below has a compile-time trip count and
@@ -328,19 +332,20 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       Qs[ii][cc] = ok ? (raw ? (double)qv[k] : B.Mv[l * R + c] * (double)qv[k]) : 0.0;
     }
     __syncthreads();
-    const int cn = R - cc0 < ZC ? R - cc0 : ZC;
-    for (int cc = 0; cc < cn; ++cc) {
-      const double tv = Ts[cc][tr], pv = P1s[cc][tr];
+    // z[i][r] += sum_c F[i][c] tmp[c][r] + q[i][c] P1[c][r] (columns past R are zeros)
 #pragma unroll
-      for (int a = 0; a < 4; ++a) acc[a] += Fs[ig + 16 * a][cc] * tv + Qs[ig + 16 * a][cc] * pv;
+    for (int k4 = 0; k4 < ZC / 4; ++k4) {
+      const int cc = 4 * k4 + lg;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Fs[16 * wv + li][cc], Ts[cc][li], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Qs[16 * wv + li][cc], P1s[cc][li], acc, 0, 0, 0);
     }
     __syncthreads();
   }
   double dot = 0.0;
-  const int rr = r0 + tr;
+  const int rr = r0 + li;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int64_t i = i0 + ig + 16 * a;
+    const int64_t i = i0 + 16 * wv + 4 * a + lg;
     if (i < I && rr < R) {
       const int64_t e = off + (int64_t)rr * I + i;
       if (raw) {
