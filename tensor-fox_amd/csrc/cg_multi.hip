@@ -1,20 +1,22 @@
 // Split-preconditioned CG (GaussNewton.py:315-398) for problems whose operands do not fit the
 // LDS of one workgroup: the refinement problem on the full tensor and large compressed cores.
-// Every CG iteration is three stream-ordered kernels over many workgroups:
-//   K1  p_j = r + beta (j == 1 ? r : p_{j-1})  (j >= 1, first iteration: p_0 = r), and
+// Every CG iteration j is two stream-ordered kernels over many workgroups:
+//   K1  the update of iteration j - 1 (alpha = rn_{j-1} / (p_{j-1} . z_{j-1}): y += alpha p,
+//       r_j = r_{j-1} - alpha z), then p_j = r_j + beta_j (j == 1 ? r_j : p_{j-1}) and
 //       A_m = q_m^T F_m with q = M p_j (split over i, FP64 partials per split)
 //   K2  z_l = M (F_l tmp_l + q_l P1_l + damp Gamma q_l),  tmp_l = sum_{m != l} P2[l][m] o A_m,
-//       and the dot p_j . z, whose last arriving workgroup forms alpha = rn / (p . z)
-//   K3  y += alpha p_j, r -= alpha z, and r . r, whose last workgroup forms beta, rn and the
-//       stop flag (rn <= tol)
-// A dot product is left as one partial per workgroup; every workgroup of the next kernel that
+//       and the per-workgroup partials of p_j . z_j, r_j . z_j, z_j . z_j and r_j . r_j
+// beta_j = rn_j / rn_{j-1} takes rn_j = rn_{j-1} - 2 alpha (r . z) + alpha^2 (z . z) (the exact
+// r_j . r_j is only known after K2(j)); alpha and the stop test use the exact dot products.
+// A dot product is left as one partial per workgroup; every workgroup of a later kernel that
 // needs it sums the partials itself in a fixed order (the kernel boundary has published them),
 // so no atomics, fences or waits are involved and all workgroups get the same value.  Partials
-// are kept per iteration parity, so a kernel reads the previous iteration's while writing its
-// own.  K1 detects convergence (rn <= tol after the previous update) and sets a stop flag;
-// kernels launched after it return at once, so the host launches maxiter iterations without
-// reading anything back.  Dots and norms are FP64; vectors are in
-// the working type S.  J^T J grad (for the dogleg) is the same K1/K2 pair without M and damping.
+// and r are kept per iteration parity, so a kernel reads the previous iteration's while writing
+// its own.  K1(j) stops the CG when rn_{j-1} <= tol (the update of iteration j - 2 was the last)
+// by setting a flag; kernels launched after it return at once, so the host launches maxiter
+// iterations without reading anything back.  The end kernel applies the last iteration's update
+// (unless its residual already met the tolerance).  Dots and norms are FP64; vectors are in the
+// working type S.  J^T J grad (for the dogleg) is the same K1/K2 pair without M and damping.
 #include "common.h"
 
 namespace {
@@ -28,8 +30,10 @@ constexpr int ZI = 64;        // rows i per workgroup in K2
 constexpr int ZC = 64;        // c per LDS chunk in K2
 constexpr int NTH = 256;
 
-// device scalar block (double) and counters (unsigned)
+// device scalars: stop flag and the last iteration whose update was applied
 enum { SC_DONE = 0, SC_ITN, SC_N };
+// K2 dot products, each nk2 partials per iteration parity
+enum { D_PZ = 0, D_RZ, D_ZZ, D_RR, D_N };
 
 struct CgPlan {
   Shape s;
@@ -38,7 +42,7 @@ struct CgPlan {
   int k1first[TFX_MAXL + 1];     // first K1 block of each mode
   int k2first[TFX_MAXL + 1];     // first K2 block of each mode
   int64_t aoff[TFX_MAXL];        // offset of mode m's partials in A_part: [split][c][r]
-  int nbv, nk2;                  // workgroups of the vector kernels (K0, K3) and of K2
+  int nbv, nk2;                  // workgroups of the vector kernels (K0, end) and of K2
   double damp, tol;
 };
 
@@ -49,15 +53,15 @@ struct CgBufs {
   const S* P2;
   const S* gamma;
   const S* grad;
-  S* r;
-  S* p[2];
+  S* r[2];          // r_j in r[j & 1]
+  S* p[2];          // p_j in p[j & 1]
   S* z;
   S* yc;
   double* A;        // A_part
   double* Mv;       // preconditioner M[l][c] = 1 / sqrt(gamma^2 + damp^2 gamma^2), L x R
-  double* pz[2];    // K2 partials of p . z, by iteration parity
-  double* rq[2];    // K3 partials of r . r by iteration parity (K0's in rr[1])
-  double* sc;       // stop flag and the iteration it was set at
+  double* dots[2];  // [parity][D_N][nk2] partials of K2
+  double* rq0;      // K0 partials of r_0 . r_0 (read only when maxiter is 0)
+  double* sc;
 };
 
 __device__ inline int mode_of_e(const Shape& s, int64_t e, int64_t* within) {
@@ -84,10 +88,8 @@ __device__ inline double wg_sum(double v, double* red) {
   return t;
 }
 
-// Publish this workgroup's partial and report whether it is the last to arrive; the last one
-// sees every partial (release fence before the counter, acquire fence after it).
 // Sum of n per-workgroup partials written by an earlier kernel: wave 0 sums them in a fixed
-// lane order and shares the result through `slot` (one LDS double per call site).
+// lane order and shares the result through `slot` (LDS).
 __device__ inline double sum_parts(const double* part, int n, double* slot) {
   if (threadIdx.x < 64) {
     double t = 0.0;
@@ -99,15 +101,40 @@ __device__ inline double sum_parts(const double* part, int n, double* slot) {
   return *slot;
 }
 
-// r . r after the update of iteration jj (jj = -1: after K0, where 0 becomes 1e-6)
+// The scalars of iteration jj from K2(jj)'s partials: rn = r_jj . r_jj (0 becomes 1e-6 at
+// jj = 0), alpha = rn / (p . z) (a zero p . z becomes 1e-6), and the recurrence for the next
+// r . r.  `slots` holds D_N LDS doubles.
+struct IterScalars {
+  double rn, alpha, rn_next;
+};
+
 template <typename S>
-__device__ inline double rn_after(const CgPlan& P, const CgBufs<S>& B, int jj, double* slot) {
-  double v = sum_parts(B.rq[jj & 1], P.nbv, slot);
-  if (jj < 0 && v == 0.0) v = 1e-6;
+__device__ inline IterScalars iter_scalars(const CgPlan& P, const CgBufs<S>& B, int jj,
+                                           double* slots) {
+  const double* d = B.dots[jj & 1];
+  if (threadIdx.x < 64) {
+    double t[D_N] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < P.nk2; i += 64)
+#pragma unroll
+      for (int q = 0; q < D_N; ++q) t[q] += d[q * P.nk2 + i];
+#pragma unroll
+    for (int q = 0; q < D_N; ++q) t[q] = wave_sum(t[q]);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int q = 0; q < D_N; ++q) slots[q] = t[q];
+  }
+  __syncthreads();
+  IterScalars v;
+  v.rn = slots[D_RR];
+  if (jj == 0 && v.rn == 0.0) v.rn = 1e-6;
+  double pz = slots[D_PZ];
+  if (pz == 0.0) pz = 1e-6;
+  v.alpha = v.rn / pz;
+  v.rn_next = v.rn - 2.0 * v.alpha * slots[D_RZ] + v.alpha * v.alpha * slots[D_ZZ];
   return v;
 }
 
-// K0: r = M grad, p_0 = r, y = 0, rn = r . r (1e-6 when zero).
+// K0: r_0 = M grad, p_0 = r_0, y = 0, partials of r_0 . r_0.
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
   __shared__ double red[NTH / 64];
@@ -119,27 +146,27 @@ __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
     const int rk = (int)(w / s.d[l]);
     const double M = B.Mv[l * s.R + rk];
     const S rr = (S)(M * (double)B.grad[e]);
-    B.r[e] = rr;
+    B.r[0][e] = rr;
     B.p[0][e] = rr;
     B.yc[e] = S(0);
     acc += (double)rr * (double)rr;
   }
   const double v = wg_sum(acc, red);
-  if (threadIdx.x == 0) B.rq[1][blockIdx.x] = v;
+  if (threadIdx.x == 0) B.rq0[blockIdx.x] = v;
 }
 
 // K1: A_part[m][split][c][r] = sum_{i in split} q_m[i][c] F_m[i][r].
-//   raw: q = grad (J^T J grad); else q = M p_j with p_j formed here (j >= 1) and written by the
-//   r-block-0 workgroups.  Block 0 of the raw pass also clears the stop flag and fills Mv.
-//   For j >= 1 every workgroup first forms rn_j = r_j . r_j: rn_j <= tol stops the CG (the
-//   previous iteration was the last), else beta = rn_j / rn_{j-1}.
+//   raw: q = grad (J^T J grad); else q = M p_j.  For j >= 1 every workgroup first forms the
+//   scalars of iteration j - 1 (stopping the CG when rn_{j-1} <= tol, j >= 2), then updates its
+//   elements: the r-block-0 workgroups store y, r_j and p_j.  Block 0 of the raw pass clears
+//   the stop flag and fills Mv.
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, int j) {
   __shared__ double qs[KI][CT + 1];
   __shared__ double fs[KI][CT + 1];
-  const Shape& s = P.s;
-  __shared__ double slots[2];
+  __shared__ double slots[D_N];
   __shared__ int stop_s;
+  const Shape& s = P.s;
   if (raw && blockIdx.x == 0 && threadIdx.x == 0) {
     B.sc[SC_DONE] = 0.0;
     B.sc[SC_ITN] = 0.0;
@@ -152,18 +179,20 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
     __syncthreads();
     if (stop_s) return;
   }
-  double beta = 0.0;
+  double alpha = 0.0, beta = 0.0;
   if (!raw && j >= 1) {
-    const double rn_j = rn_after(P, B, j - 1, &slots[0]);
-    if (rn_j <= P.tol) {
+    const IterScalars v = iter_scalars(P, B, j - 1, slots);
+    if (j >= 2 && v.rn <= P.tol) {
       if (threadIdx.x == 0) {
         B.sc[SC_DONE] = 1.0;
-        B.sc[SC_ITN] = (double)(j - 1);
+        B.sc[SC_ITN] = (double)(j - 2);
       }
       return;
     }
-    beta = rn_j / rn_after(P, B, j - 2, &slots[1]);
+    alpha = v.alpha;
+    beta = v.rn_next / v.rn;
   }
+  const S aS = (S)alpha;
   int m = 0;
   while (m + 1 < s.L && (int)blockIdx.x >= P.k1first[m + 1]) ++m;
   const int nrb = P.nrb;
@@ -179,8 +208,10 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   const int64_t off = s.off[m];
   const S* pold = B.p[(j + 1) & 1];
   S* pnew = B.p[j & 1];
-  const bool writer = !raw && j >= 1 && rbk == 0;
-  const S* src1 = raw ? B.grad : (j == 0 ? B.p[0] : B.r);
+  S* rnew = B.r[j & 1];
+  const bool upd = !raw && j >= 1;
+  const bool writer = upd && rbk == 0;
+  const S* src1 = raw ? B.grad : (j == 0 ? B.p[0] : B.r[(j + 1) & 1]);
   const int t = threadIdx.x;
   const int tc = t >> 4, tr = t & 15;
   double acc = 0.0;
@@ -188,7 +219,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
     // stage q[i0..i0+31][c0..c0+15] and F[i0..i0+31][r0..r0+15] (coalesced along i); loads
     // first (clamped indices), values after
     constexpr int NK1 = KI * CT / NTH;
-    S v1[NK1], vo[NK1], vf[NK1];
+    S v1[NK1], vo[NK1], vz[NK1], vy[NK1], vf[NK1];
 #pragma unroll
     for (int k = 0; k < NK1; ++k) {
       const int u = t + NTH * k;
@@ -198,6 +229,8 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
       const int64_t e = (i < ie && c < R) ? off + (int64_t)c * I + i : 0;
       v1[k] = src1[e];
       vo[k] = pold[e];
+      vz[k] = B.z[e];
+      vy[k] = B.yc[e];
       vf[k] = Fm[(i < ie && rr < R) ? (int64_t)rr * I + i : 0];
     }
 #pragma unroll
@@ -216,9 +249,14 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
           if (j == 0) {
             pv = v1[k];
           } else {
-            const S rv = v1[k];
+            const S rv = v1[k] - aS * vz[k];
             pv = (S)((double)rv + beta * (double)(j == 1 ? rv : vo[k]));
-            if (writer) pnew[off + (int64_t)c * I + i] = pv;
+            if (writer) {
+              const int64_t e = off + (int64_t)c * I + i;
+              B.yc[e] = vy[k] + aS * vo[k];
+              rnew[e] = rv;
+              pnew[e] = pv;
+            }
           }
           qv = B.Mv[m * R + c] * (double)pv;
         }
@@ -242,7 +280,6 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   __shared__ double Qs[ZI][ZC + 1];
   __shared__ double Ts[ZC][CT];
   __shared__ double P1s[ZC][CT];
-  __shared__ double red[NTH / 64];
   const Shape& s = P.s;
   if (!raw && B.sc[SC_DONE] != 0.0) return;
   int l = 0;
@@ -341,7 +378,8 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
     }
     __syncthreads();
   }
-  double dot = 0.0;
+  double dpz = 0.0, drz = 0.0, dzz = 0.0, drr = 0.0;
+  const S* rj = B.r[j & 1];
   const int rr = r0 + li;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -356,58 +394,78 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
         const double q = M * (double)pj[e];
         const S zv = (S)(M * (acc[a] + P.damp * g * q));
         out[e] = zv;
-        dot += (double)pj[e] * (double)zv;
+        const double zd = (double)zv, rd = (double)rj[e];
+        dpz += (double)pj[e] * zd;
+        drz += rd * zd;
+        dzz += zd * zd;
+        drr += rd * rd;
       }
     }
   }
   if (raw) return;
-  const double v = wg_sum(dot, red);
-  if (threadIdx.x == 0) B.pz[j & 1][blockIdx.x] = v;
-}
-
-// K3: alpha = rn_j / (p_j . z) (0 becomes 1e-6), y += alpha p_j, r -= alpha z, partials of
-// the new r . r.
-template <typename S>
-__global__ __launch_bounds__(NTH) void k_cgm_upd(CgPlan P, CgBufs<S> B, int j) {
-  __shared__ double red[NTH / 64];
-  __shared__ double slots[2];
-  const Shape& s = P.s;
-  if (B.sc[SC_DONE] != 0.0) return;
-  const double rn_j = rn_after(P, B, j - 1, &slots[0]);
-  double den = sum_parts(B.pz[j & 1], P.nk2, &slots[1]);
-  if (den == 0.0) den = 1e-6;
-  const S aS = (S)(rn_j / den);
-  const S* pj = B.p[j & 1];
-  double acc = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
-    B.yc[e] += aS * pj[e];
-    const S rv = B.r[e] - aS * B.z[e];
-    B.r[e] = rv;
-    acc += (double)rv * (double)rv;
+  // the four block sums in one pass (fixed order)
+  dpz = wave_sum(dpz);
+  drz = wave_sum(drz);
+  dzz = wave_sum(dzz);
+  drr = wave_sum(drr);
+  __shared__ double red4[D_N][NTH / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red4[D_PZ][w] = dpz;
+    red4[D_RZ][w] = drz;
+    red4[D_ZZ][w] = dzz;
+    red4[D_RR][w] = drr;
   }
-  const double v = wg_sum(acc, red);
-  if (threadIdx.x == 0) B.rq[j & 1][blockIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < D_N) {
+    double tot = 0.0;
+    for (int q = 0; q < NTH / 64; ++q) tot += red4[threadIdx.x][q];
+    B.dots[j & 1][threadIdx.x * P.nk2 + blockIdx.x] = tot;
+  }
 }
 
-// y_out = M y; scal = [iterations done, final residual norm^2]
+// y_out = M (y + the last update unless the CG stopped); scal = [iterations done, final
+// residual norm^2].
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_end(CgPlan P, CgBufs<S> B, int maxiter, S* y,
                                                  double* scal) {
+  __shared__ double slots[D_N];
   const Shape& s = P.s;
+  int itn;
+  double rn;
+  bool last_upd = false;
+  S aS = S(0);
+  if (B.sc[SC_DONE] != 0.0) {
+    // stopped in K1(itn + 2): rn_{itn+1} is the exact residual after the last update
+    itn = (int)B.sc[SC_ITN];
+    rn = iter_scalars(P, B, itn + 1, slots).rn;
+  } else if (maxiter <= 0) {
+    itn = -1;
+    rn = sum_parts(B.rq0, P.nbv, &slots[0]);
+    if (rn == 0.0) rn = 1e-6;
+  } else {
+    const IterScalars v = iter_scalars(P, B, maxiter - 1, slots);
+    if (maxiter >= 2 && v.rn <= P.tol) {
+      itn = maxiter - 2;    // the update of iteration maxiter - 2 already met the tolerance
+      rn = v.rn;
+    } else {
+      itn = maxiter - 1;
+      rn = v.rn_next;
+      last_upd = true;
+      aS = (S)v.alpha;
+    }
+  }
+  const S* plast = B.p[(maxiter - 1) & 1];
   for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
     int64_t w;
     const int l = mode_of_e(s, e, &w);
     const int rk = (int)(w / s.d[l]);
-    y[e] = (S)(B.Mv[l * s.R + rk] * (double)B.yc[e]);
+    const S yv = last_upd ? B.yc[e] + aS * plast[e] : B.yc[e];
+    y[e] = (S)(B.Mv[l * s.R + rk] * (double)yv);
   }
-  __shared__ double slot;
-  if (blockIdx.x == 0) {
-    const int itn = B.sc[SC_DONE] != 0.0 ? (int)B.sc[SC_ITN] : maxiter - 1;
-    const double rn = rn_after(P, B, itn, &slot);
-    if (threadIdx.x == 0) {
-      scal[0] = (double)(itn + 1);
-      scal[1] = rn;
-    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[0] = (double)(itn + 1);
+    scal[1] = rn;
   }
 }
 
@@ -447,10 +505,10 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   P.nbv = (int)nbv;
   P.nk2 = k2;
   const size_t vec = align256(sizeof(S) * (size_t)s.n);
-  const size_t bytes = 5 * vec + align256(sizeof(double) * (size_t)aoff) +
+  const size_t bytes = 6 * vec + align256(sizeof(double) * (size_t)aoff) +
                        align256(sizeof(double) * (size_t)s.L * s.R) +
-                       2 * align256(sizeof(double) * (size_t)k2) +
-                       2 * align256(sizeof(double) * (size_t)nbv) + align256(sizeof(double) * SC_N);
+                       2 * align256(sizeof(double) * D_N * (size_t)k2) +
+                       align256(sizeof(double) * (size_t)nbv) + align256(sizeof(double) * SC_N);
   void* wsp;
   int rc = tfx_workspace(ctx, bytes + 256, st, &wsp);
   if (rc) return rc;
@@ -461,7 +519,8 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   B.P2 = P2;
   B.gamma = gamma;
   B.grad = grad;
-  B.r = (S*)w; w += vec;
+  B.r[0] = (S*)w; w += vec;
+  B.r[1] = (S*)w; w += vec;
   B.p[0] = (S*)w; w += vec;
   B.p[1] = (S*)w; w += vec;
   B.z = (S*)w; w += vec;
@@ -469,13 +528,10 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   B.A = (double*)w; w += align256(sizeof(double) * (size_t)aoff);
   B.Mv = (double*)w; w += align256(sizeof(double) * (size_t)s.L * s.R);
   for (int q = 0; q < 2; ++q) {
-    B.pz[q] = (double*)w;
-    w += align256(sizeof(double) * (size_t)k2);
+    B.dots[q] = (double*)w;
+    w += align256(sizeof(double) * D_N * (size_t)k2);
   }
-  for (int q = 0; q < 2; ++q) {
-    B.rq[q] = (double*)w;
-    w += align256(sizeof(double) * (size_t)nbv);
-  }
+  B.rq0 = (double*)w; w += align256(sizeof(double) * (size_t)nbv);
   B.sc = (double*)w;
 
   // J^T J grad for the dogleg (also clears the stop flag and fills Mv)
@@ -489,8 +545,6 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
     hipLaunchKernelGGL(k_cgm_A<S>, dim3(k1), dim3(NTH), 0, st, P, B, 0, jj);
     TFX_CHECK_LAUNCH(ctx);
     hipLaunchKernelGGL(k_cgm_Z<S>, dim3(k2), dim3(NTH), 0, st, P, B, 0, jj, B.z);
-    TFX_CHECK_LAUNCH(ctx);
-    hipLaunchKernelGGL(k_cgm_upd<S>, dim3(nbv), dim3(NTH), 0, st, P, B, jj);
     TFX_CHECK_LAUNCH(ctx);
   }
   hipLaunchKernelGGL(k_cgm_end<S>, dim3(nbv), dim3(NTH), 0, st, P, B, maxiter, y, scal);
