@@ -40,8 +40,20 @@ __global__ __launch_bounds__(256) void k_gram_factors(Shape s, const S* __restri
   for (int c0 = 0; c0 < s.R; c0 += 64) {
     const int c = c0 + lane;
     S acc = S(0);
-    if (c < s.R)
-      for (int64_t i = grp; i < I; i += 4) acc += Fl[(int64_t)r * I + i] * Fl[(int64_t)c * I + i];
+    if (c < s.R) {
+      const S* fr = Fl + (int64_t)r * I;
+      const S* fc = Fl + (int64_t)c * I;
+      int64_t i = grp;
+      for (; i + 12 < I; i += 16) {   // four rows per step, loads issued together
+        const S a0 = fr[i], a1 = fr[i + 4], a2 = fr[i + 8], a3 = fr[i + 12];
+        const S b0 = fc[i], b1 = fc[i + 4], b2 = fc[i + 8], b3 = fc[i + 12];
+        acc += a0 * b0;
+        acc += a1 * b1;
+        acc += a2 * b2;
+        acc += a3 * b3;
+      }
+      for (; i < I; i += 4) acc += fr[i] * fc[i];
+    }
     __syncthreads();
     red[grp][lane] = acc;
     __syncthreads();
@@ -54,37 +66,57 @@ __global__ __launch_bounds__(256) void k_gram_factors(Shape s, const S* __restri
 // gamma[l, r] = max|P1_{L-1}| * sqrt|P1_l[r, r]| (GaussNewton.py:584-606 scales with the LAST
 // mode's max).  One workgroup.
 template <typename S>
-__global__ __launch_bounds__(1024) void k_gram_hadamard(Shape s, const S* __restrict__ Gr,
-                                                         S* __restrict__ P1, S* __restrict__ P2,
-                                                         S* __restrict__ gamma) {
-  __shared__ S red[16];
+__global__ __launch_bounds__(256) void k_gram_hadamard(Shape s, const S* __restrict__ Gr,
+                                                        S* __restrict__ P1, S* __restrict__ P2,
+                                                        S* __restrict__ gamma) {
+  // one thread per (l, entry): the L mode Grams of the entry are loaded first, then
+  // P1[l] = prod_{k != l} Gr_k and P2[l][m] = prod_{k != l, m} Gr_k (mode order, as the reference
+  // forms them); workgroup 0 also forms gamma from max |P1_{L-1}| (GaussNewton.py:584-606),
+  // recomputing the products it needs from Gr
+  __shared__ S red[4];
   const int L = s.L, R = s.R;
   const int64_t RR = (int64_t)R * R;
-  for (int64_t e = threadIdx.x; e < (int64_t)L * RR; e += blockDim.x) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (int64_t)L * RR) {
     const int l = (int)(e / RR);
-    const int64_t rc = e % RR;
+    const int64_t rc = e - (int64_t)l * RR;
+    S g[TFX_MAXL];
+#pragma unroll
+    for (int k = 0; k < TFX_MAXL; ++k) g[k] = Gr[(int64_t)(k < L ? k : 0) * RR + rc];
     S p = S(1);
-    for (int k = 0; k < L; ++k)
-      if (k != l) p *= Gr[k * RR + rc];
+#pragma unroll
+    for (int k = 0; k < TFX_MAXL; ++k)
+      if (k < L && k != l) p *= g[k];
     P1[e] = p;
     for (int m = 0; m < L; ++m) {
       S q = S(1);
       if (m != l)
-        for (int k = 0; k < L; ++k)
-          if (k != l && k != m) q *= Gr[k * RR + rc];
+#pragma unroll
+        for (int k = 0; k < TFX_MAXL; ++k)
+          if (k < L && k != l && k != m) q *= g[k];
       P2[((int64_t)l * L + m) * RR + rc] = q;
     }
   }
-  __syncthreads();
+  if (blockIdx.x != 0) return;   // uniform: the rest is workgroup 0's
   S mx = S(0);
-  for (int64_t e = threadIdx.x; e < RR; e += blockDim.x) {
-    const S v = fabs(P1[(L - 1) * RR + e]);
+  for (int64_t rc = threadIdx.x; rc < RR; rc += blockDim.x) {
+    S g[TFX_MAXL];
+#pragma unroll
+    for (int k = 0; k < TFX_MAXL; ++k) g[k] = Gr[(int64_t)(k < L - 1 ? k : 0) * RR + rc];
+    S p = S(1);
+#pragma unroll
+    for (int k = 0; k < TFX_MAXL; ++k)
+      if (k < L - 1) p *= g[k];
+    const S v = fabs(p);
     mx = v > mx ? v : mx;
   }
   mx = block_max(mx, red);
-  for (int e = threadIdx.x; e < L * R; e += blockDim.x) {
-    const int l = e / R, r = e % R;
-    gamma[e] = mx * sqrt(fabs(P1[l * RR + (int64_t)r * R + r]));
+  for (int e2 = threadIdx.x; e2 < L * R; e2 += blockDim.x) {
+    const int l = e2 / R, r = e2 % R;
+    S p = S(1);
+    for (int k = 0; k < L; ++k)
+      if (k != l) p *= Gr[k * RR + (int64_t)r * R + r];
+    gamma[e2] = mx * sqrt(fabs(p));
   }
 }
 
@@ -101,9 +133,17 @@ __global__ void k_neg_grad(Shape s, const S* __restrict__ F, const S* __restrict
   const int64_t i = w % I;
   const S* Fl = F + s.off[l];
   const S* P = P1 + (int64_t)l * s.R * s.R;
-  S acc = S(0);
-  for (int c = 0; c < s.R; ++c) acc += Fl[(int64_t)c * I + i] * P[(int64_t)c * s.R + r];
-  g[e] = N[e] - acc;
+  // four partial sums (independent loads and FMA chains), combined in a fixed order
+  S a0 = S(0), a1 = S(0), a2 = S(0), a3 = S(0);
+  int c = 0;
+  for (; c + 4 <= s.R; c += 4) {
+    a0 += Fl[(int64_t)c * I + i] * P[(int64_t)c * s.R + r];
+    a1 += Fl[(int64_t)(c + 1) * I + i] * P[(int64_t)(c + 1) * s.R + r];
+    a2 += Fl[(int64_t)(c + 2) * I + i] * P[(int64_t)(c + 2) * s.R + r];
+    a3 += Fl[(int64_t)(c + 3) * I + i] * P[(int64_t)(c + 3) * s.R + r];
+  }
+  for (; c < s.R; ++c) a0 += Fl[(int64_t)c * I + i] * P[(int64_t)c * s.R + r];
+  g[e] = N[e] - ((a0 + a1) + (a2 + a3));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -511,7 +551,8 @@ int gramians_t(tfx_ctx* ctx, const Shape& s, const S* F, S* Gr, S* P1, S* P2, S*
                hipStream_t st) {
   hipLaunchKernelGGL(k_gram_factors<S>, dim3(s.L * s.R), dim3(256), 0, st, s, F, Gr);
   TFX_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_gram_hadamard<S>, dim3(1), dim3(1024), 0, st, s, Gr, P1, P2, gamma);
+  hipLaunchKernelGGL(k_gram_hadamard<S>, dim3((unsigned)(((int64_t)s.L * s.R * s.R + 255) / 256)),
+                     dim3(256), 0, st, s, Gr, P1, P2, gamma);
   TFX_CHECK_LAUNCH(ctx);
   return TFX_OK;
 }
