@@ -76,6 +76,63 @@ __device__ inline void wg_jacobi(double* H, int m, double* V, double* cs, double
         pab[k] = make_int2(a, b);
       }
       __syncthreads();
+      if (hp <= 64) {
+        // H <- J^T H J by 2 x 2 blocks and V <- V J with shift-only index maps: item q of the H
+        // pass is the block (k1, k2) = (q >> sh, q & (2^sh - 1)) with 2^sh >= hp, item q of
+        // the V pass the pair k = q >> 6 and rows (q & 63) + 64 u; the pair indices follow
+        // from the round directly
+        auto pair_of = [&](int k, int& a, int& b) {
+          if (k == 0) {
+            a = round;
+            b = mm - 1;
+          } else {
+            a = round + k;
+            if (a >= mm - 1) a -= mm - 1;
+            b = round - k;
+            if (b < 0) b += mm - 1;
+          }
+          if (a > b) {
+            const int x = a;
+            a = b;
+            b = x;
+          }
+        };
+        const int sh = hp <= 32 ? 5 : 6;
+        const int msk = (1 << sh) - 1;
+        for (int q = t; q < (hp << sh); q += nt) {
+          const int k1 = q >> sh, k2 = q & msk;
+          if (k2 >= hp) continue;
+          int a1, b1, a2, b2;
+          pair_of(k1, a1, b1);
+          pair_of(k2, a2, b2);
+          const double2 r1 = csk[k1], r2 = csk[k2];
+          const double c1 = r1.x, s1 = r1.y, c2 = r2.x, s2 = r2.y;
+          const bool v1 = b1 < m, v2 = b2 < m;
+          const double h00 = H[a1 * m + a2];
+          const double h01 = v2 ? H[a1 * m + b2] : 0.0;
+          const double h10 = v1 ? H[b1 * m + a2] : 0.0;
+          const double h11 = (v1 && v2) ? H[b1 * m + b2] : 0.0;
+          const double x00 = c1 * h00 - s1 * h10, x01 = c1 * h01 - s1 * h11;
+          const double x10 = s1 * h00 + c1 * h10, x11 = s1 * h01 + c1 * h11;
+          H[a1 * m + a2] = c2 * x00 - s2 * x01;
+          if (v2) H[a1 * m + b2] = s2 * x00 + c2 * x01;
+          if (v1) H[b1 * m + a2] = c2 * x10 - s2 * x11;
+          if (v1 && v2) H[b1 * m + b2] = s2 * x10 + c2 * x11;
+        }
+        for (int q = t; q < (hp << 6); q += nt) {
+          const int k = q >> 6;
+          int a, b;
+          pair_of(k, a, b);
+          if (b >= m) continue;   // dummy pair
+          const double2 rk = csk[k];
+          const double c = rk.x, sn = rk.y;
+          for (int row = q & 63; row < m; row += 64) {
+            const double va = V[a * m + row], vb = V[b * m + row];
+            V[a * m + row] = c * va - sn * vb;
+            V[b * m + row] = sn * va + c * vb;
+          }
+        }
+      } else {
       // H <- J^T H J by 2 x 2 blocks (rows of pair k1, columns of pair k2), block q = k1 hp + k2
       for (int q = t; q < hp * hp; q += nt) {
         const int k1 = q / hp, k2 = q - (q / hp) * hp;
@@ -111,6 +168,7 @@ __device__ inline void wg_jacobi(double* H, int m, double* V, double* cs, double
         const double va = V[a * m + row], vb = V[b * m + row];
         V[a * m + row] = c * va - sn * vb;
         V[b * m + row] = sn * va + c * vb;
+      }
       }
       __syncthreads();
     }
