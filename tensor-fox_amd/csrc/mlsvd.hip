@@ -301,16 +301,19 @@ __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
 }
 
 // Jacobi eigensolver of the symmetric m x m matrix H (row-major, global, destroyed), one
-// workgroup with H and V staged in LDS when they fit.  Writes the eigenvectors of the k largest
-// eigenvalues, in descending order, to Vk (m x k col-major) and the eigenvalues to lam.
+// workgroup with H and V staged in LDS when they fit (IN_LDS: a compile-time choice, so that the
+// solver's accesses compile to LDS instructions rather than generic ones).  Writes the
+// eigenvectors of the k largest eigenvalues, in descending order, to Vk (m x k col-major) and the
+// eigenvalues to lam.
+template <bool IN_LDS>
 __global__ __launch_bounds__(1024) void k_jacobi_top(double* H, int m, int k, double* Vg,
-                                                     double* Vk, double* lam, int in_lds) {
+                                                     double* Vk, double* lam) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[16];
   __shared__ __align__(16) double cs[4 * 260];
   __shared__ int order[512];
-  double* Hs = in_lds ? (double*)smem : H;
-  double* Vs = in_lds ? (double*)smem + m * m : Vg;
+  double* Hs = IN_LDS ? (double*)smem : H;
+  double* Vs = IN_LDS ? (double*)smem + m * m : Vg;
   // symmetrized copy (the Rayleigh-Ritz product is symmetric only up to rounding)
   for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
     const int i = e / m, j = e % m;
@@ -548,10 +551,13 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   }
   TFX_CHECK_LAUNCH(ctx);
   const size_t lds = sizeof(double) * 2 * (size_t)m * m;
-  const int in_lds = lds <= 140 * 1024;
-  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_jacobi_top, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(in_lds ? lds : 1)));
-  hipLaunchKernelGGL(k_jacobi_top, dim3(1), dim3(1024), in_lds ? lds : 0, stream, H, m, k, V, Vk, lam, in_lds);
+  if (lds <= 140 * 1024) {
+    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_jacobi_top<true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_jacobi_top<true>, dim3(1), dim3(1024), lds, stream, H, m, k, V, Vk, lam);
+  } else {
+    hipLaunchKernelGGL(k_jacobi_top<false>, dim3(1), dim3(1024), 0, stream, H, m, k, V, Vk, lam);
+  }
   TFX_CHECK_LAUNCH(ctx);
   if (direct) {
     TFX_HIP(ctx, hipMemcpyAsync(U, Vk, sizeof(double) * n * k, hipMemcpyDeviceToDevice, stream));

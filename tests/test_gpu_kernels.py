@@ -388,7 +388,8 @@ def test_gram_mode_split16(dev, dims):
 
 @pytest.mark.parametrize('n,k,p,kind', [(40, 5, 15, 'gap'), (400, 50, 60, 'gap'), (12, 3, 13, 'full'),
                                         (100, 10, 20, 'gap'), (64, 16, 26, 'gap'), (13, 4, 13, 'full'),
-                                        (300, 31, 41, 'gap'), (1500, 20, 30, 'gap')])
+                                        (300, 31, 41, 'gap'), (1500, 20, 30, 'gap'),
+                                        (200, 60, 80, 'gap'), (300, 90, 100, 'gap')])
 def test_eig_top(dev, n, k, p, kind):
     print('eig', n, k, p, flush=True)
     rs = np.random.RandomState(9)
@@ -410,6 +411,53 @@ def test_eig_top(dev, n, k, p, kind):
     Ug = U.cpu().numpy().T
     # subspace agreement and sign convention (largest |entry| positive)
     assert np.linalg.norm(Ug @ (Ug.T @ V) - V) < 1e-8
+    for c in range(k):
+        assert Ug[np.argmax(np.abs(Ug[:, c])), c] > 0
+
+
+def _sym_case(kind, m, rs):
+    if kind == 'rand':
+        X = rs.randn(m, m)
+        return X + X.T
+    if kind == 'graded':
+        X = rs.randn(m, 3 * m) * np.logspace(0, -4, m)[:, None]
+        return X @ X.T
+    if kind == 'rank3':
+        X = rs.randn(m, 3)
+        return X @ X.T
+    if kind == 'repeated':
+        Q, _ = np.linalg.qr(rs.randn(m, m))
+        return (Q * np.repeat([1.0, 2.0], (m + 1) // 2)[:m]) @ Q.T
+    if kind == 'diag':
+        return np.diag(rs.permutation(m).astype(np.float64))
+    if kind == 'tridiag':
+        return np.diag(rs.randn(m)) + np.diag(np.ones(m - 1), 1) + np.diag(np.ones(m - 1), -1)
+    return np.zeros((m, m))
+
+
+@pytest.mark.parametrize('m', [1, 2, 3, 17, 60, 64])
+@pytest.mark.parametrize('kind', ['rand', 'graded', 'rank3', 'repeated', 'diag', 'tridiag', 'zero'])
+def test_eig_direct_small(dev, m, kind):
+    """Direct eigensolve of small blocks: eigenvalues against LAPACK's, and the residual,
+    orthonormality and sign convention of the eigenvectors (repeated eigenvalues leave the
+    vectors free within their eigenspace)."""
+    print('eig direct', m, kind, flush=True)
+    rs = np.random.RandomState(m)
+    G = _sym_case(kind, m, rs)
+    k = max(1, (2 * m) // 3)
+    Gd = dev.upload(G.ravel(), torch.float64)
+    U = dev.empty((k, m), torch.float64)
+    lam_d = dev.empty(k, torch.float64)
+    dev.eig_top(Gd, m, None, m, k, 0, U, lam_d)
+    w = np.linalg.eigvalsh(G)[::-1][:k]
+    nrm = max(np.linalg.norm(G, 2), 1e-300)
+    lam = lam_d.cpu().numpy()
+    Ug = U.cpu().numpy().T
+    assert np.max(np.abs(lam - w)) <= 1e-13 * nrm * max(m, 4)
+    assert np.all(np.diff(lam) <= 0)
+    assert np.linalg.norm(Ug.T @ Ug - np.eye(k)) < 1e-13 * max(m, 4)
+    if kind != 'zero':
+        assert np.linalg.norm(G @ Ug - Ug * lam) <= 1e-13 * nrm * max(m, 4)
     for c in range(k):
         assert Ug[np.argmax(np.abs(Ug[:, c])), c] > 0
 
