@@ -269,8 +269,18 @@ __device__ inline double wave_max(double x) {
   return x;
 }
 
+// The member's geometry, shared by the wave in LDS (indexed by mode at run time)
 struct Member {
   ldscg::Geom g;
+  int64_t st[4];          // C-order strides of the core
+  int ns;                 // core elements
+  double Ssize;           // ||S||
+};
+
+// The member's buffers, held in registers: each pointer is computed from the kernel's LDS base or
+// from its global arguments, so its accesses compile to LDS or global instructions (a pointer
+// loaded from memory would be generic)
+struct MemP {
   ldscg::Lds s;
   const double* S;        // core, C order
   double* vec[NV];
@@ -278,22 +288,19 @@ struct Member {
   double* cn;             // L x R column norms (LDS)
   double* herr;           // errors history (LDS)
   double* himp;           // improv history (LDS)
-  int64_t st[4];          // C-order strides of the core
-  int ns;                 // core elements
-  double Ssize;           // ||S||
 };
 
 // Gram matrices of the factors (from Fs), P1, P2 pairs (LDS) and gamma (GaussNewton.py:520-548,
 // 584-606).  Lane layout of an R x R tile: register j <-> row 4 j + lane / 16, column lane % 16.
 template <int L>
-__device__ inline void b_gramians(const Member& M) {
+__device__ inline void b_gramians(const Member& M, const MemP& P) {
   const ldscg::Geom& g = M.g;
   const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
   const int R = g.R;
   dbl4 gr[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) {
-    const double* fa = M.s.Fs + (int64_t)(g.row[l] + lg) * g.PR + li;
+    const double* fa = P.s.Fs + (int64_t)(g.row[l] + lg) * g.PR + li;
     gr[l] = ldscg::mfma_chain<4>(fa, 4 * g.PR, fa, 4 * g.PR, g.ip4[l] >> 2, dbl4{0.0, 0.0, 0.0, 0.0});
   }
   double mx = 0.0;
@@ -307,28 +314,28 @@ __device__ inline void b_gramians(const Member& M) {
 #pragma unroll
       for (int k = 0; k < L; ++k)
         if (k != l) p *= gr[k][j];
-      if (ok) M.s.P1s[(int64_t)(l * g.R4 + r) * g.PR + c] = p;
+      if (ok) P.s.P1s[(int64_t)(l * g.R4 + r) * g.PR + c] = p;
       if (l == L - 1 && ok) mx = fmax(mx, fabs(p));
-      if (ok && r == c) M.cn[l * R + r] = p;      // diagonal of P1_l (temporary)
+      if (ok && r == c) P.cn[l * R + r] = p;      // diagonal of P1_l (temporary)
 #pragma unroll
       for (int m2 = l + 1; m2 < L; ++m2) {
         double q = 1.0;
 #pragma unroll
         for (int k = 0; k < L; ++k)
           if (k != l && k != m2) q *= gr[k][j];
-        if (ok) M.s.P2s[(int64_t)ldscg::pair_index(l, m2, L) * R * R + r * R + c] = q;
+        if (ok) P.s.P2s[(int64_t)ldscg::pair_index(l, m2, L) * R * R + r * R + c] = q;
       }
     }
   }
   mx = wave_max(mx);
   __syncthreads();
-  if (lane < L * R) M.gam[lane] = mx * sqrt(fabs(M.cn[lane]));
+  if (lane < L * R) P.gam[lane] = mx * sqrt(fabs(P.cn[lane]));
   __syncthreads();
 }
 
 // N_l = S_(l) KR_l for one mode, in the R x R-tile layout (row i = 4 j + lane / 16, rank lane % 16)
 template <int L>
-__device__ inline dbl4 b_mttkrp_mode(const Member& M, int l) {
+__device__ inline dbl4 b_mttkrp_mode(const Member& M, const MemP& P, int l) {
   const ldscg::Geom& g = M.g;
   const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
   int oq[L - 1], od[L - 1];
@@ -348,7 +355,7 @@ __device__ inline dbl4 b_mttkrp_mode(const Member& M, int l) {
   }
   const int kl = g.I[l];
   const int64_t sl = M.st[l];
-  const double* Fs = M.s.Fs;
+  const double* Fs = P.s.Fs;
   dbl4 acc{0.0, 0.0, 0.0, 0.0};
   for (int e0 = 0; e0 < n_rest; e0 += 16) {
     double av[4], bv[4];
@@ -368,7 +375,7 @@ __device__ inline dbl4 b_mttkrp_mode(const Member& M, int l) {
         b *= Fs[(int64_t)(g.row[oq[c]] + ix) * g.PR + li];
       }
       const int ic = li < kl ? li : kl - 1;
-      const double a = M.S[ic * sl + off];
+      const double a = P.S[ic * sl + off];
       av[u] = (ok && li < kl) ? a : 0.0;
       bv[u] = ok ? b : 0.0;
     }
@@ -381,22 +388,22 @@ __device__ inline dbl4 b_mttkrp_mode(const Member& M, int l) {
 // grad (minus the gradient, N - F P1) into vec[V_G] (flat) and qs; returns max |grad| over the
 // wave's entries (reduced by the caller when needed)
 template <int L>
-__device__ inline void b_grad(const Member& M) {
+__device__ inline void b_grad(const Member& M, const MemP& P) {
   const ldscg::Geom& g = M.g;
   const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
 #pragma unroll
   for (int l = 0; l < L; ++l) {
-    const dbl4 N = b_mttkrp_mode<L>(M, l);
-    const double* fa = M.s.Fs + (int64_t)(g.row[l] + li) * g.PR + lg;
-    const double* pb = M.s.P1s + (int64_t)(l * g.R4 + lg) * g.PR + li;
+    const dbl4 N = b_mttkrp_mode<L>(M, P, l);
+    const double* fa = P.s.Fs + (int64_t)(g.row[l] + li) * g.PR + lg;
+    const double* pb = P.s.P1s + (int64_t)(l * g.R4 + lg) * g.PR + li;
     const dbl4 fp = ldscg::mfma_chain<4>(fa, 4, pb, 4 * g.PR, g.R4 >> 2, dbl4{0.0, 0.0, 0.0, 0.0});
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i = 4 * j + lg, r = li;
       if (i < g.I[l] && r < g.R) {
         const double v = N[j] - fp[j];
-        M.vec[V_G][g.off[l] + (int64_t)r * g.I[l] + i] = v;
-        M.s.qs[(int64_t)(g.row[l] + i) * g.PR + r] = v;
+        P.vec[V_G][g.off[l] + (int64_t)r * g.I[l] + i] = v;
+        P.s.qs[(int64_t)(g.row[l] + i) * g.PR + r] = v;
       }
     }
   }
@@ -405,10 +412,10 @@ __device__ inline void b_grad(const Member& M) {
 
 // ||S - [[F]]||^2 with F from Fs: rows over modes 0..L-2 (16 at a time), columns = mode L-1
 template <int L>
-__device__ inline double b_err2(const Member& M) {
+__device__ inline double b_err2(const Member& M, const MemP& P) {
   const ldscg::Geom& g = M.g;
   const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
-  const double* Fs = M.s.Fs;
+  const double* Fs = P.s.Fs;
   const int K = g.I[L - 1];
   int NR = 1;
 #pragma unroll
@@ -447,7 +454,7 @@ __device__ inline double b_err2(const Member& M) {
       const int row = rt + 4 * j + lg;
       const bool v = row < NR && li < K;
       const int rc = row < NR ? row : NR - 1, cc = li < K ? li : K - 1;
-      const double d = M.S[(int64_t)rc * K + cc] - acc[j];
+      const double d = P.S[(int64_t)rc * K + cc] - acc[j];
       e2 += v ? d * d : 0.0;
     }
   }
@@ -455,7 +462,7 @@ __device__ inline double b_err2(const Member& M) {
 }
 
 // Fs <- flat factors
-__device__ inline void b_stage_fs(const Member& M, const double* F) {
+__device__ inline void b_stage_fs(const Member& M, const MemP& P, const double* F) {
   const ldscg::Geom& g = M.g;
   for (int64_t e = threadIdx.x; e < g.n; e += 64) {
     int l = 0;
@@ -463,7 +470,7 @@ __device__ inline void b_stage_fs(const Member& M, const double* F) {
     const int64_t w = e - g.off[l];
     const int r = (int)(w / g.I[l]);
     const int i = (int)(w - (int64_t)r * g.I[l]);
-    M.s.Fs[(int64_t)(g.row[l] + i) * g.PR + r] = F[e];
+    P.s.Fs[(int64_t)(g.row[l] + i) * g.PR + r] = F[e];
   }
   __syncthreads();
 }
@@ -474,7 +481,7 @@ struct UpdOut {
 
 // xn = x + y, F = transform(equalize(xn)) (Conversion.py:361-444), the gain-ratio sums
 // (k_dgn_update in dgn.hip); Fs gets F.
-__device__ inline UpdOut b_update(const Member& M, const double* x, const double* y,
+__device__ inline UpdOut b_update(const Member& M, const MemP& P, const double* x, const double* y,
                                   const double* gv, double damp, bool eq, int symm,
                                   double fnorm, double* xn, double* F) {
   const ldscg::Geom& g = M.g;
@@ -504,7 +511,7 @@ __device__ inline UpdOut b_update(const Member& M, const double* x, const double
       const double* col = F + g.off[l] + (int64_t)r * g.I[l];
       double a = 0.0;
       for (int i = 0; i < g.I[l]; ++i) a += col[i] * col[i];
-      M.cn[t] = sqrt(a);
+      P.cn[t] = sqrt(a);
     }
     __syncthreads();
     for (int64_t e = t; e < g.n; e += 64) {
@@ -512,8 +519,8 @@ __device__ inline UpdOut b_update(const Member& M, const double* x, const double
       while (l + 1 < L && e >= g.off[l + 1]) ++l;
       const int r = (int)((e - g.off[l]) / g.I[l]);
       double prod = 1.0;
-      for (int k = 0; k < L; ++k) prod *= M.cn[k * R + r];
-      if (prod != 0.0) F[e] = pow(prod, 1.0 / L) / M.cn[l * R + r] * F[e];
+      for (int k = 0; k < L; ++k) prod *= P.cn[k * R + r];
+      if (prod != 0.0) F[e] = pow(prod, 1.0 / L) / P.cn[l * R + r] * F[e];
     }
     __syncthreads();
   }
@@ -542,7 +549,7 @@ __device__ inline UpdOut b_update(const Member& M, const double* x, const double
     }
     __syncthreads();
   }
-  b_stage_fs(M, F);
+  b_stage_fs(M, P, F);
   return o;
 }
 
@@ -563,26 +570,26 @@ __device__ inline void vcopy(double* dst, const double* src, int64_t n) {
 // path.  Leaves the best trial's factors in vec[V_F] / Fs, its x in xout and its step in
 // vec[V_Y]; returns its error.
 template <int L>
-__device__ inline double b_dogleg(const Member& M, double error, double damp, const BOpts& o,
+__device__ inline double b_dogleg(const Member& M, const MemP& P, double error, double damp, const BOpts& o,
                                   double* xout) {
   const ldscg::Geom& g = M.g;
   const int64_t n = g.n;
-  double* cx = M.vec[V_T1];
-  double* cy = M.vec[V_T2];
-  double* nx = M.vec[V_T3];
-  double* ny = M.vec[V_T4];
-  double* bx = M.vec[V_T5];
-  double* by = M.vec[V_T6];
-  double* bF = M.vec[V_T7];
-  vcopy(cy, M.vec[V_Y], n);
+  double* cx = P.vec[V_T1];
+  double* cy = P.vec[V_T2];
+  double* nx = P.vec[V_T3];
+  double* ny = P.vec[V_T4];
+  double* bx = P.vec[V_T5];
+  double* by = P.vec[V_T6];
+  double* bF = P.vec[V_T7];
+  vcopy(cy, P.vec[V_Y], n);
   vcopy(bx, cx, n);
   vcopy(by, cy, n);
-  vcopy(bF, M.vec[V_F], n);
+  vcopy(bF, P.vec[V_F], n);
   double best_error = error;
   double gr = 1.0, delta = 1.0;
   int count = 0;
-  const double* gv = M.vec[V_G];
-  const double* jg = M.vec[V_JG];
+  const double* gv = P.vec[V_G];
+  const double* jg = P.vec[V_JG];
   while (gr > 0.0) {
     const double old_error = error;
     double yy = 0.0, yg = 0.0, gg = 0.0, gjg = 0.0;
@@ -623,13 +630,13 @@ __device__ inline double b_dogleg(const Member& M, double error, double damp, co
     }
     for (int64_t e = threadIdx.x; e < n; e += 64) ny[e] = ca * cy[e] + cb * gv[e];
     __syncthreads();
-    const UpdOut u = b_update(M, cx, ny, gv, damp, false, o.symm, o.factors_norm, nx, M.vec[V_F]);
-    error = sqrt(b_err2<L>(M)) / M.Ssize;
+    const UpdOut u = b_update(M, P, cx, ny, gv, damp, false, o.symm, o.factors_norm, nx, P.vec[V_F]);
+    error = sqrt(b_err2<L>(M, P)) / M.Ssize;
     gr = gain_ratio(old_error, error, M.Ssize, u.gain);
     if (error < old_error) {
       vcopy(bx, nx, n);
       vcopy(by, ny, n);
-      vcopy(bF, M.vec[V_F], n);
+      vcopy(bF, P.vec[V_F], n);
       best_error = error;
     }
     const double step = sqrt(u.dx2);
@@ -647,10 +654,10 @@ __device__ inline double b_dogleg(const Member& M, double error, double damp, co
     ny = ty;
     if (++count > 10) break;
   }
-  vcopy(M.vec[V_F], bF, n);
+  vcopy(P.vec[V_F], bF, n);
   vcopy(xout, bx, n);
-  vcopy(M.vec[V_Y], by, n);
-  b_stage_fs(M, M.vec[V_F]);
+  vcopy(P.vec[V_Y], by, n);
+  b_stage_fs(M, P, P.vec[V_F]);
   return best_error;
 }
 
@@ -704,14 +711,6 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
     int k[ldscg::MAXL];
     for (int l = 0; l < ldscg::MAXL; ++l) k[l] = l < L ? kdims[mb * 4 + l] : 1;
     ldscg::make_geom(L, R, k, &Ms.g);
-    Ms.s = ldscg::carve(Ms.g, bsm);
-    double* tail = bsm + ldscg::lds_doubles(Ms.g);
-    Ms.gam = tail;
-    Ms.cn = tail + 64;
-    Ms.herr = tail + 128;
-    Ms.himp = Ms.herr + o.maxiter;
-    Ms.S = Sc + md[BM_S];
-    for (int v = 0; v < NV; ++v) Ms.vec[v] = work + md[BM_X] + (int64_t)v * md[BM_NX];
     Ms.st[L - 1] = 1;
     for (int q = L - 2; q >= 0; --q) Ms.st[q] = Ms.st[q + 1] * k[q + 1];
     Ms.ns = (int)(Ms.st[0] * k[0]);
@@ -719,6 +718,21 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
   }
   __syncthreads();
   Member& M = Ms;
+  MemP P;
+  P.s = ldscg::carve(M.g, bsm);
+  {
+    double* tail = bsm + ldscg::lds_doubles(M.g);
+    P.gam = tail;
+    P.cn = tail + 64;
+    P.herr = tail + 128;
+    P.himp = P.herr + o.maxiter;
+  }
+  P.S = Sc + md[BM_S];
+  {
+    const int64_t x0 = md[BM_X], nx = md[BM_NX];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) P.vec[v] = work + x0 + (int64_t)v * nx;
+  }
   double* H = hist + md[BM_H];
   const int maxiter = o.maxiter;
   double damp, error, best_error;
@@ -726,7 +740,7 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
   if (it_begin == 0) {
     double s2 = 0.0, sa = 0.0;
     for (int e = lane; e < M.ns; e += 64) {
-      const double v = M.S[e];
+      const double v = P.S[e];
       s2 += v * v;
       sa += fabs(v);
     }
@@ -742,8 +756,8 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
     stop = 5;
     xi = 0;
     dogleg = 0;
-    vcopy(M.vec[V_F], M.vec[V_X0], M.g.n);
-    vcopy(M.vec[V_BEST], M.vec[V_X0], M.g.n);
+    vcopy(P.vec[V_F], P.vec[V_X0], M.g.n);
+    vcopy(P.vec[V_BEST], P.vec[V_X0], M.g.n);
     if (lane == 0) {
       rs[RS_SSIZE] = M.Ssize;
       rs[RS_SMEAN] = sa / M.ns;
@@ -758,12 +772,12 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
     dogleg = (int)rs[RS_DOGLEG];
   }
   for (int i = lane; i < it; i += 64) {
-    M.herr[i] = H[i];
-    M.himp[i] = H[2 * maxiter + i];
+    P.herr[i] = H[i];
+    P.himp[i] = H[2 * maxiter + i];
   }
-  ldscg::zero_lds<64>(M.g, M.s);
+  ldscg::zero_lds<64>(M.g, P.s);
   __syncthreads();
-  b_stage_fs(M, M.vec[V_F]);
+  b_stage_fs(M, P, P.vec[V_F]);
   double tol = o.tol, tol_step = o.tol_step, tol_improv = o.tol_improv, tol_grad = o.tol_grad;
   if (M.g.n <= 100) tol = tol_step = tol_improv = tol_grad = 0.0;
   const int cst = 1 + maxiter / 10;
@@ -772,33 +786,33 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
   for (; it < it_end && it < maxiter; ++it) {
     const double old_error = error;
     const int cap = caps[(int64_t)mb * maxiter + it];
-    b_gramians<L>(M);
-    b_grad<L>(M);
+    b_gramians<L>(M, P);
+    b_grad<L>(M, P);
     int itn;
     double rn;
-    ldscg::cg_body<1, 4, 1>(M.g, M.s, M.gam, damp, M.vec[V_G], cap, o.cg_tol, M.vec[V_Y],
-                            M.vec[V_JG], red, &itn, &rn);
+    ldscg::cg_body<1, 4, 1>(M.g, P.s, P.gam, damp, P.vec[V_G], cap, o.cg_tol, P.vec[V_Y],
+                            P.vec[V_JG], red, &itn, &rn);
     __syncthreads();
-    double* x = M.vec[V_X0 + xi];
-    double* xn = M.vec[V_X0 + 1 - xi];
-    const UpdOut u = b_update(M, x, M.vec[V_Y], M.vec[V_G], damp, true, o.symm, o.factors_norm,
-                              xn, M.vec[V_F]);
-    error = sqrt(b_err2<L>(M)) / M.Ssize;
+    double* x = xi ? P.vec[V_X0 + 1] : P.vec[V_X0];
+    double* xn = xi ? P.vec[V_X0] : P.vec[V_X0 + 1];
+    const UpdOut u = b_update(M, P, x, P.vec[V_Y], P.vec[V_G], damp, true, o.symm, o.factors_norm,
+                              xn, P.vec[V_F]);
+    error = sqrt(b_err2<L>(M, P)) / M.Ssize;
     xi = 1 - xi;
     double gain_den = u.gain, dx2 = u.dx2;
     if (it > 3 && error > o.tol_jump * old_error) {
       ++dogleg;
-      double* xc = M.vec[V_X0 + xi];
-      const double* ox = M.vec[V_X0 + 1 - xi];
-      for (int64_t e = lane; e < M.g.n; e += 64) M.vec[V_T1][e] = xc[e] - M.vec[V_Y][e];
+      double* xc = xi ? P.vec[V_X0 + 1] : P.vec[V_X0];
+      const double* ox = xi ? P.vec[V_X0] : P.vec[V_X0 + 1];
+      for (int64_t e = lane; e < M.g.n; e += 64) P.vec[V_T1][e] = xc[e] - P.vec[V_Y][e];
       __syncthreads();
-      error = b_dogleg<L>(M, error, damp, o, xc);
+      error = b_dogleg<L>(M, P, error, damp, o, xc);
       double gd = 0.0, d2 = 0.0;
       for (int64_t e = lane; e < M.g.n; e += 64) {
         const double d = xc[e] - ox[e];
         const double v = ox[e] + d;
         const double dy = v - ox[e];
-        gd += dy * (M.vec[V_G][e] + damp * dy);
+        gd += dy * (P.vec[V_G][e] + damp * dy);
         d2 += dy * dy;
       }
       gain_den = wave_sum(gd);
@@ -809,13 +823,13 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
     else if (gr > 0.75) damp = damp / 3;
     if (error < best_error) {
       best_error = error;
-      vcopy(M.vec[V_BEST], M.vec[V_F], M.g.n);
+      vcopy(P.vec[V_BEST], P.vec[V_F], M.g.n);
     }
     const double step = sqrt(dx2) / sqrt(u.x2);
-    const double imp = it == 0 ? error : fabs(error - M.herr[it - 1]);
+    const double imp = it == 0 ? error : fabs(error - P.herr[it - 1]);
     if (lane == 0) {
-      M.herr[it] = error;
-      M.himp[it] = imp;
+      P.herr[it] = error;
+      P.himp[it] = imp;
       H[it] = error;
       H[maxiter + it] = step;
       H[2 * maxiter + it] = imp;
@@ -829,12 +843,12 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
       else if (imp < tol_improv) st = 2;
       else if (u.gmax < tol_grad) st = 3;
       else if (it > 2 * cst && it % cst == 0) {
-        const double mean1 = np_mean(M.herr + it - 2 * cst, cst);
-        const double mean2 = np_mean(M.herr + it - cst, cst);
+        const double mean1 = np_mean(P.herr + it - 2 * cst, cst);
+        const double mean2 = np_mean(P.herr + it - cst, cst);
         if (mean1 - mean2 <= tol_improv) {
           st = 4;
         } else {
-          const double mean3 = np_mean(M.himp + it - cst, cst);
+          const double mean3 = np_mean(P.himp + it - cst, cst);
           if (mean3 < 1e-3 * mean2) st = 7;
         }
       }

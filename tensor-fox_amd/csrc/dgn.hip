@@ -257,31 +257,19 @@ __global__ __launch_bounds__(SMALL_NT) void k_cg_small(
     Shape s, TileMap tm, const S* __restrict__ Fg, const S* __restrict__ P1g,
     const S* __restrict__ P2, const S* __restrict__ gamma, double damp,
     const S* __restrict__ grad, int maxiter, double tol, S* __restrict__ y, S* __restrict__ jtjg,
-    double* __restrict__ scal, S* gws, unsigned lds_mask) {
+    double* __restrict__ scal, S* gws) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[SMALL_NT / 64];
   const int L = s.L, R = s.R;
   const int64_t n = s.n;
   const int64_t LRR = (int64_t)L * R * R;
-  // carve q, F, tmp, P1 into LDS (bit set) or global workspace
-  S* lp = (S*)smem;
-  S* gp = gws;
-  auto carve = [&](int bit, int64_t len) -> S* {
-    S* r;
-    if ((lds_mask >> bit) & 1u) {
-      r = lp;
-      lp += len;
-    } else {
-      r = gp;
-      gp += len;
-    }
-    return r;
-  };
-  S* q = carve(0, n);
-  S* F = carve(1, n);
-  S* tmp = carve(2, LRR);
-  S* P1 = carve(3, LRR);
-  S* ycg = gp;            // CG iterate, flat layout, always in global memory (fewer registers)
+  // q, F, tmp, P1 in LDS (carved from the dynamic allocation, so that their accesses compile to
+  // LDS instructions)
+  S* q = (S*)smem;
+  S* F = q + n;
+  S* tmp = F + n;
+  S* P1 = tmp + LRR;
+  S* ycg = gws;           // CG iterate, flat layout, in global memory (fewer registers)
   const int t = threadIdx.x;
   // stage F (row-major per mode) and P1
   for (int64_t e = t; e < n; e += blockDim.x) {
@@ -587,7 +575,7 @@ int cg_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P2, con
       TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_small<S>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       hipLaunchKernelGGL(k_cg_small<S>, dim3(1), dim3(SMALL_NT), lds, st, s, tm, F, P1, P2, gamma,
-                         damp, grad, maxiter, tol, y, jtjg, scal, (S*)wsp, 0xfu);
+                         damp, grad, maxiter, tol, y, jtjg, scal, (S*)wsp);
       TFX_CHECK_LAUNCH(ctx);
       return TFX_OK;
     }

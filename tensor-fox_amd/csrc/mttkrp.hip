@@ -32,12 +32,15 @@ __device__ inline void row_to_idx(const Shape& s, int64_t rho, int64_t* idx) {
   }
 }
 
-// Pass 1: Y (NR x R, row-major, columns rc0..rc0+RC-1) and per-workgroup partials of N_{L-1}.
-template <typename TS, typename S>
+// Pass 1: Y (NR x R, row-major, columns rc0..rc0+RC-1) and per-workgroup partials of N_{L-1}
+// (accumulated in LDS when NP_LDS, a compile-time choice so that those accesses compile to LDS
+// instructions, else in the workgroup's slice of `part`).
+template <typename TS, typename S, bool NP_LDS>
 __global__ __launch_bounds__(256) void k_mttkrp_rows(const TS* __restrict__ T, Shape s,
                                                       const S* __restrict__ F, int rc0,
                                                       int64_t rows_per_wg, S* __restrict__ Y,
-                                                      S* __restrict__ part, int np_in_lds) {
+                                                      S* __restrict__ part) {
+  constexpr bool np_in_lds = NP_LDS;
   constexpr int RC = RCOf<S>::v;
   constexpr int JN = RC / 8;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -334,13 +337,18 @@ int mttkrp_native(tfx_ctx* ctx, const TS* T, const Shape& s, const S* F, unsigne
 
   size_t lds_base = sizeof(S) * (RB * (KC + 1) + KC * RC + RB * RC);
   size_t lds_np = sizeof(S) * (size_t)K * RC;
-  int np_in_lds = (lds_base + lds_np) <= 150 * 1024;
+  const bool np_in_lds = (lds_base + lds_np) <= 150 * 1024;
   size_t lds = lds_base + (np_in_lds ? lds_np : 0);
-  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_mttkrp_rows<TS, S>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const void* kfn = np_in_lds ? (const void*)k_mttkrp_rows<TS, S, true>
+                              : (const void*)k_mttkrp_rows<TS, S, false>;
+  TFX_HIP(ctx, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   for (int rc0 = 0; rc0 < R; rc0 += RC) {
-    hipLaunchKernelGGL((k_mttkrp_rows<TS, S>), dim3(nwg), dim3(256), lds, stream, T, s, F, rc0,
-                       rows_per_wg, Y, part, np_in_lds);
+    if (np_in_lds)
+      hipLaunchKernelGGL((k_mttkrp_rows<TS, S, true>), dim3(nwg), dim3(256), lds, stream, T, s, F,
+                         rc0, rows_per_wg, Y, part);
+    else
+      hipLaunchKernelGGL((k_mttkrp_rows<TS, S, false>), dim3(nwg), dim3(256), lds, stream, T, s, F,
+                         rc0, rows_per_wg, Y, part);
     TFX_CHECK_LAUNCH(ctx);
     if ((mask >> (L - 1)) & 1u) {
       hipLaunchKernelGGL(k_mttkrp_reduce_last<S>, dim3((unsigned)K), dim3(256), 0, stream, s, rc0,
