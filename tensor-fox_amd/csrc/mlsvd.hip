@@ -200,14 +200,20 @@ __global__ __launch_bounds__(1024) void k_orth(double* Z, int n, int p) {
 // projected twice against all finished columns (each finished column is read once per pass:
 // H = Q^T Z_P, then Z_P -= Q H), then orthonormalized within itself by CGS2 on a single
 // wavefront (each lane owns fixed rows of the panel, inner products are wave reductions, so no
-// workgroup barrier is needed there), and written back.  The in-panel CGS2 costs p * OPB
-// sequential wave reductions and the projections one pass over the finished columns per panel;
-// on 400 x 60 blocks panels of 4 measured fastest (8: 590 µs, 4: 450 µs, 2: 440 µs per call).
+// workgroup barrier is needed there), and written back.  When projections and in-panel step
+// together remove all but a fraction 1e-3 of a column's norm (the block is numerically rank
+// deficient there), what is left of that column carries rounding components along the finished
+// columns that normalization would magnify: such a panel gets one more projection against the
+// finished columns and one more in-panel pass ("twice is enough": Parlett, The Symmetric
+// Eigenvalue Problem, sec. 6-9).  The in-panel CGS2 costs p * OPB sequential wave reductions and the
+// projections one pass over the finished columns per panel; on 400 x 60 blocks panels of 4
+// measured fastest (8: 590 µs, 4: 450 µs, 2: 440 µs per call).
 constexpr int OPB = 4;
 constexpr int ORTH_MAXN = 2048;   // n * OPB doubles of LDS
 
 __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int redo_s;
   double* zp = (double*)smem;                 // [OPB][n]
   double* H = zp + (size_t)OPB * n;           // [p][OPB]
   const int t = threadIdx.x, nt = blockDim.x;
@@ -216,85 +222,104 @@ __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
     const int pb = p - j0 < OPB ? p - j0 : OPB;
     for (int u = t; u < pb * n; u += nt) zp[u] = Z[(int64_t)j0 * n + u];
     __syncthreads();
-    for (int pass = 0; pass < 2 && j0 > 0; ++pass) {
-      // H[i][c] = Q_i . zp_c: one wavefront per finished column i
-      for (int i = wave; i < j0; i += nwave) {
-        const double* qi = Z + (int64_t)i * n;
-        double acc[OPB];
-#pragma unroll
-        for (int c = 0; c < OPB; ++c) acc[c] = 0.0;
-#pragma unroll 4
-        for (int e = lane; e < n; e += 64) {
-          const double q = qi[e];
-#pragma unroll
-          for (int c = 0; c < OPB; ++c)
-            if (c < pb) acc[c] += q * zp[c * n + e];
-        }
+    for (int round = 0; round < 2; ++round) {
+      double pre[OPB];   // wave 0: squared column norms before this round's projections
+      if (wave == 0) {
 #pragma unroll
         for (int c = 0; c < OPB; ++c) {
-          const double v = wave_sum(acc[c]);
-          if (lane == 0 && c < pb) H[i * OPB + c] = v;
+          double a = 0.0;
+          if (c < pb)
+            for (int e = lane; e < n; e += 64) a += zp[c * n + e] * zp[c * n + e];
+          pre[c] = wave_sum(a);
         }
       }
-      __syncthreads();
-      // zp_c -= sum_i Q_i H[i][c] (eight finished columns' loads in flight at a time)
-      for (int e = t; e < n; e += nt) {
-        double acc[OPB];
+      for (int pass = 0; pass < 2 - round && j0 > 0; ++pass) {
+        // H[i][c] = Q_i . zp_c: one wavefront per finished column i
+        for (int i = wave; i < j0; i += nwave) {
+          const double* qi = Z + (int64_t)i * n;
+          double acc[OPB];
 #pragma unroll
-        for (int c = 0; c < OPB; ++c) acc[c] = c < pb ? zp[c * n + e] : 0.0;
-        int i = 0;
-        for (; i + 8 <= j0; i += 8) {
-          double q[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) q[u] = Z[(int64_t)(i + u) * n + e];
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-#pragma unroll
-            for (int c = 0; c < OPB; ++c) acc[c] -= q[u] * H[(i + u) * OPB + c];
-        }
-        for (; i < j0; ++i) {
-          const double q = Z[(int64_t)i * n + e];
-#pragma unroll
-          for (int c = 0; c < OPB; ++c) acc[c] -= q * H[i * OPB + c];
-        }
-#pragma unroll
-        for (int c = 0; c < OPB; ++c)
-          if (c < pb) zp[c * n + e] = acc[c];
-      }
-      __syncthreads();
-    }
-    // CGS2 inside the panel on wavefront 0 (lane-owned rows e = lane + 64 u)
-    if (wave == 0) {
-      for (int c = 0; c < pb; ++c) {
-        double* zc = zp + c * n;
-        for (int pass = 0; pass < 2 && c > 0; ++pass) {
-          double d[OPB];
-#pragma unroll
-          for (int i = 0; i < OPB; ++i) d[i] = 0.0;
+          for (int c = 0; c < OPB; ++c) acc[c] = 0.0;
+#pragma unroll 4
           for (int e = lane; e < n; e += 64) {
-            const double v = zc[e];
+            const double q = qi[e];
 #pragma unroll
-            for (int i = 0; i < OPB; ++i)
-              if (i < c) d[i] += zp[i * n + e] * v;
+            for (int c = 0; c < OPB; ++c)
+              if (c < pb) acc[c] += q * zp[c * n + e];
           }
 #pragma unroll
-          for (int i = 0; i < OPB; ++i) d[i] = i < c ? wave_sum(d[i]) : 0.0;
-          for (int e = lane; e < n; e += 64) {
-            double v = zc[e];
-#pragma unroll
-            for (int i = 0; i < OPB; ++i)
-              if (i < c) v -= d[i] * zp[i * n + e];
-            zc[e] = v;
+          for (int c = 0; c < OPB; ++c) {
+            const double v = wave_sum(acc[c]);
+            if (lane == 0 && c < pb) H[i * OPB + c] = v;
           }
         }
-        double a = 0.0;
-        for (int e = lane; e < n; e += 64) a += zc[e] * zc[e];
-        a = wave_sum(a);
-        const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
-        for (int e = lane; e < n; e += 64) zc[e] *= inv;
+        __syncthreads();
+        // zp_c -= sum_i Q_i H[i][c] (eight finished columns' loads in flight at a time)
+        for (int e = t; e < n; e += nt) {
+          double acc[OPB];
+#pragma unroll
+          for (int c = 0; c < OPB; ++c) acc[c] = c < pb ? zp[c * n + e] : 0.0;
+          int i = 0;
+          for (; i + 8 <= j0; i += 8) {
+            double q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = Z[(int64_t)(i + u) * n + e];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+              for (int c = 0; c < OPB; ++c) acc[c] -= q[u] * H[(i + u) * OPB + c];
+          }
+          for (; i < j0; ++i) {
+            const double q = Z[(int64_t)i * n + e];
+#pragma unroll
+            for (int c = 0; c < OPB; ++c) acc[c] -= q * H[i * OPB + c];
+          }
+#pragma unroll
+          for (int c = 0; c < OPB; ++c)
+            if (c < pb) zp[c * n + e] = acc[c];
+        }
+        __syncthreads();
       }
+      // CGS2 inside the panel on wavefront 0 (lane-owned rows e = lane + 64 u)
+      if (wave == 0) {
+        bool redo = false;
+#pragma unroll
+        for (int c = 0; c < OPB; ++c) {
+          if (c >= pb) break;
+          double* zc = zp + c * n;
+          for (int pass = 0; pass < 2 && c > 0; ++pass) {
+            double d[OPB];
+#pragma unroll
+            for (int i = 0; i < OPB; ++i) d[i] = 0.0;
+            for (int e = lane; e < n; e += 64) {
+              const double v = zc[e];
+#pragma unroll
+              for (int i = 0; i < OPB; ++i)
+                if (i < c) d[i] += zp[i * n + e] * v;
+            }
+#pragma unroll
+            for (int i = 0; i < OPB; ++i) d[i] = i < c ? wave_sum(d[i]) : 0.0;
+            for (int e = lane; e < n; e += 64) {
+              double v = zc[e];
+#pragma unroll
+              for (int i = 0; i < OPB; ++i)
+                if (i < c) v -= d[i] * zp[i * n + e];
+              zc[e] = v;
+            }
+          }
+          double a = 0.0;
+          for (int e = lane; e < n; e += 64) a += zc[e] * zc[e];
+          a = wave_sum(a);
+          if (a < 1e-6 * pre[c]) redo = true;
+          const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+          for (int e = lane; e < n; e += 64) zc[e] *= inv;
+        }
+        if (lane == 0) redo_s = redo && round == 0 && j0 > 0;
+      }
+      __syncthreads();
+      if (!redo_s) break;
+      __syncthreads();   // every thread has read redo_s before wave 0 may write it again
     }
-    __syncthreads();
     for (int u = t; u < pb * n; u += nt) Z[(int64_t)j0 * n + u] = zp[u];
     __syncthreads();
   }
@@ -532,19 +557,20 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
   if (direct) {
     TFX_HIP(ctx, hipMemcpyAsync(H, G, sizeof(double) * n * n, hipMemcpyDeviceToDevice, stream));
   } else {
-    // subspace iteration: Q <- orth(G Q), `iters` times, from Q0
+    // range finder Q = orth(G^iters Q0) with plain power steps, as the reference's randomized
+    // range finder (Compression.py:386-395: un-normalised power iterations, then one QR), except
+    // that the block is orthonormalized after every second product as well: direction i then
+    // keeps a relative precision of about eps (lambda_1 / lambda_i)^2 in FP64 whatever `iters`
+    // (the reference's products on the unfolding give (sigma_1 / sigma_i)^(2 iters + 1)).
     TFX_HIP(ctx, hipMemcpyAsync(Q, Q0, sizeof(double) * n * p, hipMemcpyDeviceToDevice, stream));
-    // subspace iteration Q <- orth(G Q): orthonormalizing after every product keeps the weak
-    // directions of the block at full relative precision (a chain of plain power steps would
-    // scale them by (lambda_k / lambda_1)^iters before the single QR)
     for (int it = 0; it < iters; ++it) {
       gemm(0, n, p, n, G, Q, Z);
-      orth(Z);
       double* t = Q;
       Q = Z;
       Z = t;
+      if (it % 2 == 1 && it + 1 < iters) orth(Q);
     }
-    if (iters == 0) orth(Q);
+    orth(Q);
     // Rayleigh-Ritz: H = Q^T (G Q)
     gemm(0, n, p, n, G, Q, Z);
     gemm(1, p, p, n, Q, Z, H);
