@@ -450,6 +450,9 @@ __global__ void k_stats_final(const double* __restrict__ part, int nb, double* o
 int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
                        unsigned mask, double* out, hipStream_t stream);
 
+int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const double* F,
+                       unsigned mask, double* out, hipStream_t stream);
+
 int tfx_mttkrp_native_fd(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
                          unsigned mask, double* out, hipStream_t stream) {
   return mttkrp_native<float, double>(ctx, T, s, F, mask, out, stream);
@@ -475,9 +478,17 @@ extern "C" int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L,
   if (t_dtype == TFX_F32 && f_dtype == TFX_F64)
     return mttkrp_native<float, double>(ctx, (const float*)T, s, (const double*)F_flat, mode_mask,
                                         (double*)out_flat, stream);
-  if (t_dtype == TFX_F64 && f_dtype == TFX_F64)
+  if (t_dtype == TFX_F64 && f_dtype == TFX_F64) {
+    // FP64 matrix cores; the FMA form takes unfoldings with 2^31 columns or more
+    bool fits = true;
+    for (int l = 0; l < L; ++l)
+      if (((mode_mask >> l) & 1u) && s.N / s.d[l] > 0x7fffffff) fits = false;
+    if (fits)
+      return tfx_mttkrp_f64mfma(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
+                                (double*)out_flat, stream);
     return mttkrp_native<double, double>(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
                                          (double*)out_flat, stream);
+  }
   return tfx_set_error(ctx, TFX_E_ARG, "unsupported dtype pair %d/%d", t_dtype, f_dtype);
 }
 

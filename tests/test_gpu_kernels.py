@@ -44,7 +44,8 @@ def _rel(a, b):
 
 
 MTTKRP_CASES = [((5, 6, 7), 3), ((20, 30, 40), 10), ((4, 5, 6, 7), 5), ((33, 17, 70), 70),
-                ((2, 2, 2), 3), ((50, 50, 50), 50), ((7, 3), 4), ((3, 4, 2, 3, 2), 2)]
+                ((2, 2, 2), 3), ((50, 50, 50), 50), ((7, 3), 4), ((3, 4, 2, 3, 2), 2),
+                ((2, 3, 2, 2, 3, 2, 2, 2), 3), ((40, 31, 20), 130), ((60, 700), 17)]
 
 
 @pytest.mark.parametrize('dt', [torch.float64, torch.float32])
