@@ -4,13 +4,16 @@
 //
 //   N_l[i, r] = sum_c T_(l)[i, c] KR_l[c, r],   KR_l[c, r] = prod_{m != l} F_m[i_m(c), r]
 //
-// c enumerates the indices of the modes other than l in C order (last mode fastest).  A job is
-// one wavefront: rows i0..i0+15 of mode l, the 64 ranks rc0..rc0+63 of the launch (four 16 x 16
+// c enumerates the indices of the modes other than l in C order (last mode fastest).  A wavefront
+// owns rows i0..i0+15 of mode l, the 64 ranks rc0..rc0+63 of the launch (four 16 x 16
 // accumulators in registers) and a slice of c.  Per MFMA step (4 values of c) a lane loads one
 // entry of T (the A operand: row i0 + lane % 16, c = c0 + lane / 16) and forms its B operand
 // entries KR_l[c, r] for the four rank blocks from factor entries (cached: the factors are
-// small next to T), so T is read once per mode and rank chunk.  Slices are summed in a fixed
-// order by k_m64_sum, so results do not depend on scheduling.
+// small next to T), so T is read once per mode and rank chunk.  The four wavefronts of a
+// workgroup take four consecutive slices of the same rows and add their tiles in LDS (in wave
+// order); the workgroup tiles are summed in a fixed order by k_m64_sum, so results do not depend
+// on scheduling.  Slices are short (many wavefronts) so that load latency is covered by other
+// wavefronts rather than by prefetch registers.
 //
 // Fragment layouts of v_mfma_f64_16x16x4f64 (lane l): A[l % 16][l / 16], B[l / 16][l % 16],
 // result register j = D[4 j + l / 16][l % 16].
@@ -23,18 +26,18 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int M64_NT = 256;      // 4 jobs per workgroup
+constexpr int M64_NT = 256;      // 4 wavefronts (slices) per workgroup
 constexpr int M64_RC = 64;       // ranks per launch
-constexpr int M64_MINSLICE = 64; // c per job at least (16 MFMA steps)
+constexpr int M64_MINSLICE = 16; // c per wavefront at least (4 MFMA steps)
 constexpr int M64_PART = 16 * M64_RC;
 
 struct M64Plan {
   Shape s;
   int rc0;
   int nsel;                      // selected modes
-  int modes[TFX_MAXL];           // selected modes in job order
-  int first[TFX_MAXL + 1];       // first job of each selected mode
-  int nsp[TFX_MAXL];             // c slices per row block
+  int modes[TFX_MAXL];           // selected modes in workgroup order
+  int first[TFX_MAXL + 1];       // first workgroup of each selected mode
+  int ngr[TFX_MAXL];             // slice groups (workgroups) per row block
   int64_t slice[TFX_MAXL];       // c per slice (multiple of 16)
   int64_t stride[TFX_MAXL];      // C-order strides of T
 };
@@ -43,14 +46,15 @@ template <int L>
 __global__ __launch_bounds__(M64_NT) void k_mttkrp_f64(const double* __restrict__ T, M64Plan P,
                                                        const double* __restrict__ F,
                                                        double* __restrict__ part) {
+  __shared__ f64x4 red[3][4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  const int job = blockIdx.x * (M64_NT / 64) + wave;
-  if (job >= P.first[P.nsel]) return;   // whole wavefronts; the kernel has no barrier
+  const int wg = blockIdx.x;
   int ms = 0;
-  while (ms + 1 < P.nsel && job >= P.first[ms + 1]) ++ms;
+  while (ms + 1 < P.nsel && wg >= P.first[ms + 1]) ++ms;
   const int l = P.modes[ms];
-  const int local = job - P.first[ms];
-  const int ib = local / P.nsp[ms], sp = local - (local / P.nsp[ms]) * P.nsp[ms];
+  const int local = wg - P.first[ms];
+  const int ib = local / P.ngr[ms], grp = local - (local / P.ngr[ms]) * P.ngr[ms];
+  const int sp = 4 * grp + wave;
   const Shape& s = P.s;
   const int64_t Il = s.d[l];
   const int64_t Kl = s.N / Il;
@@ -112,12 +116,24 @@ __global__ __launch_bounds__(M64_NT) void k_mttkrp_f64(const double* __restrict_
       for (int b = 0; b < 4; ++b)
         acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u][b], acc[b], 0, 0, 0);
   }
-  // partial tile: part[job][row][col], row = 4 j + lane / 16, col = 16 b + lane % 16
-  double* pj = part + (int64_t)job * M64_PART;
+  // workgroup tile = wave 0 + wave 1 + wave 2 + wave 3, then part[wg][row][col] with
+  // row = 4 j + lane / 16, col = 16 b + lane % 16
+  if (wave > 0) {
 #pragma unroll
-  for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < 4; ++b) red[wave - 1][b][lane] = acc[b];
+  }
+  __syncthreads();
+  if (wave == 0) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) pj[(4 * j + lk) * M64_RC + 16 * b + li] = acc[b][j];
+    for (int w = 0; w < 3; ++w)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[b] += red[w][b][lane];
+    double* pj = part + (int64_t)wg * M64_PART;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) pj[(4 * j + lk) * M64_RC + 16 * b + li] = acc[b][j];
+  }
 }
 
 // out[off_l + r I_l + i] = sum over the slices (in order) of the partial tiles; one thread per
@@ -139,10 +155,18 @@ __global__ __launch_bounds__(256) void k_m64_sum(M64Plan P, const double* __rest
   const int r = P.rc0 + rr;
   if (r >= s.R) return;
   const int ib = (int)(i / 16), row = (int)(i - (int64_t)ib * 16);
-  const int nsp = P.nsp[ms];
-  const double* src = part + ((int64_t)(P.first[ms] + ib * nsp) * 16 + row) * M64_RC + rr;
+  const int ng = P.ngr[ms];
+  const double* src = part + ((int64_t)(P.first[ms] + ib * ng) * 16 + row) * M64_RC + rr;
   double acc = 0.0;
-  for (int sp = 0; sp < nsp; ++sp) acc += src[(int64_t)sp * M64_PART];
+  int g = 0;
+  for (; g + 8 <= ng; g += 8) {   // eight loads in flight, added in order
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)(g + u) * M64_PART];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; g < ng; ++g) acc += src[(int64_t)g * M64_PART];
   out[s.off[l] + (int64_t)r * s.d[l] + i] = acc;
 }
 
@@ -172,9 +196,10 @@ int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const doub
     rowblocks += (s.d[l] + 15) / 16;
   }
   if (P.nsel == 0) return TFX_OK;
-  // slices: about 4 jobs per CU over all modes, at least M64_MINSLICE values of c per job
-  const int64_t target = 4 * (int64_t)ctx->cu_count;
-  int njobs = 0;
+  // slices: about 16 wavefronts per CU over all modes, at least M64_MINSLICE values of c each,
+  // in groups of four (one workgroup); the last slices of a row block may be empty
+  const int64_t target = 16 * (int64_t)ctx->cu_count;
+  int nwg = 0;
   for (int m = 0; m < P.nsel; ++m) {
     const int l = P.modes[m];
     const int64_t Kl = s.N / s.d[l];
@@ -184,17 +209,17 @@ int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const doub
     int64_t slice = (Kl + nsp - 1) / nsp;
     slice = (slice + 15) & ~int64_t(15);
     nsp = (Kl + slice - 1) / slice;
-    P.first[m] = njobs;
-    P.nsp[m] = (int)nsp;
+    const int64_t ng = (nsp + 3) / 4;
+    P.first[m] = nwg;
+    P.ngr[m] = (int)ng;
     P.slice[m] = slice;
-    njobs += (int)(nib * nsp);
+    nwg += (int)(nib * ng);
   }
-  P.first[P.nsel] = njobs;
+  P.first[P.nsel] = nwg;
   void* wsp;
-  int rc = tfx_workspace(ctx, sizeof(double) * (size_t)njobs * M64_PART + 256, stream, &wsp);
+  int rc = tfx_workspace(ctx, sizeof(double) * (size_t)nwg * M64_PART + 256, stream, &wsp);
   if (rc) return rc;
   double* part = (double*)wsp;
-  const int nwg = (njobs + M64_NT / 64 - 1) / (M64_NT / 64);
   int64_t nout = 0;
   for (int m = 0; m < P.nsel; ++m) nout += s.d[P.modes[m]] * M64_RC;
   const unsigned nsum = (unsigned)((nout + 255) / 256);

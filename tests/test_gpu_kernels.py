@@ -86,6 +86,20 @@ def test_mttkrp_split16(dev, dims, R):
         assert _rel(got[l], ref) < 3e-5, (l, _rel(got[l], ref))
 
 
+def test_mttkrp_fp64_large(dev):
+    """FP64 tensor above 2^24 entries (the matrix-core form's slicing for large tensors)."""
+    dims, R = (260, 260, 250), 3
+    print('mttkrp fp64 large', dims, R, flush=True)
+    T, F = _case(dims, R, 5)
+    Td = dev.upload(T.ravel(), torch.float64)
+    Fd = dev.upload(_flat(F), torch.float64)
+    out = dev.zeros(R * sum(dims), torch.float64)
+    dev.mttkrp(Td, dims, Fd, R, out)
+    got = _unflat(out.cpu().numpy(), dims, R)
+    for l in range(len(dims)):
+        assert _rel(got[l], o.mttkrp(T, F, l)) < 1e-12
+
+
 def test_mttkrp_mode_mask(dev):
     dims, R = (6, 7, 8), 4
     T, F = _case(dims, R, 2)
