@@ -410,30 +410,56 @@ __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restric
   const int t = threadIdx.x, nt = blockDim.x;
   const int lane = t & 63, wave = t >> 6, nwave = nt >> 6;
   double gain = 0.0, dy2 = 0.0, x2 = 0.0, gmax = 0.0;
-  // one wave per factor column (l, r): the update, the step statistics and the column's norm
-  for (int c = wave; c < L * R; c += nwave) {
-    const int l = c / R, r = c - (c / R) * R;
-    const int64_t I = s.d[l];
-    const int64_t base = s.off[l] + (int64_t)r * I;
-    double ss = 0.0;
-    for (int64_t i = lane; i < I; i += 64) {
-      const int64_t e = base + i;
-      const double xo = (double)x[e];
-      const S dyS = (S)alpha * y[e];
-      const S v = x[e] + dyS;
-      xn[e] = v;
-      F[e] = v;
-      const double dy = (double)(v - x[e]);   // x_new - x_old as the reference forms it
-      gain += dy * ((double)g[e] + damp * dy);
-      dy2 += dy * dy;
-      x2 += xo * xo;
-      const double ag = fabs((double)g[e]);
-      gmax = ag > gmax ? ag : gmax;
-      ss += (double)v * (double)v;
+  // one wave per UB consecutive factor columns (l, r) at a time: the update, the step statistics
+  // and the columns' norms; the loads of the UB columns are in flight together
+  constexpr int UB = 4;
+  for (int cb = wave * UB; cb < L * R; cb += nwave * UB) {
+    int64_t base[UB], Ic[UB];
+    int64_t Imax = 0;
+#pragma unroll
+    for (int k = 0; k < UB; ++k) {
+      const int c = cb + k < L * R ? cb + k : L * R - 1;
+      const int l = c / R, r = c - (c / R) * R;
+      Ic[k] = cb + k < L * R ? s.d[l] : 0;
+      base[k] = s.off[l] + (int64_t)r * s.d[l];
+      Imax = Ic[k] > Imax ? Ic[k] : Imax;
+    }
+    double ss[UB];
+#pragma unroll
+    for (int k = 0; k < UB; ++k) ss[k] = 0.0;
+    for (int64_t i0 = 0; i0 < Imax; i0 += 64) {
+      S xv[UB], yv[UB], gv[UB];
+#pragma unroll
+      for (int k = 0; k < UB; ++k) {
+        const int64_t e = i0 + lane < Ic[k] ? base[k] + i0 + lane : 0;
+        xv[k] = x[e];
+        yv[k] = y[e];
+        gv[k] = g[e];
+      }
+#pragma unroll
+      for (int k = 0; k < UB; ++k) {
+        if (i0 + lane < Ic[k]) {
+          const int64_t e = base[k] + i0 + lane;
+          const double xo = (double)xv[k];
+          const S v = xv[k] + (S)alpha * yv[k];
+          xn[e] = v;
+          F[e] = v;
+          const double dy = (double)(v - xv[k]);   // x_new - x_old as the reference forms it
+          gain += dy * ((double)gv[k] + damp * dy);
+          dy2 += dy * dy;
+          x2 += xo * xo;
+          const double ag = fabs((double)gv[k]);
+          gmax = ag > gmax ? ag : gmax;
+          ss[k] += (double)v * (double)v;
+        }
+      }
     }
     if (eq) {
-      ss = wave_sum(ss);
-      if (lane == 0) cn[c] = sqrt(ss);
+#pragma unroll
+      for (int k = 0; k < UB; ++k) {
+        const double a = wave_sum(ss[k]);
+        if (lane == 0 && cb + k < L * R) cn[cb + k] = sqrt(a);
+      }
     }
   }
   gain = block_sum(gain, red);
@@ -458,13 +484,27 @@ __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restric
       sc[c] = prod != 0.0 ? pow(prod, 1.0 / L) / cn[l * R + r] : 0.0;
     }
     __syncthreads();
-    for (int c = wave; c < L * R; c += nwave) {
-      const double f = sc[c];
-      if (f == 0.0) continue;
-      const int l = c / R, r = c - (c / R) * R;
-      const int64_t I = s.d[l];
-      S* col = F + s.off[l] + (int64_t)r * I;
-      for (int64_t i = lane; i < I; i += 64) col[i] = (S)(f * (double)col[i]);
+    for (int cb = wave * UB; cb < L * R; cb += nwave * UB) {
+      int64_t base[UB], Ic[UB];
+      double f[UB];
+      int64_t Imax = 0;
+#pragma unroll
+      for (int k = 0; k < UB; ++k) {
+        const int c = cb + k < L * R ? cb + k : L * R - 1;
+        const int l = c / R, r = c - (c / R) * R;
+        f[k] = sc[c];
+        Ic[k] = (cb + k < L * R && f[k] != 0.0) ? s.d[l] : 0;
+        base[k] = s.off[l] + (int64_t)r * s.d[l];
+        Imax = Ic[k] > Imax ? Ic[k] : Imax;
+      }
+      for (int64_t i0 = 0; i0 < Imax; i0 += 64) {
+        S fv[UB];
+#pragma unroll
+        for (int k = 0; k < UB; ++k) fv[k] = F[i0 + lane < Ic[k] ? base[k] + i0 + lane : 0];
+#pragma unroll
+        for (int k = 0; k < UB; ++k)
+          if (i0 + lane < Ic[k]) F[base[k] + i0 + lane] = (S)(f[k] * (double)fv[k]);
+      }
     }
     __syncthreads();
   }
