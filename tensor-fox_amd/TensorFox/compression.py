@@ -25,12 +25,26 @@ def _pick_tol(tol_mlsvd, L):
 
 
 def clean_compression(sigma, tol_mlsvd, L):
-    """Smallest i with sum(sigma[i:]^2)/sum(sigma^2) < tol_mlsvd/L (Compression.py:261-301)."""
+    """Smallest i with sum(sigma[i:]^2)/sum(sigma^2) < tol_mlsvd/L (Compression.py:261-301).
+
+    The tail sums come from one cumulative sum; an index whose ratio lies within 1e-9 (relative)
+    of the threshold is decided with the reference's own np.sum expression, so the result is the
+    reference's (the two sums differ by rounding only, about 1e-14 relative)."""
     eps = tol_mlsvd / L
-    for i in range(1, len(sigma)):
-        if np.sum(sigma[i:] ** 2) / np.sum(sigma ** 2) < eps:
+    n = len(sigma)
+    if n <= 1:
+        return n
+    sq = sigma ** 2
+    total = np.sum(sq)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        approx = np.cumsum(sq[::-1])[::-1] / total
+    for i in range(1, n):
+        a = approx[i]
+        if a >= eps * (1 + 1e-9):
+            continue
+        if a < eps * (1 - 1e-9) or np.sum(sigma[i:] ** 2) / np.sum(sigma ** 2) < eps:
             return i
-    return len(sigma)
+    return n
 
 
 def _data_prec(X, prec):
@@ -74,12 +88,21 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
     svd_n = options.svd_n_components
     U_list, sig_list = [], []
     if method == 'seq':
+        rng = np.random.mtrand._rand
+
+        def draw_shape(l, cur_dims):
+            # the reference's sketch shape for mode l (Compression.py:208-230)
+            n_comp0 = min(R, dims[l]) if svd_n is None else svd_n
+            n_rest = int(np.prod(cur_dims)) // cur_dims[l]
+            transpose = cur_dims[l] < n_rest
+            return (cur_dims[l] if transpose else n_rest, n_comp0 + 10)
+
         S, S_dims = T, list(dims)
+        pending = rng.normal(size=draw_shape(0, S_dims))
         for l in range(L):
             n_comp = min(R, dims[l]) if svd_n is None else svd_n
             n_rest = int(np.prod(S_dims)) // S_dims[l]
-            transpose = S_dims[l] < n_rest
-            draw = np.random.mtrand._rand.normal(size=(S_dims[l] if transpose else n_rest, n_comp + 10))
+            draw = pending
             n_comp = min(n_comp, S_dims[l], n_rest)
             U, lam = _top_eigvecs(dev, S, S_dims, l, n_comp, prec, draw)
             new_dims = list(S_dims)
@@ -87,6 +110,13 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
             Snew = dev.empty(int(np.prod(new_dims)), wdt)
             sig2 = dev.empty(n_comp, torch.float64)
             dev.ttm(S, S_dims, l, U, n_comp, Snew, sig2, prec=_data_prec(S, prec))
+            # next mode's sketch, drawn while the device works, for the shape it has when this
+            # mode keeps n_comp components; the RNG state is restored and the sketch drawn again
+            # when the truncation below changes that shape (the draws stay the reference's)
+            if l + 1 < L:
+                spec_shape = draw_shape(l + 1, new_dims)
+                spec_state = rng.get_state()
+                pending = rng.normal(size=spec_shape)
             sigma = np.sqrt(np.maximum(sig2.cpu().numpy(), 0.0))
             dim = clean_compression(sigma, tol_mlsvd, L)
             if dim < n_comp:
@@ -98,14 +128,21 @@ def mlsvd(dev, T, dims, R, options, wdt, prec):
             U_list.append(U)
             sig_list.append(sigma)
             S, S_dims = Snew, new_dims
+            if l + 1 < L and draw_shape(l + 1, S_dims) != spec_shape:
+                rng.set_state(spec_state)
+                pending = rng.normal(size=draw_shape(l + 1, S_dims))
     elif method == 'classic':
+        # every mode's eigensolve is queued before any eigenvalue is read back (the sketches are
+        # drawn in the reference's order; their shapes do not depend on the results)
+        queued = []
         for l in range(L):
             n_comp = min(R, dims[l]) if svd_n is None else svd_n
             n_rest = int(np.prod(dims)) // dims[l]
             transpose = dims[l] < n_rest
             draw = np.random.mtrand._rand.normal(size=(dims[l] if transpose else n_rest, n_comp + 10))
             n_comp = min(n_comp, dims[l], n_rest)
-            U, lam = _top_eigvecs(dev, T, dims, l, n_comp, prec, draw)
+            queued.append(_top_eigvecs(dev, T, dims, l, n_comp, prec, draw))
+        for U, lam in queued:
             sigma = np.sqrt(np.maximum(lam.cpu().numpy(), 0.0))
             dim = clean_compression(sigma, tol_mlsvd, L)
             U_list.append(U[:dim].contiguous())
