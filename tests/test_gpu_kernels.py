@@ -134,7 +134,9 @@ def test_residual(dev, dims, R, dt):
                                     ((100, 130), 20),        # order 2
                                     ((13, 9, 11, 67), 64),   # order 4, R at the limit
                                     ((64, 48, 128), 1),      # rank 1
-                                    ((200, 150, 130), 50)])  # many workgroups
+                                    ((200, 150, 130), 50),   # many workgroups
+                                    ((300, 2, 50), 7),       # long mode before a tiny one
+                                    ((3, 500, 2, 40), 13)])  # rank padding near the end of F
 def test_residual_mfma_fp64_factors(dev, dims, R, tdt):
     """FP64 MFMA residual (FP64 factors, R <= 64, order 2..4) against the oracle's reconstruction."""
     print('residual mfma', dims, R, tdt, flush=True)
