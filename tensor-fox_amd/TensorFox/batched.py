@@ -25,6 +25,7 @@ dims <= 64, rank <= 16.  Anything else raises NotImplementedError.
 """
 
 import ctypes
+import gc
 import json
 import os
 import time
@@ -264,31 +265,58 @@ class BatchedCPD:
                                   self.S, self.W, self.work, self.hist, self.res, self.Ffull,
                                   timings)
         t0 = time.perf_counter()
+        # Factors and histories are views into the two host arrays this run copied back (each
+        # run gets fresh ones): factor l of a member is the (d_l, R) transpose of its flat
+        # column-major block, with the values the kernels wrote.  Per-member scalars are
+        # converted in bulk.
         results = []
         z = np.array([0])
+        steps = resh[:, RS_STEPS].astype(np.int64).tolist()
+        errs = (np.sqrt(resh[:, RS_E2]) / np.sqrt(resh[:, RS_T2])).tolist()
+        stops = resh[:, RS_STOP].astype(np.int64).tolist()
+        doglegs = resh[:, RS_DOGLEG].astype(np.int64).tolist()
+        kdl = kd[:, :L].astype(np.int64).tolist()
+        fofs = plan.meta[:, 11].astype(np.int64).tolist()
+        hofs = plan.meta[:, 12].astype(np.int64).tolist()
+        prec = self.precision
+        # the cyclic collector is paused while the ~5 nb objects are created (a full collection
+        # set off in the middle of the loop costs more than the loop), and restored after
+        gc_was_on = gc.isenabled()
+        gc.disable()
+        try:
+            results = self._assemble(results, z, steps, errs, stops, doglegs, kdl, fofs, hofs, prec,
+                                     Fh, hh, maxiter, timings)
+        finally:
+            if gc_was_on:
+                gc.enable()
+        timings['outputs'] = time.perf_counter() - t0
+        timings['total'] = time.perf_counter() - t_all
+        return results
+
+    def _assemble(self, results, z, steps, errs, stops, doglegs, kdl, fofs, hofs, prec, Fh, hh,
+                  maxiter, timings):
+        nb, L, options = self.nb, self.L, self.options
         for m in range(nb):
             R = self.ranks[m]
             d = self.dims_sorted[m]
-            fo = int(plan.meta[m, 11])
+            fo = fofs[m]
             Fm = []
             for l in range(L):
-                Fm.append(Fh[fo:fo + R * d[l]].reshape(R, d[l]).T.copy())
-                fo += R * d[l]
+                n = R * d[l]
+                Fm.append(Fh[fo:fo + n].reshape(R, d[l]).T)
+                fo += n
             Fm = unsort_dims(Fm, self.orderings[m])
-            r = resh[m]
-            k = int(r[RS_STEPS])
-            h0 = int(plan.meta[m, 12])
+            k = steps[m]
+            h0 = hofs[m]
             hm = hh[h0:h0 + 4 * maxiter].reshape(4, maxiter)[:, :k]
-            final_err = float(np.sqrt(r[RS_E2]) / np.sqrt(r[RS_T2]))
-            o = Output(k, final_err, [hm[1].copy(), z], [hm[0].copy(), z], [hm[2].copy(), z],
-                       [hm[3].copy(), z], [int(r[RS_STOP]), 8], options,
-                       timings=timings if m == 0 else {}, precision=self.precision)
+            final_err = errs[m]
+            o = Output(k, final_err, [hm[1], z], [hm[0], z], [hm[2], z], [hm[3], z],
+                       [stops[m], 8], options, timings=timings if m == 0 else {},
+                       precision=prec)
             o.final_rel_error = final_err
-            o.compressed_dims = tuple(int(v) for v in kd[m, :L])
-            o.dogleg_calls = int(r[RS_DOGLEG])
+            o.compressed_dims = tuple(kdl[m])
+            o.dogleg_calls = doglegs[m]
             results.append((Fm, o))
-        timings['outputs'] = time.perf_counter() - t0
-        timings['total'] = time.perf_counter() - t_all
         return results
 
 

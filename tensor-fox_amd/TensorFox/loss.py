@@ -19,8 +19,16 @@ from .device import get_device
 
 
 def _flat64(factors):
-    """concat_l vec(F_l) (column-major per factor), FP64 -- the kernels' factor layout."""
-    return torch.cat([f.detach().to(torch.float64).t().reshape(-1) for f in factors]).contiguous()
+    """concat_l vec(F_l) (column-major per factor), FP64 -- the kernels' factor layout (one
+    converting copy per factor into its slice)."""
+    n = sum(f.numel() for f in factors)
+    x = torch.empty(n, dtype=torch.float64, device=factors[0].device)
+    s = 0
+    for f in factors:
+        d, R = f.shape
+        x[s:s + R * d].view(R, d).copy_(f.detach().t())
+        s += R * d
+    return x
 
 
 class CPDLoss(torch.autograd.Function):
@@ -46,16 +54,16 @@ class CPDLoss(torch.autograd.Function):
         x = _flat64(factors)
         out = torch.empty(1, dtype=torch.float64, device=T.device)
         dev.residual_sq(Tc.view(-1), dims, x, R, out)
-        ctx.save_for_backward(Tc, *factors)
+        ctx.save_for_backward(Tc, x)
+        ctx.fdtypes = [f.dtype for f in factors]
         return (0.5 * out[0]).to(factors[0].dtype)
 
     @staticmethod
     def backward(ctx, g):
-        Tc, *factors = ctx.saved_tensors
+        Tc, x = ctx.saved_tensors
         dev = get_device(Tc.device.index or 0)
         dims = tuple(int(d) for d in Tc.shape)
-        L, R = len(dims), int(factors[0].shape[1])
-        x = _flat64(factors)
+        L, R = len(dims), x.numel() // sum(dims)
         f64 = torch.float64
         Gr, P1, P2 = dev.empty((L, R, R), f64), dev.empty((L, R, R), f64), dev.empty((L, L, R, R), f64)
         gamma = dev.empty((L, R), f64)
@@ -65,10 +73,17 @@ class CPDLoss(torch.autograd.Function):
         dev.mttkrp(Tc.view(-1), dims, x, R, N, prec=prec)
         ng = dev.empty(x.numel(), f64)
         dev.neg_grad(dims, R, x, P1, N, ng)            # N - F P1 = minus the gradient
-        scale = -g.to(f64)
+        # gradient = -g (N - F P1), scaled in FP64 and converted once for the whole vector when
+        # the factors share a dtype; factor l's gradient is the (I_l, R) transpose of its block
+        gall = ng.mul_(-g.to(f64))
+        same = all(dt == ctx.fdtypes[0] for dt in ctx.fdtypes)
+        if same and ctx.fdtypes[0] != f64:
+            gall = gall.to(ctx.fdtypes[0])
         grads, s = [], 0
         for l, d in enumerate(dims):
-            gl = (ng[s:s + R * d].view(R, d).t() * scale).to(factors[l].dtype)
+            gl = gall[s:s + R * d].view(R, d).t()
+            if gl.dtype != ctx.fdtypes[l]:
+                gl = gl.to(ctx.fdtypes[l])
             grads.append(gl if ctx.needs_input_grad[l + 1] else None)
             s += R * d
         return (None, *grads)
