@@ -9,7 +9,9 @@
 // when it is positive (Conversion.py:412-444), and written in the flat factor layout.
 // Members run in parallel on host threads.
 #include <math.h>
+#include <sched.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <thread>
@@ -83,6 +85,26 @@ struct MT {
     return f * x2;
   }
 
+  // Advances the stream exactly as cnt calls of gauss_polar would, forming only the value left
+  // in the cache (the discarded values need no logarithm or square root).
+  void skip_gauss(int64_t cnt) {
+    if (cnt <= 0) return;
+    if (has_gauss) {
+      has_gauss = false;
+      gauss = 0.0;
+      --cnt;
+    }
+    for (int64_t p = 0; p < cnt / 2; ++p) {   // pairs whose two values are both discarded
+      double x1, x2, r2;
+      do {
+        x1 = 2.0 * next_double() - 1.0;
+        x2 = 2.0 * next_double() - 1.0;
+        r2 = x1 * x1 + x2 * x2;
+      } while (r2 >= 1.0 || r2 == 0.0);
+    }
+    if (cnt & 1) gauss_polar();   // its first value discarded, the second cached
+  }
+
   // integer in [0, rng] (rng < 2^32 - 1), masked rejection
   uint32_t bounded(uint32_t rng) {
     if (rng == 0) return 0;
@@ -127,7 +149,7 @@ extern "C" int tfx_batch_draws(int nbatch, int L, const uint32_t* seeds, const i
         const int64_t n_rest = prod / sd[l];
         const int64_t rows = sd[l] < n_rest ? sd[l] : n_rest;
         const int64_t cnt = rows * (std::min<int64_t>(R, d[l]) + 10);
-        for (int64_t i = 0; i < cnt; ++i) mt.gauss_polar();
+        mt.skip_gauss(cnt);
         sd[l] = k[l];
       }
       // starting factors, drawn row-major (k_l x R) per mode
@@ -192,7 +214,17 @@ extern "C" int tfx_batch_draws(int nbatch, int L, const uint32_t* seeds, const i
       }
     }
   };
-  int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+  int nt = nthreads;
+  if (nt <= 0) {
+    // the CPUs this process may run on, and at most OMP_NUM_THREADS when that is set
+    nt = (int)std::thread::hardware_concurrency();
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) nt = std::min(nt, (int)CPU_COUNT(&set));
+    if (const char* e = getenv("OMP_NUM_THREADS")) {
+      const int v = atoi(e);
+      if (v > 0) nt = std::min(nt, v);
+    }
+  }
   nt = std::max(1, std::min(nt, std::max(1, nbatch / 16)));
   if (nt == 1) {
     work(0, nbatch);
