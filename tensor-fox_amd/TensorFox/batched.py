@@ -423,9 +423,17 @@ def _solve(dev, plan, options, seeds, config, precision, tdt, Tbuf, meta, order,
     timings['dgn'] = t3 - t2
     dev._call('tfx_batch_finish', _p(Tbuf), _lib.TFX_F32 if tdt == torch.float32 else _lib.TFX_F64,
               int(L), int(nb), _p(meta), _p(order), _p(kdims), _p(U), _p(work), _p(Ffull), _p(res), st())
-    resh = res.cpu().numpy().reshape(nb, RS_STRIDE)
-    hh = hist.cpu().numpy()
-    Fh = Ffull.cpu().numpy()
+    # results to fresh page-locked host buffers (the returned factors and histories are views
+    # into them), the three copies in flight together
+    host = []
+    for t in (res, hist, Ffull):
+        h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        host.append(h)
+    torch.cuda.current_stream().synchronize()
+    resh = host[0].numpy().reshape(nb, RS_STRIDE)
+    hh = host[1].numpy()
+    Fh = host[2].numpy()
     timings['finish'] = time.perf_counter() - t3
     return kd, resh, hh, Fh
 
