@@ -17,13 +17,14 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int NT = 256;         // 4 waves
 constexpr int KCH = 64;         // columns per workgroup (4 tiles of 16)
 constexpr int CP = KCH + 1;     // LDS pitch of the staged C chunk (odd)
-constexpr int QMAX = 16;        // MFMA k steps: R rounded up to 4, <= 64
 
 __device__ inline dbl4 mfma64(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-template <typename TS, int LM1>
+// NQ: MFMA k steps per tile, a compile-time count (R rounded up to a multiple of 8, ranks past R
+// masked), so the step loop unrolls without branches.
+template <typename TS, int LM1, int NQ>
 __global__ __launch_bounds__(NT) void k_residual_mfma(const TS* __restrict__ T, Shape s,
                                                        const double* __restrict__ F,
                                                        int tiles_per_wave,
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(NT) void k_residual_mfma(const TS* __restrict__ T, 
   const int64_t NR = s.N / K;
   const int kc0 = blockIdx.x * KCH;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63, li = lane & 15, lg = lane >> 4;
-  const int R4 = (R + 3) & ~3, nq = R4 >> 2;
+  const int R4 = 4 * NQ;   // rows of the staged C chunk (ranks past R are zero)
   const double* FL = F + s.off[LM1];
   for (int e = t; e < R4 * KCH; e += NT) {
     const int r = e / KCH, kk = e - r * KCH;
@@ -81,30 +82,24 @@ __global__ __launch_bounds__(NT) void k_residual_mfma(const TS* __restrict__ T, 
         rem = qd;
       }
     }
-    double a[QMAX];
+    double a[NQ];
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q) {
-      a[q] = 0.0;
-      if (q < nq) {
-        const int r = 4 * q + lg;
-        const int rc = r < R ? r : R - 1;
-        double w = 1.0;
+    for (int q = 0; q < NQ; ++q) {
+      const int r = 4 * q + lg;
+      const int rc = r < R ? r : R - 1;
+      double w = 1.0;
 #pragma unroll
-        for (int m = 0; m < LM1; ++m) w *= F[offm[m] + (int64_t)rc * dm[m] + idx[m]];
-        a[q] = (rok && r < R) ? w : 0.0;
-      }
+      for (int m = 0; m < LM1; ++m) w *= F[offm[m] + (int64_t)rc * dm[m] + idx[m]];
+      a[q] = (rok && r < R) ? w : 0.0;
     }
     dbl4 acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
     const double* cb = Cs + lg * CP + li;
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q) {
-      if (q < nq) {
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) acc[ct] = mfma64(a[q], cb[4 * q * CP + 16 * ct], acc[ct]);
-      }
-    }
+      for (int ct = 0; ct < 4; ++ct) acc[ct] = mfma64(a[q], cb[4 * q * CP + 16 * ct], acc[ct]);
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
       const bool cok = kc0 + 16 * ct + li < K;
@@ -128,6 +123,13 @@ __global__ void k_sum_parts_res(const double* __restrict__ part, int n, double* 
   if (threadIdx.x == 0) *out = tot;
 }
 
+template <typename TS, int LM1, int NQ>
+void launch_nq(const TS* T, const Shape& s, const double* F, int kchunks, int groups, int tpw,
+               double* part, hipStream_t st) {
+  hipLaunchKernelGGL((k_residual_mfma<TS, LM1, NQ>), dim3((unsigned)kchunks, (unsigned)groups),
+                     dim3(NT), 0, st, T, s, F, tpw, part);
+}
+
 template <typename TS, int LM1>
 int launch(tfx_ctx* ctx, const TS* T, const Shape& s, const double* F, double* out,
            hipStream_t st) {
@@ -147,8 +149,18 @@ int launch(tfx_ctx* ctx, const TS* T, const Shape& s, const double* F, double* o
   int rc = tfx_workspace(ctx, align256(sizeof(double) * nparts), st, &wsp);
   if (rc) return rc;
   double* part = (double*)wsp;
-  hipLaunchKernelGGL((k_residual_mfma<TS, LM1>), dim3((unsigned)kchunks, (unsigned)groups),
-                     dim3(NT), 0, st, T, s, F, (int)tpw, part);
+  const int nq8 = ((s.R + 3) / 4 + 1) / 2;   // k steps in pairs: 2 nq8 steps, 8 nq8 >= R
+  const int kc = (int)kchunks, gr = (int)groups, tp = (int)tpw;
+  switch (nq8) {
+    case 1: launch_nq<TS, LM1, 2>(T, s, F, kc, gr, tp, part, st); break;
+    case 2: launch_nq<TS, LM1, 4>(T, s, F, kc, gr, tp, part, st); break;
+    case 3: launch_nq<TS, LM1, 6>(T, s, F, kc, gr, tp, part, st); break;
+    case 4: launch_nq<TS, LM1, 8>(T, s, F, kc, gr, tp, part, st); break;
+    case 5: launch_nq<TS, LM1, 10>(T, s, F, kc, gr, tp, part, st); break;
+    case 6: launch_nq<TS, LM1, 12>(T, s, F, kc, gr, tp, part, st); break;
+    case 7: launch_nq<TS, LM1, 14>(T, s, F, kc, gr, tp, part, st); break;
+    default: launch_nq<TS, LM1, 16>(T, s, F, kc, gr, tp, part, st); break;
+  }
   TFX_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_sum_parts_res, dim3(1), dim3(256), 0, st, part, (int)nparts, out);
   TFX_CHECK_LAUNCH(ctx);
