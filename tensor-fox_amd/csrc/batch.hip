@@ -301,7 +301,8 @@ __device__ inline void b_gramians(const Member& M, const MemP& P) {
 #pragma unroll
   for (int l = 0; l < L; ++l) {
     const double* fa = P.s.Fs + (int64_t)(g.row[l] + lg) * g.PR + li;
-    gr[l] = ldscg::mfma_chain<4>(fa, 4 * g.PR, fa, 4 * g.PR, g.ip4[l] >> 2, dbl4{0.0, 0.0, 0.0, 0.0});
+    gr[l] = ldscg::mfma_chain_short<4>(fa, 4 * g.PR, fa, 4 * g.PR, g.ip4[l] >> 2,
+                                       dbl4{0.0, 0.0, 0.0, 0.0});
   }
   double mx = 0.0;
 #pragma unroll
@@ -396,7 +397,8 @@ __device__ inline void b_grad(const Member& M, const MemP& P) {
     const dbl4 N = b_mttkrp_mode<L>(M, P, l);
     const double* fa = P.s.Fs + (int64_t)(g.row[l] + li) * g.PR + lg;
     const double* pb = P.s.P1s + (int64_t)(l * g.R4 + lg) * g.PR + li;
-    const dbl4 fp = ldscg::mfma_chain<4>(fa, 4, pb, 4 * g.PR, g.R4 >> 2, dbl4{0.0, 0.0, 0.0, 0.0});
+    const dbl4 fp = ldscg::mfma_chain_short<4>(fa, 4, pb, 4 * g.PR, g.R4 >> 2,
+                                               dbl4{0.0, 0.0, 0.0, 0.0});
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i = 4 * j + lg, r = li;
@@ -439,15 +441,15 @@ __device__ inline double b_err2(const Member& M, const MemP& P) {
         rem = d;
       }
     }
+    // four k steps whatever nq (bq is zero past it; Fs columns up to 15 are in the padded row),
+    // so the chain has no branches
     dbl4 acc{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (q < nq) {
-        double a = 1.0;
+      double a = 1.0;
 #pragma unroll
-        for (int p = 0; p < L - 1; ++p) a *= Fs[(int64_t)(g.row[p] + ix[p]) * g.PR + 4 * q + lg];
-        acc = mfma64(ok ? a : 0.0, bq[q], acc);
-      }
+      for (int p = 0; p < L - 1; ++p) a *= Fs[(int64_t)(g.row[p] + ix[p]) * g.PR + 4 * q + lg];
+      acc = mfma64(ok ? a : 0.0, bq[q], acc);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -790,7 +792,7 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
     b_grad<L>(M, P);
     int itn;
     double rn;
-    ldscg::cg_body<1, 4, 1>(M.g, P.s, P.gam, damp, P.vec[V_G], cap, o.cg_tol, P.vec[V_Y],
+    ldscg::cg_body<1, 4, 1, true>(M.g, P.s, P.gam, damp, P.vec[V_G], cap, o.cg_tol, P.vec[V_Y],
                             P.vec[V_JG], red, &itn, &rn);
     __syncthreads();
     double* x = xi ? P.vec[V_X0 + 1] : P.vec[V_X0];

@@ -97,6 +97,35 @@ __device__ inline dbl4 mfma_chain(const double* a, int sa, const double* b, int 
   return acc;
 }
 
+// acc += sum_q A_q B_q for steps <= U: one chunk, no loop (address clamped, A zeroed past the
+// end), for callers whose chains are known to be that short.
+template <int U>
+__device__ inline dbl4 mfma_chain_short(const double* a, int sa, const double* b, int sb,
+                                        int steps, dbl4 acc) {
+  double av[U], bv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const bool ok = u < steps;
+    const int q = ok ? u : 0;
+    av[u] = a[q * sa];
+    bv[u] = b[q * sb];
+    av[u] = ok ? av[u] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) acc = mfma64(av[u], bv[u], acc);
+  return acc;
+}
+
+// SHORT: every chain has at most 4 steps (cores of at most 16 rows per mode, R <= 16)
+template <bool SHORT>
+__device__ inline dbl4 chain(const double* a, int sa, const double* b, int sb, int steps,
+                             dbl4 acc) {
+  if constexpr (SHORT)
+    return mfma_chain_short<4>(a, sa, b, sb, steps, acc);
+  else
+    return mfma_chain<4>(a, sa, b, sb, steps, acc);
+}
+
 struct Lds {
   double *Fs, *qs, *Ts, *P1s, *P2s;
 };
@@ -123,7 +152,7 @@ __device__ inline void zjob(const Geom& g, int j, int* l, int* i0, int* r0) {
 
 // tmp_l = sum_{m != l} P2[l][m] o (q_m^T F_m) into Ts.  Job (c-tile, r-tile) per wave, every
 // mode's product formed as an MFMA chain over the rows and combined in registers.
-template <int NW, int JWMAX>
+template <int NW, int JWMAX, bool SHORT = false>
 __device__ inline void tmp_phase(const Geom& g, const Lds& s, int wave, int lane) {
   const int li = lane & 15, lg = lane >> 4;
   const int nwj = g.nrt * g.nrt;
@@ -140,7 +169,7 @@ __device__ inline void tmp_phase(const Geom& g, const Lds& s, int wave, int lane
         if (m < g.L) {
           const double* qa = s.qs + (int64_t)(g.row[m] + lg) * g.PR + c0 + li;
           const double* fb = s.Fs + (int64_t)(g.row[m] + lg) * g.PR + r0 + li;
-          acc[m] = mfma_chain<4>(qa, 4 * g.PR, fb, 4 * g.PR, g.ip4[m] >> 2, acc[m]);
+          acc[m] = chain<SHORT>(qa, 4 * g.PR, fb, 4 * g.PR, g.ip4[m] >> 2, acc[m]);
         }
       }
       const int r = r0 + li;
@@ -170,6 +199,7 @@ __device__ inline void tmp_phase(const Geom& g, const Lds& s, int wave, int lane
 
 // F_l tmp_l + q_l P1_l for the z job (l, i0, r0): register j holds row i0 + 4 j + lane / 16,
 // rank r0 + lane % 16.
+template <bool SHORT = false>
 __device__ inline dbl4 z_job(const Geom& g, const Lds& s, int l, int i0, int r0, int lane) {
   const int li = lane & 15, lg = lane >> 4;
   const double* fa = s.Fs + (int64_t)(g.row[l] + i0 + li) * g.PR + lg;
@@ -178,8 +208,8 @@ __device__ inline dbl4 z_job(const Geom& g, const Lds& s, int l, int i0, int r0,
   const double* pb = s.P1s + (int64_t)(l * g.R4 + lg) * g.PR + r0 + li;
   const dbl4 zero{0.0, 0.0, 0.0, 0.0};
   const int steps = g.R4 >> 2;
-  const dbl4 a1 = mfma_chain<4>(fa, 4, tb, 4 * g.PR, steps, zero);
-  const dbl4 a2 = mfma_chain<4>(qa, 4, pb, 4 * g.PR, steps, zero);
+  const dbl4 a1 = chain<SHORT>(fa, 4, tb, 4 * g.PR, steps, zero);
+  const dbl4 a2 = chain<SHORT>(qa, 4, pb, 4 * g.PR, steps, zero);
   return a1 + a2;
 }
 
