@@ -792,7 +792,7 @@ __global__ __launch_bounds__(64) void k_bdgn(const int64_t* __restrict__ meta,
     b_grad<L>(M, P);
     int itn;
     double rn;
-    ldscg::cg_body<1, 4, 1, true>(M.g, P.s, P.gam, damp, P.vec[V_G], cap, o.cg_tol, P.vec[V_Y],
+    ldscg::cg_body<1, 4, 1, 4, 4>(M.g, P.s, P.gam, damp, P.vec[V_G], cap, o.cg_tol, P.vec[V_Y],
                             P.vec[V_JG], red, &itn, &rn);
     __syncthreads();
     double* x = xi ? P.vec[V_X0 + 1] : P.vec[V_X0];

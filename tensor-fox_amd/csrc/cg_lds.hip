@@ -9,6 +9,10 @@ constexpr int NW = 8;       // waves
 constexpr int JZMAX = 2;    // z jobs (16 x 16 output tiles) per wave
 constexpr int JWMAX = 1;    // (c, r) tiles of tmp per wave
 
+// SHORT: 8 when every MFMA chain has at most 8 steps (rows per mode and R at most 32): the z
+// phase then runs its chains as two unrolled chunks (the tmp phase keeps its loop: unrolling its
+// L chains at once exceeds the register budget); 0 for the general chunked chains
+template <int SHORT>
 __global__ __launch_bounds__(NW * 64) void k_cg_lds(ldscg::Geom g, const double* __restrict__ F,
                                                     const double* __restrict__ P1,
                                                     const double* __restrict__ P2,
@@ -26,7 +30,8 @@ __global__ __launch_bounds__(NW * 64) void k_cg_lds(ldscg::Geom g, const double*
   __syncthreads();
   int itn;
   double rn;
-  ldscg::cg_body<NW, JZMAX, JWMAX>(g, s, gamma, damp, grad, maxiter, tol, y, jtjg, red, &itn, &rn);
+  ldscg::cg_body<NW, JZMAX, JWMAX, 0, SHORT>(g, s, gamma, damp, grad, maxiter, tol, y, jtjg,
+                                             red, &itn, &rn);
   if (threadIdx.x == 0) {
     scal[0] = (double)itn;
     scal[1] = rn;
@@ -51,10 +56,19 @@ int cg_lds_try(tfx_ctx* ctx, const Shape& s, const double* F, const double* P1, 
   if (g.nz > NW * JZMAX || g.nrt * g.nrt > NW * JWMAX) return 0;
   const size_t lds = sizeof(double) * (size_t)ldscg::lds_doubles(g);
   if (lds > 150 * 1024) return 0;
-  TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_lds,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_cg_lds, dim3(1), dim3(NW * 64), lds, st, g, F, P1, P2, gamma, damp, grad,
-                     maxiter, tol, y, jtjg, scal);
+  int maxrows = 0;
+  for (int l = 0; l < s.L; ++l) maxrows = std::max(maxrows, g.ip4[l]);
+  if (maxrows <= 32 && g.R4 <= 32) {
+    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_lds<8>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_cg_lds<8>, dim3(1), dim3(NW * 64), lds, st, g, F, P1, P2, gamma, damp,
+                       grad, maxiter, tol, y, jtjg, scal);
+  } else {
+    TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_cg_lds<0>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_cg_lds<0>, dim3(1), dim3(NW * 64), lds, st, g, F, P1, P2, gamma, damp,
+                       grad, maxiter, tol, y, jtjg, scal);
+  }
   TFX_CHECK_LAUNCH(ctx);
   return 1;
 }

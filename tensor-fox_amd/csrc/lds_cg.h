@@ -116,12 +116,37 @@ __device__ inline dbl4 mfma_chain_short(const double* a, int sa, const double* b
   return acc;
 }
 
-// SHORT: every chain has at most 4 steps (cores of at most 16 rows per mode, R <= 16)
-template <bool SHORT>
+// acc += sum_q A_q B_q for steps <= 4 C: C unrolled chunks of 4 steps with no loop or branch
+// (addresses past the end clamped to step 0, A zeroed there).
+template <int C>
+__device__ inline dbl4 mfma_chain_chunks(const double* a, int sa, const double* b, int sb,
+                                         int steps, dbl4 acc) {
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool ok = 4 * c + u < steps;
+      const int q = ok ? 4 * c + u : 0;
+      av[u] = a[q * sa];
+      bv[u] = b[q * sb];
+      av[u] = ok ? av[u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc = mfma64(av[u], bv[u], acc);
+  }
+  return acc;
+}
+
+// SHORT > 0: every chain has at most SHORT steps (cores of at most 4 SHORT rows per mode and
+// R <= 4 SHORT), run as SHORT / 4 unrolled chunks; 0: any length, in a loop of chunks of 4
+template <int SHORT>
 __device__ inline dbl4 chain(const double* a, int sa, const double* b, int sb, int steps,
                              dbl4 acc) {
-  if constexpr (SHORT)
+  if constexpr (SHORT == 4)
     return mfma_chain_short<4>(a, sa, b, sb, steps, acc);
+  else if constexpr (SHORT > 0)
+    return mfma_chain_chunks<SHORT / 4>(a, sa, b, sb, steps, acc);
   else
     return mfma_chain<4>(a, sa, b, sb, steps, acc);
 }
@@ -152,7 +177,7 @@ __device__ inline void zjob(const Geom& g, int j, int* l, int* i0, int* r0) {
 
 // tmp_l = sum_{m != l} P2[l][m] o (q_m^T F_m) into Ts.  Job (c-tile, r-tile) per wave, every
 // mode's product formed as an MFMA chain over the rows and combined in registers.
-template <int NW, int JWMAX, bool SHORT = false>
+template <int NW, int JWMAX, int SHORT = 0>
 __device__ inline void tmp_phase(const Geom& g, const Lds& s, int wave, int lane) {
   const int li = lane & 15, lg = lane >> 4;
   const int nwj = g.nrt * g.nrt;
@@ -199,7 +224,7 @@ __device__ inline void tmp_phase(const Geom& g, const Lds& s, int wave, int lane
 
 // F_l tmp_l + q_l P1_l for the z job (l, i0, r0): register j holds row i0 + 4 j + lane / 16,
 // rank r0 + lane % 16.
-template <bool SHORT = false>
+template <int SHORT = 0>
 __device__ inline dbl4 z_job(const Geom& g, const Lds& s, int l, int i0, int r0, int lane) {
   const int li = lane & 15, lg = lane >> 4;
   const double* fa = s.Fs + (int64_t)(g.row[l] + i0 + li) * g.PR + lg;

@@ -61,8 +61,9 @@ __device__ inline double inv_m64(double gm, double d) {
 
 // Needs: Fs, qs (= grad), P1s, P2s staged; Ts zero.  Writes jtjg (if not null) and y (flat).
 // Every thread of the workgroup calls it.  Returns the iteration count as the reference reports
-// it (itn + 1) and the final ||r||^2.
-template <int NW, int JZMAX, int JWMAX, bool SHORT = false>
+// it (itn + 1) and the final ||r||^2.  SHORT_T / SHORT_Z: the chain bound of the tmp and z
+// phases (see chain(); 0 = any length).
+template <int NW, int JZMAX, int JWMAX, int SHORT_T = 0, int SHORT_Z = 0>
 __device__ inline void cg_body(const Geom& g, const Lds& s, const double* gamma, double damp,
                                const double* grad, int maxiter, double tol, double* y,
                                double* jtjg, double (*red)[NW], int* itn_out, double* rn_out) {
@@ -97,13 +98,13 @@ __device__ inline void cg_body(const Geom& g, const Lds& s, const double* gamma,
   };
 
   // J^T J grad
-  tmp_phase<NW, JWMAX, SHORT>(g, s, wave, lane);
+  tmp_phase<NW, JWMAX, SHORT_T>(g, s, wave, lane);
   __syncthreads();
   dbl4 z[JZMAX];
 #pragma unroll
   for (int k = 0; k < JZMAX; ++k) {
     if (jon[k]) {
-      z[k] = z_job<SHORT>(g, s, jl[k], ji0[k], jr0[k], lane);
+      z[k] = z_job<SHORT_Z>(g, s, jl[k], ji0[k], jr0[k], lane);
       if (jtjg) {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -138,13 +139,13 @@ __device__ inline void cg_body(const Geom& g, const Lds& s, const double* gamma,
       for (int j = 0; j < 4; ++j)
         if (valid(k, j)) s.qs[qaddr(k, j)] = Mv[k] * pv[k][j];
     __syncthreads();
-    tmp_phase<NW, JWMAX, SHORT>(g, s, wave, lane);
+    tmp_phase<NW, JWMAX, SHORT_T>(g, s, wave, lane);
     __syncthreads();
     acc = 0.0;
 #pragma unroll
     for (int k = 0; k < JZMAX; ++k) {
       if (jon[k]) {
-        const dbl4 zp = z_job<SHORT>(g, s, jl[k], ji0[k], jr0[k], lane);
+        const dbl4 zp = z_job<SHORT_Z>(g, s, jl[k], ji0[k], jr0[k], lane);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const double qe = Mv[k] * pv[k][j];
