@@ -179,7 +179,7 @@ __global__ __launch_bounds__(MT) void k_bmlsvd(const TS* __restrict__ T,
     int nc = (int)min((int64_t)min(R, (int)I[l]), min((int64_t)m, n_rest));
     if (cur == 0) gram_unfold<TS>(X0, m, post, n_rest, G, V);
     else gram_unfold<double>(Xd, m, post, n_rest, G, V);
-    tfxjac::wg_jacobi(G, m, V, cs, red);
+    tfxjac::wg_jacobi(G, m, m, V, cs, red);
     for (int i = t; i < m; i += MT) {
       const double di = G[i * m + i];
       int rk = 0;

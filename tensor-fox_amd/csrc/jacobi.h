@@ -6,9 +6,11 @@
 
 namespace tfxjac {
 
-// Parallel cyclic Jacobi (round-robin pairing) on the symmetric m x m matrix H (row-major, in
-// place); V (m x m, col-major) receives the eigenvectors.  Eigenvalues end on the diagonal.
-__device__ inline void wg_jacobi(double* H, int m, double* V, double* cs, double* red) {
+// Parallel cyclic Jacobi (round-robin pairing) on the symmetric m x m matrix H (row-major with
+// row stride ld >= m, in place); V (m x m, col-major) receives the eigenvectors.  Eigenvalues
+// end on the diagonal.  An odd ld keeps the column accesses of H (the mirrored writes) free of
+// LDS bank conflicts.
+__device__ inline void wg_jacobi(double* H, int m, int ld, double* V, double* cs, double* red) {
   // Cyclic parallel Jacobi (round-robin ordering: every round rotates mm/2 disjoint pairs).
   // A round computes its rotations from the current 2 x 2 diagonal blocks, then applies both
   // sides at once: the 2 x 2 block of rows (a1, b1) and columns (a2, b2) becomes R1^T B R2, each
@@ -30,7 +32,7 @@ __device__ inline void wg_jacobi(double* H, int m, double* V, double* cs, double
     double off = 0.0, dia = 0.0;
     for (int i = wave; i < m; i += nwave)
       for (int j = lane; j < m; j += 64) {
-        const double h = H[i * m + j];
+        const double h = H[i * ld + j];
         if (i == j) dia += h * h;
         else off += h * h;
       }
@@ -57,12 +59,12 @@ __device__ inline void wg_jacobi(double* H, int m, double* V, double* cs, double
       }
       double c = 1.0, sn = 0.0;
       if (b < m) {
-        const double hab = H[a * m + b];
+        const double hab = H[a * ld + b];
         if (hab != 0.0) {
           // t = tan(theta) of the symmetric Schur rotation, as sign(tau) / (|tau| +
           // sqrt(1 + tau^2)) with tau = d / (2 hab), d = hbb - haa, written with one square
           // root and one division: t = 2 hab sgn(d) / (|d| + sqrt(d^2 + 4 hab^2)), sgn(0) = 1
-          const double d = H[b * m + b] - H[a * m + a];
+          const double d = H[b * ld + b] - H[a * ld + a];
           const double num = d >= 0.0 ? 2.0 * hab : -2.0 * hab;
           const double tt = d == 0.0 ? 1.0 : num / (fabs(d) + sqrt(d * d + 4.0 * hab * hab));
           // c = 1 / sqrt(1 + t^2) from the hardware reciprocal square root plus one Newton step
@@ -128,36 +130,36 @@ __device__ inline void wg_jacobi(double* H, int m, double* V, double* cs, double
           const double2 r1 = cur[k1], r2 = cur[k2];
           const double c1 = r1.x, s1 = r1.y, c2 = r2.x, s2 = r2.y;
           const bool v1 = b1 < m, v2 = b2 < m;
-          const double h00 = H[a1 * m + a2];
-          const double h01 = v2 ? H[a1 * m + b2] : 0.0;
-          const double h10 = v1 ? H[b1 * m + a2] : 0.0;
-          const double h11 = (v1 && v2) ? H[b1 * m + b2] : 0.0;
+          const double h00 = H[a1 * ld + a2];
+          const double h01 = v2 ? H[a1 * ld + b2] : 0.0;
+          const double h10 = v1 ? H[b1 * ld + a2] : 0.0;
+          const double h11 = (v1 && v2) ? H[b1 * ld + b2] : 0.0;
           const double x00 = c1 * h00 - s1 * h10, x01 = c1 * h01 - s1 * h11;
           const double x10 = s1 * h00 + c1 * h10, x11 = s1 * h01 + c1 * h11;
           const double y00 = c2 * x00 - s2 * x01, y01 = s2 * x00 + c2 * x01;
           const double y10 = c2 * x10 - s2 * x11, y11 = s2 * x10 + c2 * x11;
           if (k1 == k2) {
             // diagonal block: its two off-diagonal entries are one value
-            H[a1 * m + a1] = y00;
+            H[a1 * ld + a1] = y00;
             if (v1) {
-              H[a1 * m + b1] = y01;
-              H[b1 * m + a1] = y01;
-              H[b1 * m + b1] = y11;
+              H[a1 * ld + b1] = y01;
+              H[b1 * ld + a1] = y01;
+              H[b1 * ld + b1] = y11;
             }
           } else {
-            H[a1 * m + a2] = y00;
-            H[a2 * m + a1] = y00;
+            H[a1 * ld + a2] = y00;
+            H[a2 * ld + a1] = y00;
             if (v2) {
-              H[a1 * m + b2] = y01;
-              H[b2 * m + a1] = y01;
+              H[a1 * ld + b2] = y01;
+              H[b2 * ld + a1] = y01;
             }
             if (v1) {
-              H[b1 * m + a2] = y10;
-              H[a2 * m + b1] = y10;
+              H[b1 * ld + a2] = y10;
+              H[a2 * ld + b1] = y10;
             }
             if (v1 && v2) {
-              H[b1 * m + b2] = y11;
-              H[b2 * m + b1] = y11;
+              H[b1 * ld + b2] = y11;
+              H[b2 * ld + b1] = y11;
             }
           }
         }
@@ -195,18 +197,18 @@ __device__ inline void wg_jacobi(double* H, int m, double* V, double* cs, double
             const double c1 = r1.x, s1 = r1.y, c2 = r2.x, s2 = r2.y;
             const bool v1 = b1 < m;
             const bool v2 = b2 < m;
-            const double h00 = H[a1 * m + a2];
-            const double h01 = v2 ? H[a1 * m + b2] : 0.0;
-            const double h10 = v1 ? H[b1 * m + a2] : 0.0;
-            const double h11 = (v1 && v2) ? H[b1 * m + b2] : 0.0;
+            const double h00 = H[a1 * ld + a2];
+            const double h01 = v2 ? H[a1 * ld + b2] : 0.0;
+            const double h10 = v1 ? H[b1 * ld + a2] : 0.0;
+            const double h11 = (v1 && v2) ? H[b1 * ld + b2] : 0.0;
             // rows: x_a = c1 h_a - s1 h_b, x_b = s1 h_a + c1 h_b
             const double x00 = c1 * h00 - s1 * h10, x01 = c1 * h01 - s1 * h11;
             const double x10 = s1 * h00 + c1 * h10, x11 = s1 * h01 + c1 * h11;
             // columns: y_a = c2 x_a - s2 x_b, y_b = s2 x_a + c2 x_b
-            H[a1 * m + a2] = c2 * x00 - s2 * x01;
-            if (v2) H[a1 * m + b2] = s2 * x00 + c2 * x01;
-            if (v1) H[b1 * m + a2] = c2 * x10 - s2 * x11;
-            if (v1 && v2) H[b1 * m + b2] = s2 * x10 + c2 * x11;
+            H[a1 * ld + a2] = c2 * x00 - s2 * x01;
+            if (v2) H[a1 * ld + b2] = s2 * x00 + c2 * x01;
+            if (v1) H[b1 * ld + a2] = c2 * x10 - s2 * x11;
+            if (v1 && v2) H[b1 * ld + b2] = s2 * x10 + c2 * x11;
           }
         }
         // V <- V J (V[col][row] holds column col), item q = k m + row

@@ -337,24 +337,26 @@ __global__ __launch_bounds__(1024) void k_jacobi_top(double* H, int m, int k, do
   __shared__ double red[16];
   __shared__ __align__(16) double cs[4 * 260];
   __shared__ int order[512];
+  // in LDS, H has an odd row stride (no bank conflicts on its column accesses)
+  const int ld = IN_LDS ? (m | 1) : m;
   double* Hs = IN_LDS ? (double*)smem : H;
-  double* Vs = IN_LDS ? (double*)smem + m * m : Vg;
+  double* Vs = IN_LDS ? (double*)smem + m * ld : Vg;
   // symmetrized copy (the Rayleigh-Ritz product is symmetric only up to rounding)
   for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
     const int i = e / m, j = e % m;
     if (i <= j) {
       const double a = i == j ? H[e] : 0.5 * (H[i * m + j] + H[j * m + i]);
-      Hs[i * m + j] = a;
-      Hs[j * m + i] = a;
+      Hs[i * ld + j] = a;
+      Hs[j * ld + i] = a;
     }
   }
   __syncthreads();
-  tfxjac::wg_jacobi(Hs, m, Vs, cs, red);
+  tfxjac::wg_jacobi(Hs, m, ld, Vs, cs, red);
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const double di = Hs[i * m + i];
+    const double di = Hs[i * ld + i];
     int rank = 0;
     for (int j = 0; j < m; ++j) {
-      const double dj = Hs[j * m + j];
+      const double dj = Hs[j * ld + j];
       if (dj > di || (dj == di && j < i)) ++rank;
     }
     order[rank] = i;
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(1024) void k_jacobi_top(double* H, int m, int k, do
     const int r = e % m, c = e / m;
     Vk[e] = Vs[order[c] * m + r];
   }
-  for (int c = threadIdx.x; c < k; c += blockDim.x) lam[c] = Hs[order[c] * m + order[c]];
+  for (int c = threadIdx.x; c < k; c += blockDim.x) lam[c] = Hs[order[c] * ld + order[c]];
 }
 
 // Make the largest-|.| entry of every column of U (n x k col-major) positive; one workgroup per
@@ -576,7 +578,7 @@ extern "C" int tfx_eig_top(tfx_ctx* ctx, const double* G, int n, const double* Q
     gemm(1, p, p, n, Q, Z, H);
   }
   TFX_CHECK_LAUNCH(ctx);
-  const size_t lds = sizeof(double) * 2 * (size_t)m * m;
+  const size_t lds = sizeof(double) * ((size_t)m * (m | 1) + (size_t)m * m);
   if (lds <= 140 * 1024) {
     TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_jacobi_top<true>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
