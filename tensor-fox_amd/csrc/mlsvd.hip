@@ -209,6 +209,7 @@ __global__ __launch_bounds__(1024) void k_orth(double* Z, int n, int p) {
 // projections one pass over the finished columns per panel; on 400 x 60 blocks panels of 4
 // measured fastest (8: 590 µs, 4: 450 µs, 2: 440 µs per call).
 constexpr int OPB = 4;
+constexpr int ONU = 8;    // rows per lane of the register-resident in-panel CGS2 (n <= 512)
 constexpr int ORTH_MAXN = 2048;   // n * OPB doubles of LDS
 
 __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
@@ -280,8 +281,59 @@ __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
         }
         __syncthreads();
       }
-      // CGS2 inside the panel on wavefront 0 (lane-owned rows e = lane + 64 u)
-      if (wave == 0) {
+      // CGS2 inside the panel on wavefront 0 (lane-owned rows e = lane + 64 u); for n <= 64 ONU
+      // the panel stays in registers for the whole pass (same operations in the same order)
+      if (wave == 0 && n <= 64 * ONU) {
+        double zr[OPB][ONU];
+#pragma unroll
+        for (int c = 0; c < OPB; ++c)
+#pragma unroll
+          for (int u = 0; u < ONU; ++u) {
+            const int e = lane + 64 * u;
+            zr[c][u] = (c < pb && e < n) ? zp[c * n + e] : 0.0;
+          }
+        bool redo = false;
+#pragma unroll
+        for (int c = 0; c < OPB; ++c) {
+          if (c >= pb) break;
+          for (int pass = 0; pass < 2 && c > 0; ++pass) {
+            double d[OPB];
+#pragma unroll
+            for (int i = 0; i < OPB; ++i) {
+              double a = 0.0;
+              if (i < c) {
+#pragma unroll
+                for (int u = 0; u < ONU; ++u) a += zr[i][u] * zr[c][u];
+              }
+              d[i] = i < c ? wave_sum(a) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < ONU; ++u) {
+              double v = zr[c][u];
+#pragma unroll
+              for (int i = 0; i < OPB; ++i)
+                if (i < c) v -= d[i] * zr[i][u];
+              zr[c][u] = v;
+            }
+          }
+          double a = 0.0;
+#pragma unroll
+          for (int u = 0; u < ONU; ++u) a += zr[c][u] * zr[c][u];
+          a = wave_sum(a);
+          if (a < 1e-6 * pre[c]) redo = true;
+          const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+#pragma unroll
+          for (int u = 0; u < ONU; ++u) zr[c][u] *= inv;
+        }
+#pragma unroll
+        for (int c = 0; c < OPB; ++c)
+#pragma unroll
+          for (int u = 0; u < ONU; ++u) {
+            const int e = lane + 64 * u;
+            if (c < pb && e < n) zp[c * n + e] = zr[c][u];
+          }
+        if (lane == 0) redo_s = redo && round == 0 && j0 > 0;
+      } else if (wave == 0) {
         bool redo = false;
 #pragma unroll
         for (int c = 0; c < OPB; ++c) {
