@@ -225,7 +225,23 @@ __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
     __syncthreads();
     for (int round = 0; round < 2; ++round) {
       double pre[OPB];   // wave 0: squared column norms before this round's projections
-      if (wave == 0) {
+      if (wave == 0 && n <= 64 * ONU) {
+        double zv[OPB][ONU];   // all of the lane's reads in flight together
+#pragma unroll
+        for (int c = 0; c < OPB; ++c)
+#pragma unroll
+          for (int u = 0; u < ONU; ++u) {
+            const int e = lane + 64 * u;
+            zv[c][u] = (c < pb && e < n) ? zp[c * n + e] : 0.0;
+          }
+#pragma unroll
+        for (int c = 0; c < OPB; ++c) {
+          double a = 0.0;
+#pragma unroll
+          for (int u = 0; u < ONU; ++u) a += zv[c][u] * zv[c][u];
+          pre[c] = wave_sum(a);
+        }
+      } else if (wave == 0) {
 #pragma unroll
         for (int c = 0; c < OPB; ++c) {
           double a = 0.0;
