@@ -361,8 +361,8 @@ def cpu_baseline(T32, R, sample_iters, seed):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py: this build measures one GPU; the CPD hot path here is a single-GPU kernel '
-              'set (multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.',
+        print('bench.py: only one GPU is measured; the CPD hot path here is a single-GPU kernel '
+              'set (sharding it over several GPUs is not implemented). Use --gpus 1.',
               file=sys.stderr)
         sys.exit(2)
     if args.kernel_only:
