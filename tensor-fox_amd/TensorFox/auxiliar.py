@@ -27,7 +27,7 @@ class Options:
 def make_options(options=False):
     """Defaults, then every attribute the user object defines (attribute presence, as in
     Auxiliar.py:288-353).  `gpu=True` without an explicit `mlsvd_method` selects the reference's
-    'gpu' MLSVD name (Auxiliar.py:359-360), which this build maps to its own Gram method."""
+    'gpu' MLSVD name (Auxiliar.py:359-360), which this package maps to its own Gram method."""
     out = Options()
     for k, v in _DEFAULTS.items():
         setattr(out, k, v)
