@@ -104,13 +104,28 @@ int tfx_cg(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, const voi
 
 /* x_new = x + alpha*y; F = x2cpd(x_new) with equalize when eq != 0, then transform(symm,
  * factors_norm).  Replaces x = x + y / x2cpd / equalize / transform (GaussNewton.py:285-289,
- * Conversion.py:21-49, 361-387, 412-444).  scal (double, 4 entries) receives
+ * Conversion.py:21-49, 361-387, 412-444).  symm: 0 no symmetric transform; 1 the average of
+ * the L blocks (the reference's transform on separate factor arrays); 2 the reference's
+ * transform on a factor list whose entries share one array, as every symmetric transform
+ * without factors_norm leaves it (Conversion.py:437-438): the last block alone, equalized with
+ * L equal column norms, times 2^(L-1)/L, copied to every mode.  scal (double, 4 entries) receives
  * [ (alpha y).(g + damp alpha y), ||alpha y||^2, ||x||^2, max|g| ] for the gain ratio, step size
  * and gradient norm (GaussNewton.py:174, 184-185, 472-487). */
 int tfx_dgn_update(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R,
                    const void* x_flat, const void* y_flat, double alpha, const void* grad_flat,
                    double damp, int eq, int symm, double factors_norm, void* x_new_flat,
                    void* F_flat, double* scal, hipStream_t stream);
+
+/* ALS mode update F_l = N_l pinv(P1_l) (flat layouts; P1 as written by tfx_gramians, N_l the
+ * MTTKRP block of mode `mode`), which equals the reference's T_(l) pinv(KR_l^T)
+ * (Alternating_Least_Squares.py:206-231): pinv(KR^T) = KR (KR^T KR)^+ and KR^T KR = P1_l.
+ * P1_l (R <= 80) is inverted by Gauss-Jordan elimination in FP64; when it is singular to working
+ * precision the pseudo-inverse comes from its Jacobi eigendecomposition.  *status (device int)
+ * receives 0, or 1 when the pseudo-inverse path was taken.  Only the mode's block of F_flat is
+ * written. */
+int tfx_als_solve(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, int mode,
+                  const void* P1, const void* N_flat, void* F_flat, int* status,
+                  hipStream_t stream);
 
 /* dots[0..3] = [y.y, y.g, g.g, g.Jg] in FP64 (the inner products of dogleg, GaussNewton.py:766-795). */
 int tfx_dogleg_dots(tfx_ctx* ctx, int dtype, int64_t n, const void* y, const void* g,

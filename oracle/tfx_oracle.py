@@ -667,13 +667,23 @@ def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
         inner_method = inner_method[it]
     if inner_method == 'cg':
         cg_maxiter = 1 + (L - 2) * int(cg_factor * np.random.randint(1 + it ** 0.4, 2 + it ** 0.9))
-    elif inner_method != 'cg_static':
+    elif inner_method not in ('cg_static', 'als'):
         sys.exit("Wrong inner method name. Must be 'cg', 'cg_static', 'als' or 'direct'.")
-    y, grad, JT_J_grad, itn, rn = cg(T, factors, damp, cg_maxiter, cg_tol)
+    if inner_method == 'als':
+        # GaussNewton.py:273-276: one ALS sweep replaces the step; y is zeroed, the gradient is
+        # reported as [nan] and the dogleg is never tried
+        factors = als_iteration([unfold(T, l + 1) for l in range(L)], factors)
+        x = np.concatenate([f.flatten('F') for f in factors])
+        y = np.zeros(x.size)
+        grad, JT_J_grad, itn, rn = np.array([np.nan]), None, '-', np.nan
+    else:
+        y, grad, JT_J_grad, itn, rn = cg(T, factors, damp, cg_maxiter, cg_tol)
     x = x + y
     factors = x2cpd(x, factors)
     factors = transform(factors, symm, factors_norm)
     error = residual_error(T, factors, Tsize)
+    if inner_method == 'als':
+        return factors, x, y, grad, itn, Tsize * error, error
     if it > 3 and error > tol_jump * old_error:
         x = x - y
         factors, x, y, error = compute_dogleg_steps(T, Tsize, factors, grad, JT_J_grad, x, y, error,
@@ -751,12 +761,91 @@ def dGN(T, factors, R, options, trace=None):
 
 
 # ----------------------------------------------------------------------------------------------
+# Alternating least squares (Alternating_Least_Squares.py)
+# ----------------------------------------------------------------------------------------------
+
+def als_iteration(Tl, factors):
+    """One Gauss-Seidel sweep over the modes: F_l <- T_(l) pinv(KR_l^T), with KR_l the
+    Khatri-Rao product of the other factors in reversed mode order, each mode using the modes
+    already updated in this sweep.  `factors` is modified in place and returned.  Follows
+    Alternating_Least_Squares.py:206-231 (no fixed mode)."""
+    L = len(factors)
+    for l in range(L):
+        itr = [m for m in reversed(range(L)) if m != l]
+        M = factors[itr[0]]
+        for ll in range(L - 2):
+            M = khatri_rao(M, factors[itr[ll + 1]])
+        factors[l] = Tl[l] @ np.linalg.pinv(M.T)
+    return factors
+
+
+def als(T, factors, R, options):
+    """ALS iterations with the reference's stopping rules.  Follows
+    Alternating_Least_Squares.py:18-203 (no fixed mode): the step size is the absolute
+    ||x - old_x|| of the unbalanced point taken before `transform`, the 'gradient' is
+    |old_error - error| / step size, and the LAST factors are returned, not the best."""
+    maxiter = options.maxiter
+    tol, tol_step, tol_improv, tol_grad = options.tol, options.tol_step, options.tol_improv, options.tol_grad
+    L = len(factors)
+    Tsize = norm(T)
+    error = 1
+    stop = 5
+    const = 1 + int(maxiter / 10)
+    x = np.concatenate([f.flatten('F') for f in factors])
+    step_sizes = np.empty(maxiter)
+    errors = np.empty(maxiter)
+    improv = np.empty(maxiter)
+    gradients = np.empty(maxiter)
+    Tl = [unfold(T, l + 1) for l in range(L)]
+    it = 0
+    for it in range(maxiter):
+        old_x, old_error = x, error
+        factors = als_iteration(Tl, factors)
+        x = np.concatenate([f.flatten('F') for f in factors])
+        factors = transform(factors, options.symm, options.factors_norm)
+        error = fastnorm(Tl[0], cpd2unfold1(factors)) / Tsize
+        step_sizes[it] = norm(x - old_x)
+        errors[it] = error
+        with np.errstate(divide='ignore', invalid='ignore'):
+            gradients[it] = np.float64(np.abs(old_error - error)) / step_sizes[it]
+        improv[it] = errors[it] if it == 0 else np.abs(errors[it - 1] - errors[it])
+        if it > 1:
+            if errors[it] < tol:
+                stop = 0
+                break
+            if step_sizes[it] < tol_step:
+                stop = 1
+                break
+            if improv[it] < tol_improv:
+                stop = 2
+                break
+            if gradients[it] < tol_grad:
+                stop = 3
+                break
+            if it > 2 * const and it % const == 0:
+                mean1 = np.mean(errors[it - 2 * const: it - const])
+                mean2 = np.mean(errors[it - const: it])
+                if mean1 - mean2 <= tol_improv:
+                    stop = 4
+                    break
+                mean3 = np.mean(improv[it - const: it])
+                if mean3 < 1e-3 * mean2:
+                    stop = 7
+                    break
+            if error > max(1, Tsize ** 2) / (1e-16 + tol):
+                stop = 6
+                break
+    n = it + 1
+    return factors, step_sizes[:n], errors[:n], improv[:n], gradients[:n], stop
+
+
+# ----------------------------------------------------------------------------------------------
 # Drivers (TensorFox.py)
 # ----------------------------------------------------------------------------------------------
 
 def tricpd(T, R, options, stages=None):
-    """sort -> mlsvd -> init -> dGN -> U*W -> optional refinement -> final error -> unsort.
-    Follows TensorFox.py:424-655 (dense, method 'dGN').  `stages`, if a dict, receives
+    """sort -> mlsvd -> init -> dGN or ALS -> U*W -> optional refinement -> final error -> unsort.
+    Follows TensorFox.py:424-655 (dense, methods 'dGN' and 'als', :563-568, :599-604).  `stages`, if a dict, receives
     the intermediate results (core, U, init factors) for tests."""
     L = T.ndim
     T, ordering = sort_dims(T)
@@ -769,12 +858,13 @@ def tricpd(T, R, options, stages=None):
     init = starting_point(S, U, R, ordering, options)
     if stages is not None:
         stages.update(S=S, U=U, init=deepcopy(init), sigmas=sigmas, ordering=ordering)
-    factors, ss_m, e_m, im_m, g_m, stop_m = dGN(S, init, R, options)
+    solver = als if options.method == 'als' else dGN
+    factors, ss_m, e_m, im_m, g_m, stop_m = solver(S, init, R, options)
     factors = [U[l] @ factors[l] for l in range(L)]
     if stages is not None:
         stages.update(main_factors=deepcopy(factors))
     if options.refine:
-        factors, ss_r, e_r, im_r, g_r, stop_r = dGN(np.asarray(T), factors, R, options)
+        factors, ss_r, e_r, im_r, g_r, stop_r = solver(np.asarray(T), factors, R, options)
     else:
         ss_r, e_r, im_r, g_r, stop_r = np.array([0]), np.array([0]), np.array([0]), np.array([0]), 8
     T1_approx = cpd2unfold1(factors)
@@ -791,12 +881,12 @@ def tricpd(T, R, options, stages=None):
 
 
 def cpd(T, R, options=False, stages=None):
-    """Drop-in dense CPD.  Follows TensorFox.py:69-365 (dGN branch, dense input)."""
+    """Drop-in dense CPD.  Follows TensorFox.py:69-365 (dGN and ALS branches, dense input)."""
     dims = T.shape
     options = make_options(options)
     consistency(R, dims, options)
-    if options.method != 'dGN':
-        raise NotImplementedError("the oracle covers method='dGN' only")
+    if options.method not in ('dGN', 'als'):
+        raise NotImplementedError("the oracle covers methods 'dGN' and 'als' only")
     return tricpd(np.asarray(T, dtype=np.float64), R, options, stages=stages)
 
 

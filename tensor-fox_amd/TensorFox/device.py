@@ -90,8 +90,13 @@ class Device:
 
     def dgn_update(self, dims, R, x, y, alpha, grad, damp, eq, symm, factors_norm, x_new, F, scal):
         self._call('tfx_dgn_update', dtype_code(x), len(dims), _lib.dims_arg(dims), int(R), _p(x),
-                   _p(y), float(alpha), _p(grad), float(damp), int(bool(eq)), int(bool(symm)),
+                   _p(y), float(alpha), _p(grad), float(damp), int(bool(eq)), int(symm),
                    float(factors_norm), _p(x_new), _p(F), _p(scal), self.stream())
+
+    def als_solve(self, dims, R, mode, P1, N, F, status):
+        """F's block of `mode` <- N's block times pinv(P1[mode]); status: int32 device scalar."""
+        self._call('tfx_als_solve', dtype_code(F), len(dims), _lib.dims_arg(dims), int(R), int(mode),
+                   _p(P1), _p(N), _p(F), _p(status), self.stream())
 
     def dogleg_dots(self, y, g, jg, out):
         self._call('tfx_dogleg_dots', dtype_code(y), ctypes.c_int64(y.numel()), _p(y), _p(g), _p(jg),

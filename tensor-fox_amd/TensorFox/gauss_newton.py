@@ -1,4 +1,5 @@
-"""Damped Gauss-Newton on the GPU (GaussNewton.py:29-251 and compute_step :254-312).
+"""Damped Gauss-Newton on the GPU (GaussNewton.py:29-251 and compute_step :254-312), with the
+inner methods 'cg', 'cg_static' and 'als'.
 
 Per iteration the device runs: Gram/Hadamard products and regularization, the fused MTTKRP,
 minus the gradient, the persistent preconditioned CG, the point update with equalization, and
@@ -13,6 +14,8 @@ import numpy as np
 import torch
 
 from . import _lib
+from .als import als_sweep
+from .display import print_header, print_row
 
 _S_CG, _S_UPD, _S_ERR = 0, 2, 6     # offsets in the per-iteration scalar vector
 
@@ -39,6 +42,7 @@ class Problem:
         self.F = e(n, wdt)
         self.best_F = e(n, wdt)
         self.scal = dev.zeros(16, torch.float64)
+        self.als_status = dev.zeros(1, torch.int32)
         self.dogleg_calls = 0
         self.tmp_scal = dev.zeros(16, torch.float64)
 
@@ -103,7 +107,8 @@ def _dogleg_coeffs(dots, delta):
 
 def _dogleg_steps(prob, x, y, error, Tsize, damp, symm, factors_norm):
     """compute_dogleg_steps (GaussNewton.py:713-763) with device vectors.  `x` already had y
-    removed.  Returns (best_F, best_x, best_y, best_error) as device tensors / float."""
+    removed; `symm` is the update's transform form (_symm_mode).  Returns (best_F, best_x,
+    best_y, best_error) as device tensors / float."""
     dev = prob.dev
     best_x, best_y, best_error = x.clone(), y.clone(), error
     best_F = prob.F.clone()
@@ -133,9 +138,49 @@ def _dogleg_steps(prob, x, y, error, Tsize, damp, symm, factors_norm):
     return best_F, best_x, best_y, best_error
 
 
-def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
+def _stop_code(it, errors, step_sizes, improv, gradients, tol, tol_step, tol_improv, tol_grad,
+               const, error, Tsize):
+    """Stopping conditions of GaussNewton.py:215-242 (None: continue)."""
+    if it <= 1:
+        return None
+    if errors[it] < tol:
+        return 0
+    if step_sizes[it] < tol_step:
+        return 1
+    if improv[it] < tol_improv:
+        return 2
+    if gradients[it] < tol_grad:
+        return 3
+    if it > 2 * const and it % const == 0:
+        mean1 = np.mean(errors[it - 2 * const: it - const])
+        mean2 = np.mean(errors[it - const: it])
+        if mean1 - mean2 <= tol_improv:
+            return 4
+        mean3 = np.mean(improv[it - const: it])
+        if mean3 < 1e-3 * mean2:
+            return 7
+    if error > max(1, Tsize ** 2) / (1e-16 + tol):
+        return 6
+    return None
+
+
+def _symm_mode(symm, factors_norm, aliased):
+    """Which form of the reference's symmetric transform an update reproduces: 0 none; 1 the
+    average of separate factor arrays; 2 the factor list whose entries all reference one array,
+    as left by an earlier symmetric transform without factors_norm (Conversion.py:432-438)."""
+    if not symm:
+        return 0
+    if factors_norm > 0:
+        return 1
+    return 2 if aliased else 1
+
+
+def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
     """Damped Gauss-Newton from the flat factors F0 (device, working dtype).  Returns
-    (best flat factors, step_sizes, errors, improv, gradients, stop) like GaussNewton.py:29-251."""
+    (best flat factors, step_sizes, errors, improv, gradients, stop) like GaussNewton.py:29-251.
+    `aliased`: the starting factors came out of a symmetric transform (random initialization
+    with symm, Initialization.py:74-83), so the reference's factor list already holds one shared
+    array."""
     dev = prob.dev
     dims, R = prob.dims, prob.R
     L = len(dims)
@@ -144,6 +189,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
     tol, tol_step, tol_improv, tol_grad = options.tol, options.tol_step, options.tol_improv, options.tol_grad
     tol_jump = options.tol_jump
     symm, factors_norm = options.symm, options.factors_norm
+    display = options.display
     error = 1
     best_error = np.inf
     stop = 5
@@ -160,6 +206,8 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
     errors = np.zeros(maxiter)
     improv = np.zeros(maxiter)
     gradients = np.zeros(maxiter)
+    if display > 1:
+        print_header(display)
     it = 0
     for it in range(maxiter):
         old_error = error
@@ -168,11 +216,50 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
             cap = 1 + (L - 2) * int(options.cg_factor * np.random.randint(1 + it ** 0.4, 2 + it ** 0.9))
         elif inner == 'cg_static':
             cap = options.cg_maxiter
-        elif inner in ('als', 'direct'):
-            raise NotImplementedError("inner_method '%s' is not on the MI355X path; use 'cg' or "
-                                      "'cg_static'" % inner)
-        else:
+        elif inner == 'direct':
+            raise NotImplementedError("inner_method 'direct' is not on the MI355X path; use 'cg', "
+                                      "'cg_static' or 'als'")
+        elif inner != 'als':
             sys.exit("Wrong inner method name. Must be 'cg', 'cg_static', 'als' or 'direct'.")
+        if inner == 'als':
+            # --- compute_step: one ALS sweep replaces the step (GaussNewton.py:273-276); y is
+            # zeroed, x is the swept factors, then x2cpd + equalize + transform as below -------
+            als_sweep(prob, prob.F)
+            prob.y.zero_()
+            x_als = torch.empty_like(x)
+            x_als.copy_(prob.F)
+            dev.dgn_update(dims, R, x_als, prob.y, 1.0, prob.y, damp, True,
+                           _symm_mode(symm, factors_norm, False), factors_norm, x_spare, prob.F,
+                           prob.scal[_S_UPD:_S_UPD + 4])
+            aliased = True
+            prob.error_sq(prob.F, prob.scal[_S_ERR:_S_ERR + 1])
+            d = torch.empty_like(x)
+            dev.axpby(1.0, x_spare, -1.0, x, d)
+            dev.dogleg_dots(d, x, x, prob.tmp_scal[0:4])
+            h = prob.scal[:8].cpu().numpy()
+            hs = prob.tmp_scal[0:4].cpu().numpy()
+            x, x_spare = x_spare, x
+            error = float(np.sqrt(h[_S_ERR])) / Tsize
+            gain_ratio = np.nan      # the reference's gradient is [nan] here (GaussNewton.py:309)
+            if trace is not None:
+                trace.append((error, damp, gain_ratio, 0))
+            damp = _update_damp(damp, init_damp, gain_ratio, it)
+            if error < best_error:
+                best_error = error
+                prob.best_F.copy_(prob.F)
+            errors[it] = error
+            step_sizes[it] = np.sqrt(hs[0]) / np.sqrt(hs[2])
+            gradients[it] = np.nan
+            improv[it] = errors[it] if it == 0 else abs(errors[it] - errors[it - 1])
+            if display > 1:
+                print_row(display, it, errors[it], step_sizes[it], improv[it], gradients[it],
+                          Tsize * error, '-')
+            code = _stop_code(it, errors, step_sizes, improv, gradients, tol, tol_step, tol_improv,
+                              tol_grad, const, error, Tsize)
+            if code is not None:
+                stop = code
+                break
+            continue
         # --- compute_step: cg ------------------------------------------------------------------
         dev.gramians(dims, R, prob.F, prob.Gr, prob.P1, prob.P2, prob.gamma)
         dev.mttkrp(prob.T, dims, prob.F, R, prob.N, prec=_mttkrp_prec(prob))
@@ -180,8 +267,10 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
         dev.cg(dims, R, prob.F, prob.P1, prob.P2, prob.gamma, damp, prob.grad, cap, options.cg_tol,
                prob.y, prob.jtjg, prob.scal[_S_CG:_S_CG + 2])
         # --- x = x + y; factors = transform(equalize(x2cpd(x))); error ----------------------------
-        dev.dgn_update(dims, R, x, prob.y, 1.0, prob.grad, damp, True, symm, factors_norm, x_spare,
-                       prob.F, prob.scal[_S_UPD:_S_UPD + 4])
+        dev.dgn_update(dims, R, x, prob.y, 1.0, prob.grad, damp, True,
+                       _symm_mode(symm, factors_norm, aliased), factors_norm, x_spare, prob.F,
+                       prob.scal[_S_UPD:_S_UPD + 4])
+        aliased = True
         prob.error_sq(prob.F, prob.scal[_S_ERR:_S_ERR + 1])
         h = prob.scal[:8].cpu().numpy()
         old_x = x
@@ -195,8 +284,8 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
             prob.dogleg_calls += 1
             x_back = torch.empty_like(x)
             dev.axpby(1.0, x, -1.0, prob.y, x_back)
-            bF, bx, by, error = _dogleg_steps(prob, x_back, prob.y.clone(), error, Tsize, damp, symm,
-                                              factors_norm)
+            bF, bx, by, error = _dogleg_steps(prob, x_back, prob.y.clone(), error, Tsize, damp,
+                                              _symm_mode(symm, factors_norm, True), factors_norm)
             prob.F.copy_(bF)
             x.copy_(bx)
             prob.y.copy_(by)
@@ -219,31 +308,13 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None):
         step_sizes[it] = np.sqrt(dx2) / np.sqrt(x2)
         gradients[it] = gmax
         improv[it] = errors[it] if it == 0 else abs(errors[it] - errors[it - 1])
-        if it > 1:
-            if errors[it] < tol:
-                stop = 0
-                break
-            if step_sizes[it] < tol_step:
-                stop = 1
-                break
-            if improv[it] < tol_improv:
-                stop = 2
-                break
-            if gradients[it] < tol_grad:
-                stop = 3
-                break
-            if it > 2 * const and it % const == 0:
-                mean1 = np.mean(errors[it - 2 * const: it - const])
-                mean2 = np.mean(errors[it - const: it])
-                if mean1 - mean2 <= tol_improv:
-                    stop = 4
-                    break
-                mean3 = np.mean(improv[it - const: it])
-                if mean3 < 1e-3 * mean2:
-                    stop = 7
-                    break
-            if error > max(1, Tsize ** 2) / (1e-16 + tol):
-                stop = 6
-                break
+        if display > 1:
+            print_row(display, it, errors[it], step_sizes[it], improv[it], gradients[it],
+                      float(h[_S_CG + 1]), int(h[_S_CG]))
+        code = _stop_code(it, errors, step_sizes, improv, gradients, tol, tol_step, tol_improv,
+                          tol_grad, const, error, Tsize)
+        if code is not None:
+            stop = code
+            break
     k = it + 1
     return prob.best_F.clone(), step_sizes[:k], errors[:k], improv[:k], gradients[:k], stop

@@ -1,6 +1,6 @@
 """Drop-in `cpd(T, R, options)` for dense tensors on one MI355X (TensorFox.py:69-365 -> tricpd
-:424-655): sort dims -> MLSVD -> initialization -> dGN on the core -> uncompression -> optional
-refinement on the full tensor -> final error -> original mode order.
+:424-655): sort dims -> MLSVD -> initialization -> dGN (or ALS, method='als') on the core ->
+uncompression -> optional refinement on the full tensor -> final error -> original mode order.
 
 Precision forms (option `precision`):
   'fp64'  -- the tensor is stored in FP64 and every product is FP64 (default for float64 input);
@@ -20,6 +20,7 @@ import torch
 
 from . import _lib
 from .auxiliar import Output, consistency, make_options, sort_dims, unsort_dims
+from .als import als
 from .compression import mlsvd
 from .device import get_device
 from .gauss_newton import Problem, dGN
@@ -192,11 +193,15 @@ def tricpd(T, R, options):
     t2 = time.perf_counter()
     timings['init'] = t2 - t1
 
-    # main dGN on the core
-    if options.method != 'dGN':
+    # main stage on the core: dGN, or ALS (TensorFox.py:563-568)
+    if options.method not in ('dGN', 'als'):
         raise NotImplementedError("method '%s' is not on the MI355X path" % options.method)
     prob = Problem(dev, S, S_dims, R, wdt, prec)
-    Fm, ss_m, e_m, im_m, g_m, stop_m = dGN(prob, F0, options, Ssize, S_mean_abs)
+    if options.method == 'als':
+        Fm, ss_m, e_m, im_m, g_m, stop_m = als(prob, F0, options, Ssize)
+    else:
+        Fm, ss_m, e_m, im_m, g_m, stop_m = dGN(prob, F0, options, Ssize, S_mean_abs,
+                                               aliased=type(init) != list)
     _sync()
     t3 = time.perf_counter()
     timings['main'] = t3 - t2
@@ -221,7 +226,10 @@ def tricpd(T, R, options):
         # matrix cores (split FP32 operands), the FP64 form with FP64 FMA
         rprec = _lib.TFX_PREC_SPLIT16 if precision == 'mixed' else _lib.TFX_PREC_NATIVE
         probT = Problem(dev, Tdev, dims, R, wdt, rprec)
-        Ffull, ss_r, e_r, im_r, g_r, stop_r = dGN(probT, Ffull, options, Tsize, mean_abs)
+        if options.method == 'als':
+            Ffull, ss_r, e_r, im_r, g_r, stop_r = als(probT, Ffull, options, Tsize)
+        else:
+            Ffull, ss_r, e_r, im_r, g_r, stop_r = dGN(probT, Ffull, options, Tsize, mean_abs)
     else:
         ss_r, e_r, im_r, g_r, stop_r = np.array([0]), np.array([0]), np.array([0]), np.array([0]), 8
     _sync()
@@ -264,6 +272,7 @@ def cpd(T, R, options=False):
     dims_orig = tuple(T.shape)
     options = make_options(options)
     consistency(R, dims_orig, options)
-    if options.method == 'dGN':
+    if options.method in ('dGN', 'als'):
         return tricpd(T, R, options)
-    raise NotImplementedError("method '%s' is not on the MI355X path (dense 'dGN' only)" % options.method)
+    raise NotImplementedError("method '%s' is not on the MI355X path (dense 'dGN' and 'als' only)"
+                              % options.method)

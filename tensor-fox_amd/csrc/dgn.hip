@@ -473,7 +473,7 @@ __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restric
     scal[3] = gmax;
   }
   __syncthreads();
-  if (eq) {
+  if (eq && symm != 2) {
     // column r of mode l scaled by (prod_k ||f_k^r||)^(1/L) / ||f_l^r|| (Conversion.py:361-387);
     // a column whose norm product is zero is left as it is (scale 0 marks it)
     double* sc = cn + L * R;
@@ -508,13 +508,46 @@ __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restric
     }
     __syncthreads();
   }
-  if (symm) {
+  if (symm == 1) {
+    // transform on separate factor arrays (Conversion.py:432-437): s = F_0 accumulates the other
+    // modes left to right, F_0 = s / L, every mode takes F_0
     const int64_t I = s.d[0];
     for (int64_t e = t; e < (int64_t)R * I; e += nt) {
       S acc = F[e];
       for (int l = 1; l < L; ++l) acc += F[s.off[l] + e];
       acc = acc / (S)L;
       for (int l = 0; l < L; ++l) F[s.off[l] + e] = acc;
+    }
+    __syncthreads();
+  } else if (symm == 2) {
+    // the reference's factor list after one symmetric transform holds L references to a single
+    // array (Conversion.py:437-438).  Then x2cpd writes every mode's block into that array in
+    // turn, so only the last mode's block remains (Conversion.py:39-44); equalize sees L equal
+    // column norms n and scales the column L times by (n^L)^(1/L) / n (Conversion.py:378-385);
+    // transform's in-place `s += factors[l]` doubles the array L - 1 times before the division
+    // by L (Conversion.py:433-436).  The result is copied to every mode.
+    const int64_t I = s.d[0];
+    const int64_t last = s.off[L - 1];
+    double* sc = cn + L * R;
+    if (eq) {
+      for (int r = t; r < R; r += nt) {
+        const double nr = cn[(L - 1) * R + r];
+        double prod = 1.0;
+        for (int k = 0; k < L; ++k) prod *= nr;
+        sc[r] = prod != 0.0 ? pow(prod, 1.0 / L) / nr : 0.0;
+      }
+      __syncthreads();
+    }
+    const double dbl = (double)(1 << (L - 1));
+    for (int64_t e = t; e < (int64_t)R * I; e += nt) {
+      S v = F[last + e];
+      if (eq) {
+        const double f = sc[e / I];
+        if (f != 0.0)
+          for (int k = 0; k < L; ++k) v = (S)(f * (double)v);
+      }
+      v = (S)((double)v * dbl) / (S)L;
+      for (int l = 0; l < L; ++l) F[s.off[l] + e] = v;
     }
     __syncthreads();
   }

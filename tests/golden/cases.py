@@ -14,6 +14,13 @@ CASE_OPTIONS = {
     'cjump_dogleg': dict(tol_jump=0.9, maxiter=30),
     'cjump_sched': dict(init_damp=[0.5] * 25, maxiter=25,
                         inner_method=['cg'] * 10 + ['cg_static'] * 15),
+    'csymm': dict(symm=True, maxiter=40),
+    'csymm_refine': dict(symm=True, refine=True, maxiter=30),
+    'csymm_norm': dict(symm=True, factors_norm=2.0, maxiter=30),
+    'cals': dict(method='als', maxiter=60),
+    'cals_refine': dict(method='als', refine=True, maxiter=50),
+    'c4_als': dict(method='als', maxiter=40),
+    'cinner_als': dict(maxiter=20, inner_method=['cg'] * 4 + ['als'] * 6 + ['cg'] * 10),
 }
 
 # Stop codes decided by a roundoff-level tie (mean1 - mean2 vs a zeroed tol_improv on a

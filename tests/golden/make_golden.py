@@ -110,6 +110,19 @@ def main(ref_root):
             out['mlsvd_%s_U%d' % (method, l)] = U[l]
             out['mlsvd_%s_sigma%d' % (method, l)] = sig[l]
 
+    # --- one ALS sweep on fixed inputs (Alternating_Least_Squares.py:229-253) -----------------
+    import TensorFox.Alternating_Least_Squares as alsq
+    rs = np.random.RandomState(31)
+    dims, R = (4, 5, 6), 3
+    T = rs.randn(*dims)
+    F = [rs.randn(d, R) for d in dims]
+    out['als_T'] = T
+    for l in range(3):
+        out['als_F%d' % l] = F[l].copy()
+    Fo = alsq.als_iteration([cnv.unfold(T, l + 1) for l in range(3)], [f.copy() for f in F], -1)
+    for l in range(3):
+        out['als_out%d' % l] = Fo[l]
+
     # --- whole cpd trajectories -------------------------------------------------------------
     T222 = np.zeros((2, 2, 2))
     s = 0
@@ -126,6 +139,11 @@ def main(ref_root):
     T4 = cnv.cpd2tens([rs.randn(d, 3) for d in (5, 4, 6, 3)]) + 1e-4 * rs.randn(5, 4, 6, 3)
     rs = np.random.RandomState(24)
     T_jump = cnv.cpd2tens([rs.randn(d, 3) for d in (6, 5, 7)]) + 1e-2 * rs.randn(6, 5, 7)
+    rs = np.random.RandomState(25)
+    a = rs.randn(6, 2)
+    E = rs.randn(6, 6, 6)
+    E = sum(np.transpose(E, p) for p in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)))
+    T_symm = cnv.cpd2tens([a, a, a]) + 1e-3 * E
 
     cases = [
         ('c222_default', T222, 3, dict(), 1),
@@ -139,6 +157,13 @@ def main(ref_root):
         ('cjump_dogleg', T_jump, 3, dict(tol_jump=0.9, maxiter=30), 9),
         ('cjump_sched', T_jump, 3, dict(init_damp=[0.5] * 25, maxiter=25,
                                           inner_method=['cg'] * 10 + ['cg_static'] * 15), 10),
+        ('csymm', T_symm, 2, dict(symm=True, maxiter=40), 11),
+        ('csymm_refine', T_symm, 2, dict(symm=True, refine=True, maxiter=30), 12),
+        ('csymm_norm', T_symm, 2, dict(symm=True, factors_norm=2.0, maxiter=30), 13),
+        ('cals', T_big, 4, dict(method='als', maxiter=60), 14),
+        ('cals_refine', T567, 2, dict(method='als', refine=True, maxiter=50), 15),
+        ('c4_als', T4, 3, dict(method='als', maxiter=40), 16),
+        ('cinner_als', T_big, 4, dict(maxiter=20, inner_method=['cg'] * 4 + ['als'] * 6 + ['cg'] * 10), 17),
     ]
     names = []
     for name, T, R, kw, seed in cases:

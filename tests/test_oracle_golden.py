@@ -1,7 +1,8 @@
 """Pins the FP64 oracle against vectors produced by the reference itself (tests/golden/make_golden.py):
-CG on fixed inputs, seeded randomized SVD and MLSVD, and ten seeded cpd trajectories covering
-defaults, user init, refinement, cg_static, classic MLSVD, order 4, the dogleg path and scheduled
-damping / inner methods."""
+CG and one ALS sweep on fixed inputs, seeded randomized SVD and MLSVD, and seventeen seeded cpd
+trajectories covering defaults, user init, refinement, cg_static, classic MLSVD, order 4, the
+dogleg path, scheduled damping / inner methods, symmetric factors (shared-array and factors_norm
+forms), method 'als' and inner_method 'als'."""
 
 import numpy as np
 import pytest
@@ -101,3 +102,13 @@ def test_cpd_trajectory_matches_reference(golden, name):
         k = min(10, e.size)
         assert np.allclose(e[:k], e_ref[:k], rtol=1e-9, atol=1e-15)
         assert np.allclose(e, e_ref, rtol=0.5, atol=1e-14)
+
+
+def test_als_iteration_matches_reference(golden):
+    """One ALS sweep on fixed inputs (Alternating_Least_Squares.py:206-231)."""
+    T = golden['als_T']
+    F = [golden['als_F%d' % l].copy() for l in range(3)]
+    out = o.als_iteration([o.unfold(T, l + 1) for l in range(3)], F)
+    for l in range(3):
+        ref = golden['als_out%d' % l]
+        assert np.abs(out[l] - ref).max() <= 1e-12 * np.abs(ref).max()

@@ -322,3 +322,57 @@ def test_make_options_defaults():
     class u:
         gpu = True
     assert o.make_options(u).mlsvd_method == 'gpu'
+
+
+def test_als_sweep_is_sequential_least_squares():
+    """Each mode update of an ALS sweep is the least-squares solution min ||T_(l) - F KR_l^T||
+    given the modes already updated (lstsq on the explicit Khatri-Rao product)."""
+    rs = np.random.RandomState(4)
+    dims, R = (5, 6, 7), 3
+    T = rs.randn(*dims)
+    F = [rs.randn(d, R) for d in dims]
+    G = [f.copy() for f in F]
+    out = o.als_iteration([o.unfold(T, l + 1) for l in range(3)], F)
+    for l in range(3):
+        other = [m for m in reversed(range(3)) if m != l]
+        KR = o.khatri_rao(G[other[0]], G[other[1]])
+        sol = np.linalg.lstsq(KR, o.unfold(T, l + 1).T, rcond=None)[0].T
+        assert np.allclose(out[l], sol, atol=1e-11)
+        G[l] = sol
+
+
+def test_als_recovers_exact_low_rank():
+    rs = np.random.RandomState(5)
+    F = [rs.randn(d, 3) for d in (8, 9, 10)]
+    T = o.cpd2tens(F)
+
+    class opt:
+        method = 'als'
+        maxiter = 400
+        tol_step = tol_improv = tol_grad = 0
+    np.random.seed(1)
+    _, out = o.cpd(T, 3, opt)
+    assert out.rel_error < 1e-9
+
+
+def test_transform_symmetric_shared_array_closed_form():
+    """After one symmetric transform without factors_norm the reference's factor list holds L
+    references to one array; x2cpd then leaves only the last mode's block in it and transform
+    doubles it L - 1 times before dividing by L (Conversion.py:21-49, 412-444)."""
+    rs = np.random.RandomState(6)
+    L, I, R = 3, 4, 2
+    F = o.transform([rs.randn(I, R) for _ in range(L)], True, 0)
+    assert all(F[l] is F[0] for l in range(L))
+    x = rs.randn(L * I * R)
+    last = x[(L - 1) * I * R:].reshape(R, I).T
+    G = o.x2cpd(x, F, eq=False)
+    G = o.transform(G, True, 0)
+    assert np.allclose(G[0], last * 2 ** (L - 1) / L, rtol=1e-15, atol=0)
+    assert all(G[l] is G[0] for l in range(L))
+    # with factors_norm the arrays are separate, so the symmetric transform is the plain mean
+    H = o.transform([rs.randn(I, R) for _ in range(L)], True, 1.5)
+    assert all(H[l] is not H[0] for l in range(1, L))
+    blocks = [x[l * I * R:(l + 1) * I * R].reshape(R, I).T for l in range(L)]
+    H = o.transform(o.x2cpd(x, H, eq=False), True, 1.5)
+    mean = sum(blocks) / L
+    assert np.allclose(H[1], 1.5 * mean / np.linalg.norm(mean), rtol=1e-14)
