@@ -65,6 +65,15 @@ int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L, const int6
                      const void* F_flat, int f_dtype, int R, unsigned mode_mask, void* out_flat,
                      int prec, hipStream_t stream);
 
+/* Kernel family tfx_mttkrp_mixed uses for these arguments (host only, no GPU, no context):
+ * TFX_PATH_SPLIT16_MFMA (bf16x3 tiles; needs an FP32 tensor whose last dimension K is a multiple
+ * of 4 with 64 K 4 < 2^31 bytes), TFX_PATH_F64_MFMA (FP64 tensors whose selected unfoldings have
+ * fewer than 2^31 columns) or TFX_PATH_FMA; negative on bad arguments.  No reference
+ * counterpart: the reference always multiplies unfoldings with BLAS (GaussNewton.py:423). */
+enum { TFX_PATH_SPLIT16_MFMA = 0, TFX_PATH_FMA = 1, TFX_PATH_F64_MFMA = 2 };
+int tfx_mttkrp_plan(int t_dtype, int f_dtype, int prec, int L, const int64_t* dims, int R,
+                    unsigned mode_mask);
+
 /* ||T - [[F]]||_F^2, accumulated in FP64, written to *out_sqnorm (device).
  * Replaces cpd2unfold1 + fastnorm (Conversion.py:117-141, Critical.py:14-22). */
 int tfx_cpd_residual(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,

@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libtfxhip.so')
 
 TFX_F32, TFX_F64 = 0, 1
+TFX_MAXL = 8          # largest tensor order of the library (include/tfxhip.h)
 TFX_PREC_NATIVE, TFX_PREC_SPLIT16 = 0, 1
 
 _c_int, _c_i64, _c_dbl, _c_vp, _c_uint = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_uint
@@ -54,6 +55,13 @@ SIGNATURES = {
 }
 
 
+# host-only entry points (no context argument)
+HOST_SIGNATURES = {
+    'tfx_mttkrp_plan': [_c_int, _c_int, _c_int, _c_int, _I64P, _c_int, _c_uint],
+}
+TFX_PATH_SPLIT16_MFMA, TFX_PATH_FMA, TFX_PATH_F64_MFMA = 0, 1, 2
+
+
 class BatchOpts(ctypes.Structure):
     """tfx_batch_opts (include/tfxhip.h)."""
     _fields_ = [('maxiter', _c_int), ('symm', _c_int), ('tol', _c_dbl), ('tol_step', _c_dbl),
@@ -74,7 +82,7 @@ def load():
             raise ImportError('libtfxhip.so not found at %s; build it with '
                               '`python tensor-fox_amd/build_lib.py`' % LIB_PATH)
         lib = ctypes.CDLL(LIB_PATH)
-        for name, args in SIGNATURES.items():
+        for name, args in list(SIGNATURES.items()) + list(HOST_SIGNATURES.items()):
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int

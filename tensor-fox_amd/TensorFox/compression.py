@@ -15,6 +15,9 @@ import torch
 from . import _lib
 
 SUBSPACE_ITERS = 2
+# largest mode size of the device MLSVD: the mode Gram is formed as an I x I matrix and its
+# eigensolver stages I x p blocks (tfx_gram_mode, tfx_eig_top)
+MAX_MODE_DIM = 8192
 _PRIVATE_RNG_SEED = 20240611
 
 

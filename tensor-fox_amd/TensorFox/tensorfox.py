@@ -21,7 +21,7 @@ import torch
 from . import _lib
 from .auxiliar import Output, consistency, make_options, sort_dims, unsort_dims
 from .als import als
-from .compression import mlsvd
+from .compression import MAX_MODE_DIM, mlsvd
 from .device import get_device
 from .gauss_newton import Problem, dGN
 
@@ -265,6 +265,21 @@ def tricpd(T, R, options):
     return factors, out
 
 
+def check_supported(dims):
+    """Shapes that pass the reference's validation but not this path's limits raise
+    NotImplementedError before anything is uploaded: orders above the library's TFX_MAXL (the
+    reference accepts up to 12, Auxiliar.py:31-33) and modes longer than the device MLSVD's
+    Gram size."""
+    L = len(dims)
+    if L > _lib.TFX_MAXL:
+        raise NotImplementedError('tensors of order %d are not on the MI355X path (orders 2-%d)'
+                                  % (L, _lib.TFX_MAXL))
+    big = [int(d) for d in dims if int(d) > MAX_MODE_DIM]
+    if big:
+        raise NotImplementedError('mode sizes above %d (%s) are not on the MI355X path: the MLSVD '
+                                  'forms the I x I Gram matrix of every mode' % (MAX_MODE_DIM, big))
+
+
 def cpd(T, R, options=False):
     """Canonical polyadic decomposition of a dense tensor, same call and outputs as
     TensorFox.cpd (TensorFox.py:69-365).  T: NumPy array or torch tensor (CPU or GPU), float32
@@ -272,6 +287,7 @@ def cpd(T, R, options=False):
     dims_orig = tuple(T.shape)
     options = make_options(options)
     consistency(R, dims_orig, options)
+    check_supported(dims_orig)
     if options.method in ('dGN', 'als'):
         return tricpd(T, R, options)
     raise NotImplementedError("method '%s' is not on the MI355X path (dense 'dGN' and 'als' only)"

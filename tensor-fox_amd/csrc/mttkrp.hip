@@ -458,6 +458,33 @@ int tfx_mttkrp_native_fd(tfx_ctx* ctx, const float* T, const Shape& s, const dou
   return mttkrp_native<float, double>(ctx, T, s, F, mask, out, stream);
 }
 
+int tfx_mttkrp_split16_fits(const Shape& s);
+
+// Kernel family chosen for a call (host only).
+static int mttkrp_path(const Shape& s, int t_dtype, int f_dtype, unsigned mask, int prec) {
+  if (prec == TFX_PREC_SPLIT16) {
+    if (t_dtype != TFX_F32 || f_dtype != TFX_F64) return TFX_E_ARG;
+    return tfx_mttkrp_split16_fits(s) ? TFX_PATH_SPLIT16_MFMA : TFX_PATH_FMA;
+  }
+  if (t_dtype == TFX_F32 && (f_dtype == TFX_F32 || f_dtype == TFX_F64)) return TFX_PATH_FMA;
+  if (t_dtype == TFX_F64 && f_dtype == TFX_F64) {
+    // FP64 matrix cores; the FMA form takes unfoldings with 2^31 columns or more
+    for (int l = 0; l < s.L; ++l)
+      if (((mask >> l) & 1u) && s.N / s.d[l] > 0x7fffffff) return TFX_PATH_FMA;
+    return TFX_PATH_F64_MFMA;
+  }
+  return TFX_E_ARG;
+}
+
+extern "C" int tfx_mttkrp_plan(int t_dtype, int f_dtype, int prec, int L, const int64_t* dims,
+                               int R, unsigned mode_mask) {
+  if (!dims) return TFX_E_ARG;
+  Shape s;
+  const int rc = tfx_make_shape(nullptr, L, dims, R, &s);
+  if (rc) return rc;
+  return mttkrp_path(s, t_dtype, f_dtype, mode_mask, prec);
+}
+
 extern "C" int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L,
                                 const int64_t* dims, const void* F_flat, int f_dtype, int R,
                                 unsigned mode_mask, void* out_flat, int prec, hipStream_t stream) {
@@ -466,30 +493,27 @@ extern "C" int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L,
   int rc = tfx_make_shape(ctx, L, dims, R, &s);
   if (rc) return rc;
   if (mode_mask == 0) return TFX_OK;
-  if (prec == TFX_PREC_SPLIT16) {
-    if (t_dtype != TFX_F32 || f_dtype != TFX_F64)
-      return tfx_set_error(ctx, TFX_E_ARG, "split16 MTTKRP needs an FP32 tensor and FP64 factors");
-    return tfx_mttkrp_split16(ctx, (const float*)T, s, (const double*)F_flat, mode_mask,
-                              (double*)out_flat, stream);
-  }
-  if (t_dtype == TFX_F32 && f_dtype == TFX_F32)
-    return mttkrp_native<float, float>(ctx, (const float*)T, s, (const float*)F_flat, mode_mask,
-                                       (float*)out_flat, stream);
-  if (t_dtype == TFX_F32 && f_dtype == TFX_F64)
+  const int path = mttkrp_path(s, t_dtype, f_dtype, mode_mask, prec);
+  if (path < 0)
+    return tfx_set_error(ctx, TFX_E_ARG,
+                         prec == TFX_PREC_SPLIT16
+                             ? "split16 MTTKRP needs an FP32 tensor and FP64 factors"
+                             : "unsupported dtype pair %d/%d", t_dtype, f_dtype);
+  if (t_dtype == TFX_F32 && f_dtype == TFX_F64) {
+    if (path == TFX_PATH_SPLIT16_MFMA)
+      return tfx_mttkrp_split16(ctx, (const float*)T, s, (const double*)F_flat, mode_mask,
+                                (double*)out_flat, stream);
     return mttkrp_native<float, double>(ctx, (const float*)T, s, (const double*)F_flat, mode_mask,
                                         (double*)out_flat, stream);
-  if (t_dtype == TFX_F64 && f_dtype == TFX_F64) {
-    // FP64 matrix cores; the FMA form takes unfoldings with 2^31 columns or more
-    bool fits = true;
-    for (int l = 0; l < L; ++l)
-      if (((mode_mask >> l) & 1u) && s.N / s.d[l] > 0x7fffffff) fits = false;
-    if (fits)
-      return tfx_mttkrp_f64mfma(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
-                                (double*)out_flat, stream);
-    return mttkrp_native<double, double>(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
-                                         (double*)out_flat, stream);
   }
-  return tfx_set_error(ctx, TFX_E_ARG, "unsupported dtype pair %d/%d", t_dtype, f_dtype);
+  if (t_dtype == TFX_F32)
+    return mttkrp_native<float, float>(ctx, (const float*)T, s, (const float*)F_flat, mode_mask,
+                                       (float*)out_flat, stream);
+  if (path == TFX_PATH_F64_MFMA)
+    return tfx_mttkrp_f64mfma(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
+                              (double*)out_flat, stream);
+  return mttkrp_native<double, double>(ctx, (const double*)T, s, (const double*)F_flat, mode_mask,
+                                       (double*)out_flat, stream);
 }
 
 extern "C" int tfx_mttkrp(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,

@@ -499,14 +499,19 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
-                       unsigned mask, double* out, hipStream_t stream) {
+namespace {
+
+// Byte offsets of the brick partials in the kernel's scratch buffer.
+struct SplitScratch {
+  size_t k, j, o, s, total;
+};
+
+// Plans the tiled kernel for shape s (no device call): fills P and the scratch layout.
+void split16_plan(const tfx_ctx* ctx, const Shape& s, MPlan& P, SplitScratch& W) {
   const int L = s.L;
   const int64_t K = s.d[L - 1];
-  // shapes outside the tiled kernel's envelope use the FMA path
-  if (L < 2 || K % 4 != 0) return tfx_mttkrp_native_fd(ctx, T, s, F, mask, out, stream);
-  MPlan P;
   P.s = s;
+  P.rc0 = 0;
   P.K = K;
   P.J = s.d[L - 2];
   P.NO = s.N / (K * P.J);
@@ -525,14 +530,25 @@ int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const doubl
   const size_t nj = (size_t)P.kblocks * P.osplit * P.J * RC;
   const size_t no = L > 2 ? (size_t)bricks * P.NO * RC : 0;
   const size_t ns = L > 3 ? (size_t)P.NO * RC : 0;
-  void* wsp;
-  int rc = tfx_workspace(ctx, align256(4 * nk) + align256(4 * nj) + align256(4 * no) +
-                                  align256(8 * ns) + 256, stream, &wsp);
-  if (rc) return rc;
-  float* Pk = (float*)wsp;
-  float* Pj = (float*)((char*)Pk + align256(4 * nk));
-  float* Po = (float*)((char*)Pj + align256(4 * nj));
-  double* Ps = (double*)((char*)Po + align256(4 * no));
+  W.k = 0;
+  W.j = W.k + align256(4 * nk);
+  W.o = W.j + align256(4 * nj);
+  W.s = W.o + align256(4 * no);
+  W.total = W.s + align256(8 * ns);
+}
+
+// Launches the tiled kernel and the partial sums with the scratch buffer at wsp (every partial
+// is written before it is read, so the buffer needs no clearing).
+int split16_run(tfx_ctx* ctx, const float* T, MPlan P, const SplitScratch& W, const double* F,
+                unsigned mask, double* out, void* wsp, hipStream_t stream) {
+  const Shape& s = P.s;
+  const int L = s.L;
+  const int64_t K = P.K;
+  const int64_t bricks = (int64_t)P.kblocks * P.jblocks;
+  float* Pk = (float*)((char*)wsp + W.k);
+  float* Pj = (float*)((char*)wsp + W.j);
+  float* Po = (float*)((char*)wsp + W.o);
+  double* Ps = (double*)((char*)wsp + W.s);
   TFX_HIP(ctx, hipFuncSetAttribute((const void*)k_mttkrp_mfma,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_TOTAL));
   for (int rc0 = 0; rc0 < s.R; rc0 += RC) {
@@ -562,4 +578,28 @@ int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const doubl
     }
   }
   return TFX_OK;
+}
+
+}  // namespace
+
+int tfx_mttkrp_split16_fits(const Shape& s) {
+  const int L = s.L;
+  if (L < 2) return 0;
+  const int64_t K = s.d[L - 1];
+  // rows of a 64-row tile are addressed by 32-bit buffer offsets (row * K + k) * 4 bytes from the
+  // tile's first row: 64 rows of K floats must stay below 2^31 bytes
+  return K % 4 == 0 && (int64_t)TB * K * 4 < ((int64_t)1 << 31);
+}
+
+int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const double* F,
+                       unsigned mask, double* out, hipStream_t stream) {
+  if (!tfx_mttkrp_split16_fits(s))
+    return tfx_set_error(ctx, TFX_E_ARG, "shape outside the tiled split16 kernel");
+  MPlan P;
+  SplitScratch W;
+  split16_plan(ctx, s, P, W);
+  void* wsp;
+  int rc = tfx_workspace(ctx, W.total + 256, stream, &wsp);
+  if (rc) return rc;
+  return split16_run(ctx, T, P, W, F, mask, out, wsp, stream);
 }

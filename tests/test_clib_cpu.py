@@ -31,7 +31,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     # the ctypes prototypes cover every declared compute symbol
     for name in _declared_symbols():
         if name != 'tfx_last_error':
-            assert name in _lib.SIGNATURES, name
+            assert name in _lib.SIGNATURES or name in _lib.HOST_SIGNATURES, name
 
 
 def test_make_options_matches_oracle():
@@ -183,3 +183,34 @@ def test_native_batch_draws_match_numpy_stream():
         got = x0[xoff[m]:xoff[m] + nx[m]]
         assert np.abs(got - x).max() <= 1e-13 * np.abs(x).max(), m
     assert not hz.any()
+
+
+def test_mttkrp_planner_routes_large_last_dims_to_fma():
+    """The split16 kernel addresses 64-row tiles with 32-bit byte offsets: a last dimension K with
+    64 K 4 >= 2^31 (or K % 4 != 0) goes to the FMA path; FP64 tensors use the FP64 matrix cores
+    unless an unfolding has 2^31 columns or more."""
+    from TensorFox import _lib
+    lib = _lib.load()
+    S16, FMA, F64 = _lib.TFX_PATH_SPLIT16_MFMA, _lib.TFX_PATH_FMA, _lib.TFX_PATH_F64_MFMA
+
+    def plan(dims, t=_lib.TFX_F32, f=_lib.TFX_F64, prec=_lib.TFX_PREC_SPLIT16, mask=7):
+        return lib.tfx_mttkrp_plan(t, f, prec, len(dims), _lib.dims_arg(dims), 50, mask)
+    assert plan((400, 400, 400)) == S16
+    assert plan((4, 4, 8388604)) == S16                 # 64 * K * 4 = 2^31 - 1024
+    assert plan((4, 4, 8388608)) == FMA                 # 64 * K * 4 = 2^31
+    assert plan((4, 4, 10000000)) == FMA
+    assert plan((400, 400, 402)) == FMA                 # K % 4 != 0
+    assert plan((30, 20, 10), prec=_lib.TFX_PREC_NATIVE) == FMA
+    assert plan((30, 20, 10), t=_lib.TFX_F64, prec=_lib.TFX_PREC_NATIVE) == F64
+    assert plan((2, 2, 2 ** 31), t=_lib.TFX_F64, prec=_lib.TFX_PREC_NATIVE, mask=1) == FMA
+    assert plan((30, 20, 10), t=_lib.TFX_F64, prec=_lib.TFX_PREC_SPLIT16) < 0
+
+
+def test_unsupported_shapes_fail_before_upload():
+    """Orders above TFX_MAXL and modes above the MLSVD's Gram size pass the reference's checks
+    but raise NotImplementedError before the tensor reaches the device."""
+    import TensorFox as tfx
+    with pytest.raises(NotImplementedError, match='order 9'):
+        tfx.cpd(np.ones((2,) * 9), 2)
+    with pytest.raises(NotImplementedError, match='8192'):
+        tfx.cpd(np.ones((8193, 2, 2)), 2)
