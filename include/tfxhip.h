@@ -74,6 +74,19 @@ enum { TFX_PATH_SPLIT16_MFMA = 0, TFX_PATH_FMA = 1, TFX_PATH_F64_MFMA = 2 };
 int tfx_mttkrp_plan(int t_dtype, int f_dtype, int prec, int L, const int64_t* dims, int R,
                     unsigned mode_mask);
 
+/* The split16 MTTKRP of tfx_mttkrp_mixed (FP32 tensor, FP64 factors and outputs,
+ * TFX_PREC_SPLIT16) without any allocation: the brick partials live in a static device array of
+ * TFX_NOALLOC_SCRATCH_MB MiB per GPU (held until the process ends) instead of the context's
+ * workspace.  No counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Shapes outside the tiled kernel (see tfx_mttkrp_plan) or needing more scratch than the array
+ * holds return TFX_E_UNSUPPORTED.  Calls on one GPU share the array: calls that overlap on one
+ * GPU (different streams or host threads) race. */
+#define TFX_NOALLOC_SCRATCH_MB 64
+int tfx_mttkrp_noalloc(tfx_ctx* ctx, const void* T, int L, const int64_t* dims,
+                       const void* F_flat, int R, unsigned mode_mask, void* out_flat,
+                       hipStream_t stream);
+
 /* ||T - [[F]]||_F^2, accumulated in FP64, written to *out_sqnorm (device).
  * Replaces cpd2unfold1 + fastnorm (Conversion.py:117-141, Critical.py:14-22). */
 int tfx_cpd_residual(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,

@@ -26,6 +26,7 @@ SIGNATURES = {
     'tfx_device_info': [_c_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)],
     'tfx_mttkrp': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_uint, _c_vp, _c_int, _c_vp],
     'tfx_mttkrp_mixed': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_int, _c_uint, _c_vp, _c_int, _c_vp],
+    'tfx_mttkrp_noalloc': [_c_vp, _c_vp, _c_int, _I64P, _c_vp, _c_int, _c_uint, _c_vp, _c_vp],
     'tfx_ttm_mixed': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp],
     'tfx_cpd_residual': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_vp, _c_vp],
     'tfx_cpd_residual_mixed': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_int, _c_vp, _c_vp],

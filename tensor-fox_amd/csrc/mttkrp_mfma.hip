@@ -34,6 +34,8 @@
 // Only the D layout matters for correctness: A and B share the k assignment of a slot group.
 #include "common.h"
 
+#include <mutex>
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -580,6 +582,35 @@ int split16_run(tfx_ctx* ctx, const float* T, MPlan P, const SplitScratch& W, co
   return TFX_OK;
 }
 
+// Scratch of tfx_mttkrp_noalloc: one copy per GPU, held until the process ends.  Host code uses
+// the device address from hipGetSymbolAddress, never the array's name.
+constexpr size_t NOALLOC_BYTES = (size_t)TFX_NOALLOC_SCRATCH_MB << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+void* g_scratch_ptr[MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+int scratch_address(tfx_ctx* ctx, void** out) {
+  if (ctx->device < 0 || ctx->device >= MAX_DEVICES)
+    return tfx_set_error(ctx, TFX_E_ARG, "device %d outside [0, %d)", ctx->device, MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    int cur = 0;
+    TFX_HIP(ctx, hipGetDevice(&cur));
+    TFX_HIP(ctx, hipSetDevice(ctx->device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    TFX_HIP(ctx, hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return tfx_set_error(ctx, TFX_E_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                           hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = p;
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return TFX_OK;
+}
+
 }  // namespace
 
 int tfx_mttkrp_split16_fits(const Shape& s) {
@@ -602,4 +633,30 @@ int tfx_mttkrp_split16(tfx_ctx* ctx, const float* T, const Shape& s, const doubl
   int rc = tfx_workspace(ctx, W.total + 256, stream, &wsp);
   if (rc) return rc;
   return split16_run(ctx, T, P, W, F, mask, out, wsp, stream);
+}
+
+extern "C" int tfx_mttkrp_noalloc(tfx_ctx* ctx, const void* T, int L, const int64_t* dims,
+                                  const void* F_flat, int R, unsigned mode_mask, void* out_flat,
+                                  hipStream_t stream) {
+  if (!ctx || !T || !F_flat || !out_flat) return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  Shape s;
+  int rc = tfx_make_shape(ctx, L, dims, R, &s);
+  if (rc) return rc;
+  if (mode_mask == 0) return TFX_OK;
+  if (!tfx_mttkrp_split16_fits(s))
+    return tfx_set_error(ctx, TFX_E_UNSUPPORTED,
+                         "tfx_mttkrp_noalloc: the last dimension must be a multiple of 4 below "
+                         "2^23 (got %lld)", (long long)s.d[L - 1]);
+  MPlan P;
+  SplitScratch W;
+  split16_plan(ctx, s, P, W);
+  if (W.total > NOALLOC_BYTES)
+    return tfx_set_error(ctx, TFX_E_UNSUPPORTED,
+                         "tfx_mttkrp_noalloc: this shape needs %zu bytes of scratch, more than the "
+                         "static %zu", W.total, NOALLOC_BYTES);
+  void* wsp;
+  rc = scratch_address(ctx, &wsp);
+  if (rc) return rc;
+  return split16_run(ctx, (const float*)T, P, W, (const double*)F_flat, mode_mask,
+                     (double*)out_flat, wsp, stream);
 }
