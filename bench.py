@@ -1,16 +1,19 @@
 """Benchmark of the dense CPD hot path on one MI355X.
 
-Workload (a "step" = one full `cpd` call: MLSVD -> dGN on the core -> uncompression ->
-refinement -> final FP64 error): dense 400 x 400 x 400 FP32 tensor, rank 50, factors N(0, 1),
-relative noise 1e-5, seed 2 (an instance on which the reference algorithm converges to the noise
-floor); the tensor is resident in HBM before the timed region.  The same problem with the first
-factor's columns scaled by 10^U(-1, 1) (seed 0), on which the reference algorithm stalls at a
-relative error of about 3e-2, is timed once and reported under "colscale_instance".  The mixed-precision form is timed for `--steps`
-steps; the FP64 GPU form runs the same problem once for the speedup; the dominant kernel (the
-fused MTTKRP of the refinement) is timed with HIP events on the stream it runs on; the CPU
-baseline times the FP64 NumPy restatement of the reference algorithm on a bounded sample.
+Headline workload (a "step" = one full `cpd` call: MLSVD -> dGN on the core -> uncompression ->
+refinement -> final FP64 error): dense 400 x 400 x 400 FP32 tensor, rank 50, factors N(0, 1)
+with the first factor's columns scaled by 10^U(-1, 1), relative noise 1e-5, seed 0, refine=True,
+mixed precision; the tensor is resident in HBM before the timed region.  The reference algorithm
+does not converge on this problem (it stalls near 3e-2, and its final value is decided by
+roundoff), so the line also carries the oracle's result from tests/golden/c2_oracle.json, the
+iterations over which the GPU history follows it, the FP64 GPU form on the same problem, an
+instance that converges (seed 2 without scaling: time to the noise floor) and a sweep over seeds
+0-7 with and without the scaling.  The dominant tensor kernel (the fused MTTKRP of the
+refinement) is timed with HIP events on the stream it runs on; the core CG is timed per
+iteration (latency-bound); the CPU baseline runs the FP64 NumPy restatement of the reference
+algorithm (the oracle) on the same tensor: one warm-up and the median of --cpu-runs full runs.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu] ...
 """
 
 import argparse
@@ -31,6 +34,9 @@ HBM_PEAK_GBS = 8000.0
 # plain 256 MB device copy in the same run (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
 # reads 262.4 MB + writes 12.2 MB (brick partials) + the partial sums (6.1 MB read).
 MEASURED_TRAFFIC = {((400, 400, 400), 50): 280.7e6}
+TRAFFIC_SOURCE = ('rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --kernel-only` '
+                  '(profiles/mttkrp_split16_400x400x400_r50.md), not measured in this run')
+FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
 def log(*a):
@@ -42,11 +48,15 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--dims', type=int, nargs='+', default=[400, 400, 400])
+    ap.add_argument('--dims', type=int, nargs='+', default=[400, 400, 400],
+                    help='shape of the --kernel-only / --compression-only tensor')
     ap.add_argument('--rank', type=int, default=50)
-    ap.add_argument('--noise', type=float, default=1e-5)
-    ap.add_argument('--seed', type=int, default=2)
-    ap.add_argument('--no-colscale', action='store_true', help='skip the column-scaled instance')
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--no-sweep', action='store_true',
+                    help='skip the seed sweep (seeds 0-7 with and without the column scaling)')
+    ap.add_argument('--pool-members', type=int, default=256,
+                    help='batch members the process-pool CPU baseline runs')
+    ap.add_argument('--cpu-runs', type=int, default=3, help='timed oracle runs (after one warm-up)')
     ap.add_argument('--no-fp64', action='store_true', help='skip the FP64 GPU form')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--no-loss', action='store_true', help='skip the CPDLoss forward/backward timing')
@@ -54,9 +64,8 @@ def parse():
     ap.add_argument('--no-batched', action='store_true', help='skip the 4096-member batch')
     ap.add_argument('--batch-size', type=int, default=4096)
     ap.add_argument('--no-pipeline', action='store_true',
-                    help='skip the 400^3 pipeline (and its FP64, column-scaled and CPU legs): time only '
+                    help='skip the 400^3 pipeline (and its FP64, sweep and CPU legs): time only '
                          'the other sections, e.g. to profile one of them')
-    ap.add_argument('--cpu-sample-iters', type=int, default=1)
     ap.add_argument('--kernel-reps', type=int, default=20)
     ap.add_argument('--kernel-only', action='store_true',
                     help='time only the fused MTTKRP on a random resident tensor of the same shape, '
@@ -292,10 +301,21 @@ def make_batch(nb, seed=0):
     return tensors, [int(r) for r in ranks]
 
 
-def batched_time(tfx, nb, steps):
+def _oracle_member(args):
+    """One oracle cpd of batch member m (single-threaded BLAS in a pool worker)."""
+    import numpy as np
+    import oracle.tfx_oracle as o
+    T, R, seed = args
+    np.random.seed(seed)
+    o.cpd(T.astype(np.float64), R)
+    return 0
+
+
+def batched_time(tfx, nb, steps, pool_members, pool):
     """The batch resident on the device, autotuned launch configuration, then `steps` timed
-    solves of the whole batch (member m seeded with m); plus the oracle on a bounded sample of
-    members on the host cores."""
+    solves of the whole batch (member m seeded with m); plus the oracle on bounded samples of
+    members on the host cores: a sequential loop (how the reference would run it) and a process
+    pool of single-threaded-BLAS workers over all cores."""
     import numpy as np
     import oracle.tfx_oracle as o
     tensors, ranks = make_batch(nb)
@@ -319,43 +339,173 @@ def batched_time(tfx, nb, steps):
            'rel_error_median': float(np.median(errs)),
            'rel_error_below_1e-3': float(np.mean(errs < 1e-3)),
            'dgn_steps_mean': float(nsteps.mean())}
-    # oracle on a bounded sample (every 512th member), scaled to the batch
+    cores = os.cpu_count()
+    # sequential oracle on every 512th member, scaled to the batch
     idx = list(range(0, nb, max(1, nb // 8)))
     t0 = time.perf_counter()
     for i in idx:
         np.random.seed(seeds[i])
         o.cpd(tensors[i].astype(np.float64), ranks[i])
     tc = time.perf_counter() - t0
-    out['cpu_baseline'] = {'value': round(tc / len(idx) * nb, 2), 'unit': 's', 'cores': os.cpu_count(),
-                           'kind': 'port',
-                           'sample': 'oracle (FP64 NumPy restatement of the reference) on %d members '
-                                     '(%.2f s), scaled to %d members' % (len(idx), tc, nb)}
+    seq = tc / len(idx) * nb
+    # process pool over all cores, single-threaded BLAS per worker, on the first members
+    npool = min(nb, pool_members)
+    pool.map(_oracle_member, [(tensors[0], ranks[0], 0)] * cores)    # warm the workers
+    t0 = time.perf_counter()
+    pool.map(_oracle_member, [(tensors[m], ranks[m], seeds[m]) for m in range(npool)], chunksize=1)
+    tp = time.perf_counter() - t0
+    out['cpu_baseline'] = {
+        'value': round(seq, 2), 'unit': 's', 'cores': 1, 'kind': 'port',
+        'sample': 'oracle (FP64 NumPy restatement of the reference), sequential, on %d members '
+                  '(every %d-th, %.2f s), scaled to %d members' % (len(idx), max(1, nb // 8), tc, nb)}
+    out['cpu_baseline_pool'] = {
+        'value': round(tp / npool * nb, 2), 'unit': 's', 'cores': cores, 'kind': 'port',
+        'sample': 'oracle in a pool of %d single-threaded-BLAS processes on members 0-%d (%.2f s), '
+                  'scaled to %d members' % (cores, npool - 1, tp, nb)}
     return out
 
 
-def cpu_baseline(T32, R, sample_iters, seed):
-    """FP64 NumPy restatement of the reference algorithm (the oracle) on the host cores: the
-    MLSVD, `sample_iters` main-stage and `sample_iters` refinement dGN iterations are timed, then
-    scaled to the iteration counts of a full run given by the caller."""
+def oracle_phases(T32, R, seed):
+    """One full oracle run (the FP64 NumPy restatement of the reference algorithm) of the cpd
+    pipeline with refinement, timed per phase as TensorFox.py:424-655 orders them."""
     import numpy as np
     import oracle.tfx_oracle as o
-    T = T32.astype(np.float64)
 
     class opt:
         refine = True
-        maxiter = sample_iters
+    options = o.make_options(opt)
+    ph = {}
+    t = time.perf_counter()
+    T = T32.astype(np.float64)
     np.random.seed(seed)
-    t0 = time.perf_counter()
     Ts, ordering = o.sort_dims(T)
-    S, U, T1, sig = o.mlsvd(Ts, np.linalg.norm(Ts), R, o.make_options(opt))
+    Tsize = np.linalg.norm(Ts)
     t1 = time.perf_counter()
-    init = o.starting_point(S, U, R, ordering, o.make_options(opt))
-    F, *_ = o.dGN(S, init, R, o.make_options(opt))
+    ph['prepare'] = t1 - t
+    S, U, T1, _ = o.mlsvd(Ts, Tsize, R, options)
     t2 = time.perf_counter()
-    F = [U[l] @ F[l] for l in range(3)]
-    o.dGN(np.asarray(Ts), F, R, o.make_options(opt))
+    ph['mlsvd'] = t2 - t1
+    init = o.starting_point(S, U, R, ordering, options)
     t3 = time.perf_counter()
-    return dict(mlsvd=t1 - t0, main_iter=(t2 - t1) / sample_iters, refine_iter=(t3 - t2) / sample_iters)
+    ph['init'] = t3 - t2
+    F, ss_m, *_ = o.dGN(S, init, R, options)
+    t4 = time.perf_counter()
+    ph['main'] = t4 - t3
+    F = [U[l] @ F[l] for l in range(len(F))]
+    t5 = time.perf_counter()
+    ph['uncompress'] = t5 - t4
+    F, ss_r, e_r, *_ = o.dGN(np.asarray(Ts), F, R, options)
+    t6 = time.perf_counter()
+    ph['refine'] = t6 - t5
+    err = o.fastnorm(T1, o.cpd2unfold1(F)) / Tsize
+    t7 = time.perf_counter()
+    ph['final'] = t7 - t6
+    ph['total'] = t7 - t
+    return ph, float(e_r[-1]), [int(ss_m.size), int(ss_r.size)], float(err)
+
+
+def cpu_baseline(T32, R, seed, runs):
+    """One warm-up and the median of `runs` full oracle runs on the host cores (every thread the
+    BLAS library uses), with per-phase timers."""
+    import numpy as np
+    oracle_phases(T32, R, seed)
+    res = [oracle_phases(T32, R, seed) for _ in range(runs)]
+    tot = [r[0]['total'] for r in res]
+    med = res[int(np.argsort(tot)[len(tot) // 2])]
+    return {
+        'value': round(float(np.median(tot)), 3), 'unit': 's', 'cores': os.cpu_count(), 'kind': 'port',
+        'sample': 'oracle (FP64 NumPy restatement of the reference algorithm, not the reference '
+                  'itself) on the same tensor and seed with refine=True: one warm-up and the median '
+                  'of %d full runs (the reference\'s 1 s sleep in consistency() is not part of the '
+                  'oracle)' % runs,
+        'runs_s': [round(x, 3) for x in tot],
+        'median_phase_seconds': {a: round(b, 3) for a, b in med[0].items()},
+        'median_run_rel_error': med[1], 'median_run_steps': med[2]}
+
+
+def _history_horizon(e, e_ref, tol=1e-6):
+    """Number of leading main-stage iterations whose errors agree within tol."""
+    import numpy as np
+    e, e_ref = np.asarray(e), np.asarray(e_ref)
+    n = min(e.size, e_ref.size)
+    bad = np.nonzero(np.abs(e[:n] - e_ref[:n]) > tol)[0]
+    return int(bad[0]) if bad.size else n
+
+
+def _load_fixture():
+    with open(os.path.join(ROOT, 'tests', 'golden', 'c2_oracle.json')) as fh:
+        return json.load(fh)
+
+
+def _gen(args):
+    """(seed, col_scale) -> the FP32 400^3 rank-50 instance (pool worker)."""
+    import numpy as np
+    from TensorFox.auxiliar import make_lowrank
+    seed, col_scale = args
+    return make_lowrank((400, 400, 400), 50, 1e-5, seed=seed, col_scale=col_scale, dtype=np.float32)[0]
+
+
+def seed_sweep(tfx, fix, pool):
+    """Seeds 0-7 with and without the column scaling, one mixed-precision cpd each (warm library),
+    against the oracle's results in the fixture; the instances are generated by the host pool."""
+    import numpy as np
+    import torch
+    jobs = [(s, cs) for cs in (True, False) for s in range(8)]
+    rows = []
+    for (seed, cs), T in zip(jobs, pool.imap(_gen, jobs)):
+        Td = torch.from_numpy(T).to('cuda')
+        t, _, out = time_cpd(tfx, Td, 50, 'mixed', seed)
+        ref = [r for r in fix['sweep'] if r['seed'] == seed and r['col_scale'] == cs][0]
+        rows.append({'seed': seed, 'col_scale': cs, 'seconds': round(t, 5),
+                     'rel_error': out.rel_error, 'oracle_rel_error': ref['rel_error'],
+                     'abs_diff': abs(out.rel_error - ref['rel_error']),
+                     'main_steps': int(out.errors[0].size), 'oracle_main_steps': ref['steps'][0],
+                     'converged': bool(out.rel_error < 2e-5),
+                     'oracle_converged': bool(ref['rel_error'] < 2e-5)})
+        log('sweep seed %d col_scale %s: %.3f s rel_error %.4e oracle %.4e' % (
+            seed, cs, t, out.rel_error, ref['rel_error']))
+        del Td
+    summ = {}
+    for cs in (True, False):
+        sub = [r for r in rows if r['col_scale'] == cs]
+        conv = [r for r in sub if r['oracle_converged']]
+        summ['col_scale' if cs else 'unscaled'] = {
+            'gpu_converged': sum(r['converged'] for r in sub),
+            'oracle_converged': len(conv), 'of': len(sub),
+            'max_abs_diff_where_oracle_converged': max([r['abs_diff'] for r in conv], default=None),
+            'mean_seconds': round(float(np.mean([r['seconds'] for r in sub])), 5)}
+    return {'workload': 'dense CPD 400^3 rank 50, noise 1e-5, FP32, refine, mixed precision; seeds 0-7 '
+                        'with and without the first factor\'s columns scaled by 10^U(-1,1)',
+            'summary': summ, 'runs': rows}
+
+
+def cg_matvec_time(dev, reps):
+    """The CG of the main stage on the 50^3 core (rank 50, FP64, 20 CG iterations per call, no
+    early stop) timed with HIP events: seconds per CG iteration and the algorithmic bytes of one
+    iteration (the point-sized vectors it reads and writes, 3 n FP64 values) as GB/s.  The core is
+    tiny: the figure is latency-bound, not a bandwidth measurement."""
+    import numpy as np
+    import torch
+    dims, R = (50, 50, 50), 50
+    L = 3
+    n = R * sum(dims)
+    rs = np.random.RandomState(2)
+    F = dev.upload(rs.standard_normal(n) / np.sqrt(50), torch.float64)
+    g = dev.upload(rs.standard_normal(n), torch.float64)
+    Gr, P1 = dev.empty((L, R, R), torch.float64), dev.empty((L, R, R), torch.float64)
+    P2, gamma = dev.empty((L, L, R, R), torch.float64), dev.empty((L, R), torch.float64)
+    dev.gramians(dims, R, F, Gr, P1, P2, gamma)
+    y, jtjg = dev.empty(n, torch.float64), dev.empty(n, torch.float64)
+    scal = dev.zeros(2, torch.float64)
+    its = 20
+    t = _event_time(lambda: dev.cg(dims, R, F, P1, P2, gamma, 0.5, g, its, 0.0, y, jtjg, scal), reps)
+    done = int(scal[0].item())
+    per_it = t / max(1, done)
+    b = 3 * n * 8
+    return {'workload': 'CG on the 50^3 rank-50 core (n = %d unknowns), FP64, %d iterations per call' % (n, done),
+            'bound': 'latency', 'seconds_per_call': t, 'seconds_per_cg_iteration': per_it,
+            'algorithmic_bytes_per_iteration': b,
+            'achieved_GBs_latency_bound': round(b / per_it / 1e9, 2)}
 
 
 def main():
@@ -371,6 +521,27 @@ def main():
     if args.compression_only:
         compression_only(args)
         return
+    # host worker pool (instance generation for the sweep, the batched CPU baseline), started
+    # before anything touches the GPU, one single-threaded-BLAS process per core
+    import multiprocessing as mp
+    env_keys = ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS')
+    saved = {k: os.environ.get(k) for k in env_keys}
+    for k in env_keys:
+        os.environ[k] = '1'
+    pool = mp.get_context('spawn').Pool(os.cpu_count())
+    for k, v in saved.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+    try:
+        run(args, pool)
+    finally:
+        pool.terminate()
+        pool.join()
+
+
+def run(args, pool):
     import numpy as np
     import torch
     import TensorFox as tfx
@@ -378,25 +549,26 @@ def main():
     from TensorFox.device import get_device
 
     dev = get_device(0)
-    result = {'metric': 'time to rel.err 1e-6, 400^3 R=50: mixed vs FP64 GPU dGN; MTTKRP GB/s',
-              'value': None, 'unit': 's', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-              'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None}
+    metric = 'time to rel.err 1e-6, 400^3 R=50: mixed vs FP64 GPU dGN; MTTKRP GB/s'
+    result = {'metric': metric, 'value': None, 'unit': 's', 'n_gpus': 1, 'steps': args.steps,
+              'warmup': args.warmup, 'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None}
+    fix = _load_fixture()
     if not args.no_pipeline:
-        dims, R, nu = tuple(args.dims), args.rank, args.noise
-        log('generating', dims, R, nu)
-        T, Ftrue = tfx.make_lowrank(dims, R, nu, seed=args.seed, dtype=np.float32)
+        dims, R, nu, seed = (400, 400, 400), 50, 1e-5, args.seed
+        log('generating', dims, R, nu, 'column-scaled, seed', seed)
+        T, Ftrue = tfx.make_lowrank(dims, R, nu, seed=seed, col_scale=True, dtype=np.float32)
         Tdev = torch.from_numpy(T).to(dev.dev)
         torch.cuda.synchronize()
         T0 = tfx.cpd2tens(Ftrue)
         T0n = float(np.linalg.norm(T0))
 
         for w in range(args.warmup):
-            t, _, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
+            t, _, out = time_cpd(tfx, Tdev, R, 'mixed', seed)
             log('warmup %d: %.3f s rel_error %.3e' % (w, t, out.rel_error))
         times, outs, facs = [], [], []
         torch.cuda.synchronize()
         for k in range(args.steps):
-            t, f, out = time_cpd(tfx, Tdev, R, 'mixed', args.seed)
+            t, f, out = time_cpd(tfx, Tdev, R, 'mixed', seed)
             times.append(t)
             outs.append(out)
             facs.append(f)
@@ -406,9 +578,12 @@ def main():
         ms = 1e3 * float(np.mean(times))
         out = outs[-1]
         model_err = float(np.linalg.norm(tfx.cpd2tens(facs[-1]) - T0)) / T0n
+        base = fix['colscale_seed0'] if seed == 0 else None
+        ens = ([fix['colscale_seed0']] + fix['colscale_seed0_perturbed']) if seed == 0 else []
+        n_main = int(out.errors[0].size)
 
         result = {
-            'metric': 'time to rel.err 1e-6, 400^3 R=50: mixed vs FP64 GPU dGN; MTTKRP GB/s',
+            'metric': metric,
             'value': round(ms / 1e3, 5),
             'unit': 's',
             'n_gpus': 1,
@@ -419,19 +594,33 @@ def main():
             'scaling': 'weak',
             'vs_baseline': None,
             'dtype': 'f32+f64',
-            'data': 'synthetic (seeded low-rank + 1e-5 relative noise)',
-            'config': {'workload': 'dense CPD %s rank %d, noise %g, seed %d, refine, mixed precision' % (
-                           'x'.join(map(str, dims)), R, nu, args.seed),
-                       'dims': list(dims), 'rank': R, 'noise': nu, 'seed': args.seed, 'refine': True,
-                       'precision': 'mixed'},
+            'data': 'synthetic (seeded low-rank, first factor columns x 10^U(-1,1), 1e-5 relative noise)',
+            'config': {'workload': 'dense CPD 400x400x400 rank 50, factors N(0,1) with the first factor\'s '
+                                   'columns scaled by 10^U(-1,1), noise 1e-5 relative, stored FP32, '
+                                   'seed %d, refine, mixed precision' % seed,
+                       'dims': list(dims), 'rank': R, 'noise': nu, 'col_scale': True, 'seed': seed,
+                       'refine': True, 'precision': 'mixed'},
             'rel_error': out.rel_error,
             'final_rel_error': out.final_rel_error,
             'model_error_vs_noise_free': model_err,
             'num_steps': int(out.num_steps),
+            'stop': [int(v) for v in out.stop],
+            'main_iterations': n_main,
+            'main_seconds_per_iteration': round(out.timings['main'] / n_main, 6),
             'stage_seconds': {a: round(b, 5) for a, b in out.timings.items()},
         }
+        if base is not None:
+            result['oracle'] = {
+                'source': 'tests/golden/c2_oracle.json (oracle, %d BLAS threads)' % fix['blas_threads'],
+                'rel_error': base['rel_error'], 'stop': base['stop'], 'steps': base['steps'],
+                'abs_diff': abs(out.rel_error - base['rel_error']),
+                'perturbed_runs_rel_error_range': [min(r['rel_error'] for r in ens),
+                                                   max(r['rel_error'] for r in ens)],
+                'history_agreement_iterations': _history_horizon(out.errors[0], base['errors_main']),
+                'note': 'the reference algorithm stalls on this problem; runs on T (1 + 1e-14 E) end '
+                        'anywhere in the range above, so the final error is decided by roundoff'}
 
-        # dominant kernel: fused MTTKRP over the full tensor
+        # dominant tensor kernel: fused MTTKRP over the full tensor
         kt = mttkrp_kernel_time(dev, Tdev, dims, R, _lib.TFX_PREC_SPLIT16, args.kernel_reps)
         N = int(np.prod(dims))
         alg_bytes = 4 * N + 8 * R * sum(dims) * 2
@@ -439,40 +628,69 @@ def main():
         result['roofline'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                               'frac': round(gbs / HBM_PEAK_GBS, 4),
                               'traffic': MEASURED_TRAFFIC.get((dims, R)),
-                              'kernel': 'tfx_mttkrp split16 (all modes, FP32 tensor, bf16x3 MFMA)', 'avg_launch_s': kt}
+                              'traffic_source': TRAFFIC_SOURCE,
+                              'kernel': 'tfx_mttkrp split16 (all modes, FP32 tensor, bf16x3 MFMA)',
+                              'avg_launch_s': kt}
         result['compression_roofline'] = compression_time(dev, Tdev, dims, R, args.kernel_reps)
         log('compression:', result['compression_roofline'])
+        result['cg_matvec'] = cg_matvec_time(dev, args.kernel_reps)
+        log('cg:', result['cg_matvec'])
 
         if not args.no_fp64:
             T64 = Tdev.to(torch.float64)          # the FP64 form's input, resident before timing
-            time_cpd(tfx, T64, R, 'fp64', args.seed)
-            t64, f64, out64 = time_cpd(tfx, T64, R, 'fp64', args.seed)
+            time_cpd(tfx, T64, R, 'fp64', seed)
+            t64, f64, out64 = time_cpd(tfx, T64, R, 'fp64', seed)
+            kt64 = mttkrp_kernel_time(dev, T64, dims, R, _lib.TFX_PREC_NATIVE, 5)
             del T64
             log('fp64: %.3f s rel_error %.6e timings %s' % (t64, out64.rel_error, out64.timings))
-            result['fp64_seconds'] = round(t64, 5)
-            result['fp64_rel_error'] = out64.rel_error
+            n64 = int(out64.errors[0].size)
+            result['fp64'] = {
+                'seconds': round(t64, 5), 'rel_error': out64.rel_error, 'stop': [int(v) for v in out64.stop],
+                'main_iterations': n64, 'main_seconds_per_iteration': round(out64.timings['main'] / n64, 6),
+                'stage_seconds': {a: round(b, 5) for a, b in out64.timings.items()},
+                'mttkrp_avg_launch_s': kt64,
+                'mttkrp_fp64_mfma_TFLOPs': round(4 * N * R / kt64 / 1e12, 2),
+                'mttkrp_fp64_mfma_frac': round(4 * N * R / kt64 / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4)}
+            if base is not None:
+                result['fp64']['abs_diff_vs_oracle'] = abs(out64.rel_error - base['rel_error'])
+                result['fp64']['history_agreement_iterations'] = _history_horizon(out64.errors[0],
+                                                                                  base['errors_main'])
             result['speedup_mixed_vs_fp64'] = round(t64 / (ms / 1e3), 3)
-            result['abs_err_diff_vs_fp64'] = abs(out.rel_error - out64.rel_error)
+            result['speedup_per_main_iteration'] = round(
+                result['fp64']['main_seconds_per_iteration'] / result['main_seconds_per_iteration'], 3)
 
-        if not args.no_colscale:
-            Tc, Fc = tfx.make_lowrank(dims, R, nu, seed=0, col_scale=True, dtype=np.float32)
-            Tcd = torch.from_numpy(Tc).to(dev.dev)
-            tm, _, om = time_cpd(tfx, Tcd, R, 'mixed', 0)
-            res = {'workload': 'same, first factor columns x 10^U(-1,1), seed 0', 'mixed_seconds': round(tm, 5),
-                   'mixed_rel_error': om.rel_error}
-            if not args.no_fp64:
-                t6, _, o6 = time_cpd(tfx, Tcd.to(torch.float64), R, 'fp64', 0)
-                res.update(fp64_seconds=round(t6, 5), fp64_rel_error=o6.rel_error)
-            log('colscale instance:', res)
-            result['colscale_instance'] = res
-            del Tcd
+        # an instance on which the reference algorithm converges: time to the noise floor
+        Tc, Fc = tfx.make_lowrank(dims, R, nu, seed=2, dtype=np.float32)
+        Tcd = torch.from_numpy(Tc).to(dev.dev)
+        ref2 = [r for r in fix['sweep'] if r['seed'] == 2 and not r['col_scale']][0]
+        conv = {'workload': 'same without the column scaling, seed 2 (converges under the reference '
+                            'algorithm; seeds 5 and 7 also do, 0, 1, 3, 4, 6 stall)'}
+        for prec in ('mixed', 'fp64'):
+            X = Tcd if prec == 'mixed' else Tcd.to(torch.float64)
+            time_cpd(tfx, X, R, prec, 2)
+            tm, fm, om = time_cpd(tfx, X, R, prec, 2)
+            conv[prec] = {'seconds': round(tm, 5), 'rel_error': om.rel_error,
+                          'abs_diff_vs_oracle': abs(om.rel_error - ref2['rel_error']),
+                          'model_error_vs_noise_free': float(np.linalg.norm(tfx.cpd2tens(fm) - tfx.cpd2tens(Fc)))
+                          / float(np.linalg.norm(tfx.cpd2tens(Fc))),
+                          'stage_seconds': {a: round(b, 5) for a, b in om.timings.items()}}
+            del X
+        conv['oracle_rel_error'] = ref2['rel_error']
+        conv['speedup_mixed_vs_fp64'] = round(conv['fp64']['seconds'] / conv['mixed']['seconds'], 3)
+        log('converging instance:', conv)
+        result['converging_instance'] = conv
+        del Tcd
+
+        if not args.no_sweep:
+            result['seed_sweep'] = seed_sweep(tfx, fix, pool)
+            log('sweep:', result['seed_sweep']['summary'])
 
     if not args.no_loss:
         result['cpdloss'] = cpdloss_time(tfx, dev, args.kernel_reps)
         log('cpdloss:', result['cpdloss'])
 
     if not args.no_batched:
-        result['batched'] = batched_time(tfx, args.batch_size, args.steps)
+        result['batched'] = batched_time(tfx, args.batch_size, args.steps, args.pool_members, pool)
         log('batched:', {k: v for k, v in result['batched'].items() if k != 'tune_seconds'})
 
     if not args.no_order4:
@@ -480,19 +698,9 @@ def main():
         log('order4:', result['order4'])
 
     if not args.no_cpu and not args.no_pipeline:
-        cores = os.cpu_count()
-        log('cpu baseline on %d cores' % cores)
-        c = cpu_baseline(T, R, args.cpu_sample_iters, args.seed)
-        n_main = int(out.errors[0].size)
-        n_ref = int(out.errors[1].size)
-        est = c['mlsvd'] + n_main * c['main_iter'] + n_ref * c['refine_iter']
-        result['cpu_baseline'] = {
-            'value': round(est, 3), 'unit': 's', 'cores': cores, 'kind': 'port',
-            'sample': 'oracle (FP64 NumPy restatement of the reference) on the same tensor: MLSVD + '
-                      '%d main + %d refinement dGN iterations timed (%.2f s, %.3f s/it, %.3f s/it), '
-                      'scaled to the %d + %d iterations of the GPU run' % (
-                          args.cpu_sample_iters, args.cpu_sample_iters, c['mlsvd'], c['main_iter'],
-                          c['refine_iter'], n_main, n_ref)}
+        log('cpu baseline on %d cores' % os.cpu_count())
+        result['cpu_baseline'] = cpu_baseline(T, 50, args.seed, args.cpu_runs)
+        log('cpu baseline:', result['cpu_baseline'])
     print(json.dumps(result), flush=True)
 
 

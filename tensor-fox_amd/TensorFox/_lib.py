@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, 'libtfxhip.so')
 TFX_F32, TFX_F64 = 0, 1
 TFX_MAXL = 8          # largest tensor order of the library (include/tfxhip.h)
 TFX_PREC_NATIVE, TFX_PREC_SPLIT16 = 0, 1
+TFX_OK, TFX_E_ARG, TFX_E_HIP, TFX_E_NOMEM, TFX_E_UNSUPPORTED = 0, -1, -2, -3, -4
 
 _c_int, _c_i64, _c_dbl, _c_vp, _c_uint = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_uint
 _I64P = ctypes.POINTER(ctypes.c_int64)

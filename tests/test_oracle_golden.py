@@ -112,3 +112,24 @@ def test_als_iteration_matches_reference(golden):
     for l in range(3):
         ref = golden['als_out%d' % l]
         assert np.abs(out[l] - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_c2_oracle_fixture_is_chaotic_beyond_the_horizon():
+    """tests/golden/c2_oracle.json (oracle runs of the 400^3 rank-50 column-scaled problem, seed
+    0): the runs on T * (1 + 1e-14 E) agree with the unperturbed run to 1e-8 over the first 16
+    main-stage iterations, and end at final errors more than 1e-3 apart."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), 'golden', 'c2_oracle.json')) as fh:
+        fix = json.load(fh)
+    base = np.asarray(fix['colscale_seed0']['errors_main'])
+    finals = []
+    for r in fix['colscale_seed0_perturbed']:
+        e = np.asarray(r['errors_main'])
+        assert np.abs(e[:16] - base[:16]).max() <= 1e-8
+        finals.append(r['rel_error'])
+    assert max(finals) - min(finals) > 1e-3
+    # every column-scaled seed stalls; three of the eight unscaled seeds reach the noise floor
+    sweep = fix['sweep']
+    assert all(r['rel_error'] > 1e-3 for r in sweep if r['col_scale'])
+    assert sorted(r['seed'] for r in sweep if not r['col_scale'] and r['rel_error'] < 2e-5) == [2, 5, 7]
