@@ -155,32 +155,84 @@ __device__ void wg_symm_mul(const double* G, int n, const double* B, int p, doub
   __syncthreads();
 }
 
-// Classical Gram-Schmidt applied twice (CGS2) on the p columns of Z (n x p col-major), in place.
-// For column j the j inner products with the previous columns are computed one wavefront each
-// (no workgroup barrier per product), then subtracted in one pass.  `dots` holds >= p doubles.
-__device__ void wg_orthonormalize(double* Z, int n, int p, double* red, double* dots) {
+// z_j -= sum_{i<j} (z_i . z_j) z_i over the finished columns, applied twice.  The j inner
+// products are computed one wavefront each (no workgroup barrier per product), then subtracted
+// in one pass.  `dots` holds >= p doubles.
+__device__ void wg_project_twice(double* Z, int n, int j, double* dots) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  double* zj = Z + (int64_t)j * n;
+  for (int pass = 0; pass < 2 && j > 0; ++pass) {
+    for (int i = wave; i < j; i += nwave) {
+      const double* zi = Z + (int64_t)i * n;
+      double a = 0.0;
+      for (int e = lane; e < n; e += 64) a += zi[e] * zj[e];
+      a = wave_sum(a);
+      if (lane == 0) dots[i] = a;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      double v = zj[e];
+      for (int i = 0; i < j; ++i) v -= dots[i] * Z[(int64_t)i * n + e];
+      zj[e] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// Row r minimizing sum_{i<j} Z[i][r]^2 (the coordinate direction least covered by the finished
+// columns; the smallest r among equal values), returned in every thread.  For j < n orthonormal
+// columns that sum is at most j/n on average, so e_r keeps at least 1 - j/n of its norm.
+__device__ int wg_least_covered_row(const double* Z, int n, int j, double* red) {
+  __shared__ int ired[16];
+  double best = 1e300;
+  int arg = n;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double c = 0.0;
+    for (int i = 0; i < j; ++i) c += Z[(int64_t)i * n + r] * Z[(int64_t)i * n + r];
+    if (c < best) { best = c; arg = r; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const int oa = __shfl_xor(arg, o, 64);
+    if (ob < best || (ob == best && oa < arg)) { best = ob; arg = oa; }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) { red[w] = best; ired[w] = arg; }
+  __syncthreads();
+  best = red[0];
+  arg = ired[0];
+  for (int i = 1; i < nw; ++i)
+    if (red[i] < best || (red[i] == best && ired[i] < arg)) { best = red[i]; arg = ired[i]; }
+  __syncthreads();
+  return arg;
+}
+
+// Classical Gram-Schmidt applied twice (CGS2) on the p columns of Z (n x p col-major, p < n), in
+// place.  A column that the two projections leave with less than 1e-12 of its norm lies in the
+// span of the finished columns up to rounding (the block is rank deficient, e.g. power steps on a
+// Gram matrix of rank below p, or with rows that are exactly zero): it is replaced by the unit
+// vector of the least covered row, projected twice and normalized, so that the block is
+// orthonormal whatever its rank, as a Householder QR makes it.
+__device__ void wg_orthonormalize(double* Z, int n, int p, double* red, double* dots) {
   for (int j = 0; j < p; ++j) {
     double* zj = Z + (int64_t)j * n;
-    for (int pass = 0; pass < 2 && j > 0; ++pass) {
-      for (int i = wave; i < j; i += nwave) {
-        const double* zi = Z + (int64_t)i * n;
-        double a = 0.0;
-        for (int e = lane; e < n; e += 64) a += zi[e] * zj[e];
-        a = wave_sum(a);
-        if (lane == 0) dots[i] = a;
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        double v = zj[e];
-        for (int i = 0; i < j; ++i) v -= dots[i] * Z[(int64_t)i * n + e];
-        zj[e] = v;
-      }
-      __syncthreads();
-    }
+    double a0 = 0.0;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) a0 += zj[e] * zj[e];
+    a0 = block_sum(a0, red);
+    wg_project_twice(Z, n, j, dots);
     double a = 0.0;
     for (int e = threadIdx.x; e < n; e += blockDim.x) a += zj[e] * zj[e];
     a = block_sum(a, red);
+    if (!(a > 1e-24 * a0) && j < n) {
+      const int r = wg_least_covered_row(Z, n, j, red);
+      for (int e = threadIdx.x; e < n; e += blockDim.x) zj[e] = e == r ? 1.0 : 0.0;
+      __syncthreads();
+      wg_project_twice(Z, n, j, dots);
+      a = 0.0;
+      for (int e = threadIdx.x; e < n; e += blockDim.x) a += zj[e] * zj[e];
+      a = block_sum(a, red);
+    }
     const double nr = sqrt(a);
     const double inv = nr > 0.0 ? 1.0 / nr : 0.0;
     for (int e = threadIdx.x; e < n; e += blockDim.x) zj[e] *= inv;
@@ -215,6 +267,10 @@ constexpr int ORTH_MAXN = 2048;   // n * OPB doubles of LDS
 __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int redo_s;
+  __shared__ int collapse_s;   // a column still dependent after the extra round
+  __shared__ double red_s[16];
+  if (threadIdx.x == 0) collapse_s = 0;
+  __syncthreads();
   double* zp = (double*)smem;                 // [OPB][n]
   double* H = zp + (size_t)OPB * n;           // [p][OPB]
   const int t = threadIdx.x, nt = blockDim.x;
@@ -337,6 +393,7 @@ __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
           for (int u = 0; u < ONU; ++u) a += zr[c][u] * zr[c][u];
           a = wave_sum(a);
           if (a < 1e-6 * pre[c]) redo = true;
+          if (!(a >= 1e-6 * pre[c]) && (round == 1 || j0 == 0) && lane == 0) collapse_s = 1;
           const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
 #pragma unroll
           for (int u = 0; u < ONU; ++u) zr[c][u] *= inv;
@@ -379,6 +436,7 @@ __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
           for (int e = lane; e < n; e += 64) a += zc[e] * zc[e];
           a = wave_sum(a);
           if (a < 1e-6 * pre[c]) redo = true;
+          if (!(a >= 1e-6 * pre[c]) && (round == 1 || j0 == 0) && lane == 0) collapse_s = 1;
           const double inv = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
           for (int e = lane; e < n; e += 64) zc[e] *= inv;
         }
@@ -391,6 +449,9 @@ __global__ __launch_bounds__(1024) void k_orth_block(double* Z, int n, int p) {
     for (int u = t; u < pb * n; u += nt) Z[(int64_t)j0 * n + u] = zp[u];
     __syncthreads();
   }
+  // rank-deficient block: one column-by-column pass that replaces the dependent columns (the
+  // independent ones only move by rounding); H's p * OPB doubles hold the inner products
+  if (collapse_s) wg_orthonormalize(Z, n, p, red_s, H);
 }
 
 // Jacobi eigensolver of the symmetric m x m matrix H (row-major, global, destroyed), one

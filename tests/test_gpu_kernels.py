@@ -443,6 +443,34 @@ def test_eig_top(dev, n, k, p, kind):
         assert Ug[np.argmax(np.abs(Ug[:, c])), c] > 0
 
 
+@pytest.mark.parametrize('n,r,k,p,zero_rows', [(30, 4, 13, 23, 6), (40, 3, 11, 21, 0), (40, 3, 11, 21, 25),
+                                                (300, 5, 20, 30, 100), (2500, 6, 20, 30, 700)])
+@pytest.mark.parametrize('iters', [2, 3])
+def test_eig_top_rank_deficient(dev, n, r, k, p, zero_rows, iters):
+    """Gram matrices of rank r below the sketch size p, with exactly zero rows (an unfolding's
+    rows without entries): the power-step block is rank deficient, and the orthonormalization
+    must still return an orthonormal basis (dependent columns replaced), so the top r eigenpairs
+    are LAPACK's and the others are roundoff-sized.  n = 2500 takes the column CGS2 path."""
+    print('eig rank-deficient', n, r, k, p, zero_rows, iters, flush=True)
+    rs = np.random.RandomState(n + r)
+    X = rs.randn(n, r) * 10.0 ** rs.uniform(0, 1, r)
+    X[rs.permutation(n)[:zero_rows]] = 0.0
+    G = X @ X.T
+    Gd = dev.upload(G.ravel(), torch.float64)
+    Q0 = dev.upload(rs.randn(n, p).ravel(order='F'), torch.float64)
+    U = dev.empty((k, n), torch.float64)
+    lam_d = dev.empty(k, torch.float64)
+    dev.eig_top(Gd, n, Q0, p, k, iters, U, lam_d)
+    w, V = np.linalg.eigh(G)
+    w, V = w[::-1], V[:, ::-1]
+    lam = lam_d.cpu().numpy()
+    assert np.allclose(lam[:r], w[:r], rtol=1e-10, atol=0)
+    assert np.abs(lam[r:]).max() <= 1e-10 * w[0]
+    Ug = U.cpu().numpy().T
+    assert np.linalg.norm(Ug[:, :r] @ (Ug[:, :r].T @ V[:, :r]) - V[:, :r]) < 1e-8
+    assert np.abs(Ug[:, :r].T @ Ug[:, :r] - np.eye(r)).max() < 1e-10
+
+
 def _sym_case(kind, m, rs):
     if kind == 'rand':
         X = rs.randn(m, m)
