@@ -196,6 +196,37 @@ int tfx_gemm_small(tfx_ctx* ctx, int dtype, int transA, int m, int n, int kk, co
 int tfx_convert(tfx_ctx* ctx, int64_t n, const void* src, int src_dtype, void* dst, int dst_dtype,
                 hipStream_t stream);
 
+/* ---- sparse COO input T = [data, idxs, dims] (TensorFox.py:256-280) ----------------------
+ * Entries are handed over as data (nnz, FP64) and coordinates (nnz x L, uint64, row-major, the
+ * caller's mode order).  tfx_sparse_pack turns the coordinates into L columns of nnz int32 in
+ * the sorted mode order (idx[l * nnz + e] = coords[e * L + ordering[l]]), checking every one
+ * against sorted_dims; *bad (device int) is set to 1 when any is out of range (the entry is
+ * then stored as 0).  The other sparse entry points take that packed form and assume it was
+ * validated.  Replaces the coordinate permutation of sort_dims (Auxiliar.py:99-106) and the
+ * index check of sparse_unfold (Conversion.py:293-297). */
+int tfx_sparse_pack(tfx_ctx* ctx, int64_t nnz, int L, const uint64_t* coords, const int32_t* ordering,
+                    const int64_t* sorted_dims, int32_t* idx, int* bad, hipStream_t stream);
+
+/* G = T_(mode) T_(mode)^T (dims[mode] x dims[mode], FP64, zeroed here): entries sorted by the
+ * key of their other coordinates, then every pair in a run of equal keys adds d_a d_b
+ * (duplicate coordinates therefore count as their sum).  Replaces sparse_unfold +
+ * sparse_dot_mkl_call / sparse_dot (Conversion.py:255-302, Compression.py:211-215, 240-258,
+ * MultilinearAlgebra.py:524-571). */
+int tfx_sparse_gram(tfx_ctx* ctx, int64_t nnz, int L, const int64_t* dims, const int32_t* idx,
+                    const double* data, int mode, double* G, hipStream_t stream);
+
+/* Y = T x_0 U^T: dense C-order (k, dims[1], ..., dims[L-1]) FP64 (zeroed here), U dims[0] x k
+ * column-major FP64.  The first mode of sparse_multilin_mult (MultilinearAlgebra.py:145-175,
+ * Critical.py:2704-3096); the remaining modes are dense products (tfx_ttm). */
+int tfx_sparse_ttm0(tfx_ctx* ctx, int64_t nnz, int L, const int64_t* dims, const int32_t* idx,
+                    const double* data, const double* U, int k, double* Y, hipStream_t stream);
+
+/* out[0] = sum_e (data_e - sum_r prod_l F_l[idx_le, r])^2 over the stored entries only; F is
+ * the flat FP64 factor layout (factor l column-major dims[l] x R).  Replaces sparse_fastnorm
+ * (Critical.py:25-53), squared. */
+int tfx_sparse_residual(tfx_ctx* ctx, int64_t nnz, int L, const int64_t* dims, const int32_t* idx,
+                        const double* data, const double* F, int R, double* out, hipStream_t stream);
+
 /* ---- batches of small CPDs --------------------------------------------------------------
  * A batch runs many independent problems of uneven shape and rank, each taken through the
  * reference pipeline with default options (TensorFox.py:424-655: sorted modes, MLSVD 'seq',

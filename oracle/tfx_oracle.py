@@ -399,11 +399,129 @@ def starting_point(S, U, R, ordering, options):
         return [U[l].T @ init[ordering[l]] for l in range(L)]
     if init == 'random':
         F = [np.random.randn(dims[l], R) for l in range(L)]
+    elif init == 'smart_random':
+        F = smart_random(S, dims, R)
+    elif init == 'smart':
+        F = smart(S, dims, R)
     else:
         sys.exit('Error with init parameter initialization.')
     F = clean_zeros(F, dims, R)
     F = equalize(F, R)
     return transform(F, options.symm, options.factors_norm)
+
+
+def smart_sample(S, dims, R):
+    """R core coordinates from the triangular index distribution; the rank-one term r is
+    S[c_r] e_{c_r}.  Follows Initialization.py:144-191 (linear search over the cumulative sums,
+    as assign_values does)."""
+    L = len(dims)
+    F = [np.zeros((dims[l], R)) for l in range(L)]
+    arr = [dims[l] * np.ones(dims[l]) - np.arange(dims[l]) for l in range(L)]
+    high = [np.sum(arr[l]) for l in range(L)]
+    C = [np.random.randint(high[l], size=R) for l in range(L)]
+    for r in range(R):
+        idx = []
+        for l in range(L):
+            for i in range(dims[l]):
+                if np.sum(arr[l][0:i]) <= C[l][r] < np.sum(arr[l][0:i + 1]):
+                    F[l][i, r] = 1
+                    idx.append(i)
+                    break
+        F[0][idx[0], r] = S[tuple(idx)]
+    return F
+
+
+def smart_random(S, dims, R):
+    """Best of 1 + int(sqrt(prod(dims))) smart_sample draws by |S1 - S1_init| / |S| (the full
+    unfolding is formed for every sample).  Follows Initialization.py:97-141."""
+    samples = 1 + int(np.sqrt(int(np.prod(dims))))
+    S1 = unfold(S, 1)
+    Ssize = norm(S)
+    best_error, best = np.inf, None
+    for _ in range(samples):
+        F = smart_sample(S, dims, R)
+        e = norm(S1 - cpd2unfold1(F)) / Ssize
+        if e < best_error:
+            best_error, best = e, [f.copy() for f in F]
+    return best
+
+
+def join_rank1(idx):
+    """Follows Initialization.py:258-324 literally (list.remove on the working copy)."""
+    R = len(idx)
+    L = len(idx[0])
+    count = 0
+    tmp = idx.copy()
+    joined = [[] for _ in range(L * R)]
+    i = 0
+    for l in range(L):
+        indexes = [j for j in range(L) if j != l]
+        r = 0
+        while r < len(tmp):
+            term = tuple(tmp[r][j] for j in indexes)
+            rr = r + 1
+            while rr < len(tmp):
+                if tuple(tmp[rr][j] for j in indexes) == term:
+                    joined[i].append(tmp[rr])
+                    tmp.remove(tmp[rr])
+                    count += 1
+                rr += 1
+            if len(joined[i]) > 0:
+                joined[i].append(tmp[r])
+                tmp.remove(tmp[r])
+            i += 1
+            r += 1
+    return count, [joined[i] for i in range(L * R) if len(joined[i]) > 0]
+
+
+def find_pivot(x):
+    """Follows Initialization.py:327-358."""
+    if len(x) == 1:
+        return 0
+    for l in range(len(x[0])):
+        if x[0][l] != x[1][l]:
+            return l
+
+
+def smart(S, dims, R):
+    """Largest-magnitude entries merged into rank-one terms.  Follows Initialization.py:194-255,
+    with the reference's repeated argmax over |S| (O(|S|^2): small cores only)."""
+    L = len(dims)
+    S_cp = np.abs(S).copy()
+    idx = []
+    for i in range(S.size):
+        idx.append(tuple(int(v) for v in np.unravel_index(np.argmax(S_cp), S_cp.shape)))
+        S_cp[idx[i]] = -np.inf
+    final_idx = idx[:R].copy()
+    idx = idx[R:]
+    previous_count = 0
+    for _ in range(100):
+        count, joined = join_rank1(final_idx)
+        if count - previous_count <= 0:
+            break
+        final_idx = final_idx + idx[:count - previous_count]
+        idx = idx[count - previous_count:]
+        previous_count = count
+    rank1_idx = [[] for _ in range(R)]
+    r = 0
+    for x in joined:
+        for y in x:
+            rank1_idx[r].append(y)
+            final_idx.remove(y)
+        r += 1
+    for x in final_idx:
+        rank1_idx[r].append(x)
+        r += 1
+    F = [np.zeros((dims[l], R)) for l in range(L)]
+    for r in range(R):
+        mode = find_pivot(rank1_idx[r])
+        for x in rank1_idx[r]:
+            for l in range(L):
+                if l == mode:
+                    F[l][x[l], r] += S[x]
+                else:
+                    F[l][x[l], r] = 1
+    return F
 
 
 # ----------------------------------------------------------------------------------------------
@@ -843,6 +961,138 @@ def als(T, factors, R, options):
 # Drivers (TensorFox.py)
 # ----------------------------------------------------------------------------------------------
 
+# ----------------------------------------------------------------------------------------------
+# Sparse COO input T = [data, idxs, dims] (TensorFox.py:256-280, Compression.py:122-137)
+# ----------------------------------------------------------------------------------------------
+
+def sparse_sort_dims(T):
+    """Descending-dimension permutation of the coordinate columns.  Follows Auxiliar.py:99-106."""
+    data, idxs, dims = T
+    idxs = np.array(idxs, dtype=np.uint64)
+    dims = np.array(dims)
+    ordering = np.argsort(-dims)
+    return [data, idxs[:, ordering], tuple(dims[ordering])], ordering
+
+
+def sparse_unfold(data, idxs, dims, mode):
+    """CSR mode-`mode` unfolding (1-based); column = remaining coordinates, first remaining
+    fastest.  Follows Conversion.py:255-302."""
+    from scipy.sparse import coo_matrix
+    L = len(dims)
+    dims = [int(d) for d in dims]
+    rest = [l for l in range(L) if l != mode - 1]
+    K = np.zeros(L, dtype=np.uint64)
+    s = 1
+    for l in rest:
+        K[l] = s
+        s *= dims[l]
+    idxs = np.asarray(idxs, dtype=np.uint64)
+    rows = idxs[:, mode - 1]
+    cols = np.sum(K * idxs, axis=1, dtype=np.uint64)
+    return coo_matrix((np.asarray(data, dtype=np.float64), (rows, cols)),
+                      shape=(dims[mode - 1], s)).tocsr()
+
+
+def sparse_gram(data, idxs, dims, mode):
+    """Dense T_(mode) T_(mode)^T.  Follows Compression.py:211-215 (sparse_dot_mkl / sparse_dot)."""
+    Tl = sparse_unfold(data, idxs, dims, mode)
+    return np.asarray((Tl @ Tl.T).toarray(), dtype=np.float64)
+
+
+def sparse_multilin_mult(UT, data, idxs, dims):
+    """S = (U_1^T, ..., U_L^T) * T = sum_k data_k (x)_l U_l^T[:, idxs[k, l]] (dense core).
+    Follows MultilinearAlgebra.py:145-175 / Critical.py:2704-2721 (orders 3-12)."""
+    L = len(dims)
+    idxs = np.asarray(idxs).astype(np.int64)
+    nnz = len(data)
+    M = np.asarray(data, dtype=np.float64)[None, :]
+    for l in range(L):
+        Ul = UT[l][:, idxs[:, l]]
+        M = (M[:, None, :] * Ul[None, :, :]).reshape(-1, nnz)
+    return M.sum(axis=1).reshape([u.shape[0] for u in UT])
+
+
+def sparse_fastnorm(data, idxs, dims, factors):
+    """|data - (model at the stored coordinates)|, zeros not counted.  Follows Critical.py:25-53."""
+    idxs = np.asarray(idxs).astype(np.int64)
+    P = np.ones((len(data), factors[0].shape[1]))
+    for l in range(len(factors)):
+        P = P * factors[l][idxs[:, l], :]
+    return float(norm(np.asarray(data, dtype=np.float64) - P.sum(axis=1)))
+
+
+def sparse_mlsvd(T, R, options):
+    """Per mode: rSVD of the dense Gram of the sparse unfolding with
+    n_components = int(max(L, log10(I_l^2 L)) * min(R, I_l)), sigma = sqrt(s), energy
+    truncation; then the sparse multilinear product.  Follows Compression.py:122-137, 192-237
+    (sparse CPU branch) and the trunc_dims step :171-177."""
+    data, idxs, dims = T
+    dims = np.array(dims)
+    L = len(dims)
+    tol_mlsvd = _pick_tol(options.tol_mlsvd, L)
+    U, sigmas = [], []
+    for l in range(L):
+        G = sparse_gram(data, idxs, dims, l + 1)
+        low_rank = min(R, dims[l])
+        if options.svd_n_components is None:
+            n_components = int(max(L, np.log10(dims[l] * dims[l] * L)) * low_rank)
+        else:
+            n_components = options.svd_n_components
+        Ul, sl, Vlt = randomized_svd(G, n_components, n_oversamples=10, n_iter=2)
+        sl = np.sqrt(sl)
+        Ul, sl, Vlt, dim = clean_compression(Ul, sl, Vlt, tol_mlsvd, L)
+        U.append(Ul)
+        sigmas.append(sl)
+    S = sparse_multilin_mult([u.T for u in U], data, idxs, dims)
+    if type(options.trunc_dims) == list:
+        slices = []
+        for l in range(L):
+            slices.append(slice(0, options.trunc_dims[l]))
+            U[l] = U[l][:, :min(options.trunc_dims[l], U[l].shape[1])]
+        S = S[tuple(slices)]
+    return S, U, sigmas
+
+
+def sparse_tricpd(T, R, options, stages=None):
+    """tricpd for T = [data, idxs, dims]: sparse MLSVD, dGN/ALS on the dense core, U*W, no
+    refinement, error over the stored entries.  Follows TensorFox.py:457-500, 577-581,
+    627-640 and Auxiliar.py:150-159."""
+    data_orig = np.array(T[0], dtype=np.float64)
+    idxs_orig = np.array(T[1], dtype=np.uint64)
+    dims_orig = np.array(T[2])
+    L = len(dims_orig)
+    tol = options.tol_mlsvd[0] if type(options.tol_mlsvd) == list else options.tol_mlsvd
+    T, ordering = sparse_sort_dims([data_orig, idxs_orig, dims_orig])
+    Tsize = norm(T[0])
+    if tol < 0:
+        options.tol_mlsvd = 1e-16
+    S, U, sigmas = sparse_mlsvd(T, R, options)
+    if options.symm:
+        R_min = min(S.shape)
+        S = S[tuple(slice(R_min) for _ in range(L))]
+        U = [u[:, :R_min] for u in U]
+    init = starting_point(S, U, R, ordering, options)
+    if stages is not None:
+        stages.update(S=S, U=U, init=deepcopy(init), sigmas=sigmas, ordering=ordering)
+    solver = als if options.method == 'als' else dGN
+    factors, ss_m, e_m, im_m, g_m, stop_m = solver(S, init, R, options)
+    factors = [U[l] @ factors[l] for l in range(L)]
+    options.refine = False
+    factors = unsort_dims(factors, ordering)
+    rel_error = sparse_fastnorm(data_orig, idxs_orig, dims_orig, factors) / Tsize
+    z = np.array([0])
+    out = Output(ss_m.size, rel_error, [ss_m, z], [e_m, z], [im_m, z], [g_m, z], [stop_m, 8],
+                 options)
+    return factors, out
+
+
+def sparse2dense(data, idxs, dims):
+    """Dense array of a COO tensor (duplicates summed).  Follows Conversion.py:144-170."""
+    T = np.zeros([int(d) for d in dims])
+    np.add.at(T, tuple(np.asarray(idxs).astype(np.int64).T), np.asarray(data, dtype=np.float64))
+    return T
+
+
 def tricpd(T, R, options, stages=None):
     """sort -> mlsvd -> init -> dGN or ALS -> U*W -> optional refinement -> final error -> unsort.
     Follows TensorFox.py:424-655 (dense, methods 'dGN' and 'als', :563-568, :599-604).  `stages`, if a dict, receives
@@ -882,11 +1132,13 @@ def tricpd(T, R, options, stages=None):
 
 def cpd(T, R, options=False, stages=None):
     """Drop-in dense CPD.  Follows TensorFox.py:69-365 (dGN and ALS branches, dense input)."""
-    dims = T.shape
+    dims = T[2] if type(T) == list else T.shape
     options = make_options(options)
     consistency(R, dims, options)
     if options.method not in ('dGN', 'als'):
         raise NotImplementedError("the oracle covers methods 'dGN' and 'als' only")
+    if type(T) == list:
+        return sparse_tricpd(T, R, options, stages=stages)
     return tricpd(np.asarray(T, dtype=np.float64), R, options, stages=stages)
 
 

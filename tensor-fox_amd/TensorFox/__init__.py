@@ -3,6 +3,7 @@ modules/TensorFox/__init__.py) on hand-written gfx950 HIP kernels.
 
     import TensorFox as tfx
     factors, output = tfx.cpd(T, R, options)
+    factors, output = tfx.cpd([data, idxs, dims], R, options)   # sparse COO input
     loss = tfx.cpd_loss(T, A, B, C)      # differentiable 0.5 ||T - [[A, B, C]]||^2
     results = tfx.cpd_batched(tensors, ranks, options, seeds)   # many small CPDs at once
 """
@@ -15,7 +16,8 @@ def version():
 
 
 from .auxiliar import (Output, consistency, gen_rand_tensor, make_lowrank, make_options,  # noqa: E402
-                       sort_dims, unsort_dims, cpd2tens)
+                       sort_dims, unsort_dims, cpd2tens, gen_rand_sparse_tensor, sparse2dense,
+                       sparse2txt)
 from .tensorfox import cpd, tricpd  # noqa: E402
 from .loss import CPDLoss, cpd_loss  # noqa: E402
 from .batched import BatchedCPD, autotune_batched, cpd_batched  # noqa: E402

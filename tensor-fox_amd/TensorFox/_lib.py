@@ -46,6 +46,10 @@ SIGNATURES = {
     'tfx_ttm': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_vp],
     'tfx_gemm_small': [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_convert': [_c_vp, _c_i64, _c_vp, _c_int, _c_vp, _c_int, _c_vp],
+    'tfx_sparse_pack': [_c_vp, _c_i64, _c_int, _c_vp, _c_vp, _I64P, _c_vp, _c_vp, _c_vp],
+    'tfx_sparse_gram': [_c_vp, _c_i64, _c_int, _I64P, _c_vp, _c_vp, _c_int, _c_vp, _c_vp],
+    'tfx_sparse_ttm0': [_c_vp, _c_i64, _c_int, _I64P, _c_vp, _c_vp, _c_vp, _c_int, _c_vp, _c_vp],
+    'tfx_sparse_residual': [_c_vp, _c_i64, _c_int, _I64P, _c_vp, _c_vp, _c_vp, _c_int, _c_vp, _c_vp],
     'tfx_batch_mlsvd': [_c_vp, _c_vp, _c_int, _c_int, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp, _c_vp,
                         _c_vp, _c_vp],
     'tfx_batch_dgn': [_c_vp, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,

@@ -185,3 +185,61 @@ def make_lowrank(dims, R, nu, seed=0, col_scale=False, collinear=None, dtype=np.
     T0 = cpd2tens(F)
     T = add_relative_noise(T0, nu, rs) if nu else T0
     return T.astype(dtype), F
+
+
+# ----------------------------------------------------------------------------------------------
+# sparse COO tensors [data, idxs, dims]
+# ----------------------------------------------------------------------------------------------
+
+def gen_rand_sparse_tensor(dims, R, nnz):
+    """Sparse random tensor from sparse factors, drawing from the global legacy RNG in the
+    order of Auxiliar.py:575-647 (per mode: randn(nnz) values, randint rows, randint columns), so
+    a seeded call returns the reference's tensor.  As there:
+    - a factor's (row, column) pairs are made unique and sorted, and the first len(unique) values
+      are kept in that order;
+    - the stored coordinates are the set of all combinations of the factors' row indexes (set
+      order, as the reference iterates it);
+    - an entry is sum_r prod over the modes that HAVE an entry (i_l, r) of that entry (a mode
+      without one contributes 1, not 0).
+    Returns (data, idxs, dims, factors) with factors as dicts {(row, col): value}."""
+    import itertools
+    L = len(dims)
+    factors = []
+    for l in range(L):
+        vals = np.random.randn(nnz)
+        rows = np.random.randint(0, dims[l], size=nnz)
+        cols = np.random.randint(0, R, size=nnz)
+        pairs = np.unique(np.stack([rows, cols], axis=1), axis=0)
+        vals = vals[:pairs.shape[0]]
+        factors.append({tuple(p): v for p, v in zip(pairs, vals)})
+    rows_per_mode = [[key[0] for key in factors[l].keys()] for l in range(L)]
+    coords = list(set(itertools.product(*rows_per_mode)))
+    data = []
+    for c in coords:
+        acc = 0
+        for r in range(R):
+            p = 1
+            for l in range(L):
+                v = factors[l].get((c[l], r))
+                if v is not None:
+                    p *= v
+            acc += p
+        data.append(acc)
+    return np.array(data), np.array(coords), np.array(dims), factors
+
+
+def sparse2dense(data, idxs, dims):
+    """Dense FP64 array of a COO tensor, repeated coordinates summed (Conversion.py:144-170)."""
+    T = np.zeros([int(d) for d in dims])
+    idx = np.asarray(idxs).astype(np.int64).reshape(-1, len(dims))
+    np.add.at(T, tuple(idx.T), np.asarray(data, dtype=np.float64))
+    return T
+
+
+def sparse2txt(idxs, data, filename, start=1):
+    """One line per entry: the coordinates shifted by start + 1, then the value (Auxiliar.py:
+    650-673 adds `start` and then 1 again; kept, so files match the reference's)."""
+    with open(filename, 'w') as fh:
+        for i, d in zip(idxs, data):
+            i = np.asarray(i) + start
+            fh.write(''.join('%d ' % x for x in (i + 1)) + str(d) + '\n')

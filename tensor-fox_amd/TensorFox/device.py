@@ -122,6 +122,24 @@ class Device:
         self._call('tfx_gemm_small', dtype_code(B), int(bool(transA)), int(m), int(n), int(kk), _p(A),
                    _p(B), _p(C), self.stream())
 
+    def sparse_pack(self, coords, ordering, sorted_dims, idx, bad):
+        nnz = coords.shape[0] if coords.dim() == 2 else 0
+        order = (ctypes.c_int32 * len(ordering))(*[int(o) for o in ordering])
+        self._call('tfx_sparse_pack', ctypes.c_int64(nnz), len(sorted_dims), _p(coords), order,
+                   _lib.dims_arg(sorted_dims), _p(idx), _p(bad), self.stream())
+
+    def sparse_gram(self, nnz, dims, idx, data, mode, G):
+        self._call('tfx_sparse_gram', ctypes.c_int64(nnz), len(dims), _lib.dims_arg(dims), _p(idx),
+                   _p(data), int(mode), _p(G), self.stream())
+
+    def sparse_ttm0(self, nnz, dims, idx, data, U, k, Y):
+        self._call('tfx_sparse_ttm0', ctypes.c_int64(nnz), len(dims), _lib.dims_arg(dims), _p(idx),
+                   _p(data), _p(U), int(k), _p(Y), self.stream())
+
+    def sparse_residual(self, nnz, dims, idx, data, F, R, out):
+        self._call('tfx_sparse_residual', ctypes.c_int64(nnz), len(dims), _lib.dims_arg(dims),
+                   _p(idx), _p(data), _p(F), int(R), _p(out), self.stream())
+
     def convert(self, src, dst):
         self._call('tfx_convert', ctypes.c_int64(src.numel()), _p(src), dtype_code(src), _p(dst),
                    dtype_code(dst), self.stream())

@@ -18,12 +18,13 @@ from copy import deepcopy
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, sparse
 from .auxiliar import Output, consistency, make_options, sort_dims, unsort_dims
 from .als import als
 from .compression import MAX_MODE_DIM, mlsvd
 from .device import get_device
 from .gauss_newton import Problem, dGN
+from .initialization import smart, smart_random
 
 _CLEAN_RNG = np.random.RandomState(98765)
 
@@ -124,27 +125,47 @@ def tricpd(T, R, options):
     t_all = time.perf_counter()
     timings = {}
     dev = get_device(options.device)
-    precision = _precision_of(T, options)
-    wdt = torch.float64                                   # factors, CG, core
-    tdt = torch.float32 if precision == 'mixed' else torch.float64   # the tensor in HBM
-    prec = _lib.TFX_PREC_NATIVE
-    # compression of an FP32 tensor: bf16x3 MFMA Gram, FP32 MFMA mode product
-    mprec = _lib.TFX_PREC_SPLIT16 if precision == 'mixed' else _lib.TFX_PREC_NATIVE
-    L = T.ndim
-    dims, ordering = sort_dims(T.shape)
-    Tdev = _to_device(dev, T, ordering, tdt)
-    N = int(np.prod(dims))
-    stats = dev.zeros(2, torch.float64)
-    dev.tensor_stats(Tdev, stats)
-    st = stats.cpu().numpy()
-    Tsize = float(np.sqrt(st[1]))
-    mean_abs = float(st[0]) / N
     display = options.display
+    wdt = torch.float64                                   # factors, CG, core
+    prec = _lib.TFX_PREC_NATIVE
+    is_sp = sparse.is_sparse(T)
+    if is_sp:
+        # stored entries in FP64 (TensorFox.py:476-498); the passes over them are FP64
+        data, idxs, dims_in = sparse.coo_arrays(T)
+        if len(data) == 0:
+            raise ValueError('sparse tensor with no stored entries')
+        sparse.report(data, dims_in, display)
+        precision = 'fp64'
+        L = len(dims_in)
+        dims, ordering = sort_dims(tuple(dims_in))
+        Xsp = sparse.SparseTensor(dev, data, idxs, dims_in, ordering)
+        Tsize = Xsp.Tsize
+        sparse.fix_tol(options)
+    else:
+        precision = _precision_of(T, options)
+        tdt = torch.float32 if precision == 'mixed' else torch.float64   # the tensor in HBM
+        # compression of an FP32 tensor: bf16x3 MFMA Gram, FP32 MFMA mode product
+        mprec = _lib.TFX_PREC_SPLIT16 if precision == 'mixed' else _lib.TFX_PREC_NATIVE
+        L = T.ndim
+        dims, ordering = sort_dims(T.shape)
+        Tdev = _to_device(dev, T, ordering, tdt)
+        N = int(np.prod(dims))
+        stats = dev.zeros(2, torch.float64)
+        dev.tensor_stats(Tdev, stats)
+        st = stats.cpu().numpy()
+        Tsize = float(np.sqrt(st[1]))
+        mean_abs = float(st[0]) / N
     t0 = time.perf_counter()
     timings['upload'] = t0 - t_all
 
     # compression
-    S, S_dims, U, sigmas = mlsvd(dev, Tdev, dims, R, options, wdt, mprec)
+    if is_sp:
+        if display != 0:
+            print('-----------------------------------------------------------------------------------------------')
+            print('Computing MLSVD')
+        S, S_dims, U, sigmas = sparse.mlsvd(dev, Xsp, R, options)
+    else:
+        S, S_dims, U, sigmas = mlsvd(dev, Tdev, dims, R, options, wdt, mprec)
     if options.symm:
         R_min = min(S_dims)
         view = S.view(*S_dims)
@@ -157,8 +178,9 @@ def tricpd(T, R, options):
     t1 = time.perf_counter()
     timings['mlsvd'] = t1 - t0
     if display > 0:
-        print('-----------------------------------------------------------------------------------------------')
-        print('Computing MLSVD')
+        if not is_sp:
+            print('-----------------------------------------------------------------------------------------------')
+            print('Computing MLSVD')
         if tuple(S_dims) == tuple(dims):
             print('    No compression detected')
             print('    Working with dimensions', tuple(dims))
@@ -175,14 +197,17 @@ def tricpd(T, R, options):
             B = dev.upload(np.asarray(init[ordering[l]], dtype=np.float64).flatten('F'), wdt)
             dev.gemm_small(True, S_dims[l], R, dims[l], U[l], B, F0[off:off + R * S_dims[l]])
             off += R * S_dims[l]
-    elif init == 'random':
-        F = [np.random.randn(S_dims[l], R) for l in range(L)]
+    elif init in ('random', 'smart_random', 'smart'):
+        if init == 'random':
+            F = [np.random.randn(S_dims[l], R) for l in range(L)]
+        else:
+            # index selection on the core (Initialization.py:97-358), read back once
+            S_host = S.to(torch.float64).cpu().numpy().reshape(S_dims)
+            F = (smart_random if init == 'smart_random' else smart)(S_host, S_dims, R)
         F = _clean_zeros(F, R)
         F = _equalize(F, R)
         F = _transform(F, options.symm, options.factors_norm)
         F0 = dev.upload(_flat(F), wdt)
-    elif init in ('smart', 'smart_random'):
-        raise NotImplementedError("initialization '%s' is not on the MI355X path" % init)
     else:
         sys.exit('Error with init parameter initialization.')
     Sstats = dev.zeros(2, torch.float64)
@@ -218,7 +243,9 @@ def tricpd(T, R, options):
     t4 = time.perf_counter()
     timings['uncompress'] = t4 - t3
 
-    # refinement (TensorFox.py:583-611)
+    # refinement (TensorFox.py:583-611); none for sparse input (TensorFox.py:577-581)
+    if is_sp:
+        options.refine = False
     if options.refine:
         if display > 0:
             print('Computing refinement of solution')
@@ -239,8 +266,12 @@ def tricpd(T, R, options):
     # final error in FP64 and host factors
     F64 = dev.empty(R * sum(dims), torch.float64)
     dev.convert(Ffull, F64)
-    err2 = dev.zeros(1, torch.float64)
-    dev.residual_sq(Tdev, dims, F64, R, err2)
+    if is_sp:
+        # error over the stored entries (Auxiliar.py:150-159, Critical.py:25-53)
+        err2 = sparse.residual_sq(dev, Xsp, F64, R)
+    else:
+        err2 = dev.zeros(1, torch.float64)
+        dev.residual_sq(Tdev, dims, F64, R, err2)
     final_err = float(np.sqrt(err2.cpu().numpy()[0])) / Tsize
     factors = _unflat(F64.cpu().numpy(), dims, R)
     factors = unsort_dims(factors, ordering)
@@ -284,10 +315,13 @@ def cpd(T, R, options=False):
     """Canonical polyadic decomposition of a dense tensor, same call and outputs as
     TensorFox.cpd (TensorFox.py:69-365).  T: NumPy array or torch tensor (CPU or GPU), float32
     or float64; R: int; options: any object whose attributes override the defaults."""
-    dims_orig = tuple(T.shape)
+    dims_orig = tuple(int(d) for d in T[2]) if sparse.is_sparse(T) else tuple(T.shape)
     options = make_options(options)
     consistency(R, dims_orig, options)
-    check_supported(dims_orig)
+    if sparse.is_sparse(T):
+        sparse.check_supported(dims_orig, R, options.svd_n_components)
+    else:
+        check_supported(dims_orig)
     if options.method in ('dGN', 'als'):
         return tricpd(T, R, options)
     raise NotImplementedError("method '%s' is not on the MI355X path (dense 'dGN' and 'als' only)"

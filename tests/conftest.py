@@ -22,6 +22,12 @@ def golden():
 
 
 @pytest.fixture(scope='session')
+def sgolden():
+    import numpy as np
+    return np.load(os.path.join(ROOT, 'tests', 'golden', 'sparse_init_vectors.npz'))
+
+
+@pytest.fixture(scope='session')
 def swimmer():
     import numpy as np
     return np.load(os.path.join(ROOT, 'tests', 'golden', 'swimmer.npz'))['swimmer']
