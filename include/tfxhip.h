@@ -124,6 +124,15 @@ int tfx_cg(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, const voi
            int maxiter, double tol, void* y_flat, void* jtjg_flat, double* scal,
            hipStream_t stream);
 
+/* Direct inner solver (FP64): forms H = J^T J (n x n, n = R * sum(dims)) from the flat factors
+ * and P1 / P2 of tfx_gramians, solves (H + damp diag(Gamma)) y = grad by Cholesky (y = 0 when a
+ * pivot is not positive) and writes scal[0] = 0, scal[1] = ||H y - grad||^2.  n <= 4096.
+ * Replaces direct + hessian + compute_blocks (GaussNewton.py:630-710); the reference's QR of the
+ * row-scaled system M^2 (H + damp diag(Gamma)) has the same solution. */
+int tfx_dgn_direct(tfx_ctx* ctx, int L, const int64_t* dims, int R, const double* F_flat,
+                   const double* P1, const double* P2, const double* gamma, double damp,
+                   const double* grad_flat, double* y_flat, double* scal, hipStream_t stream);
+
 /* x_new = x + alpha*y; F = x2cpd(x_new) with equalize when eq != 0, then transform(symm,
  * factors_norm).  Replaces x = x + y / x2cpd / equalize / transform (GaussNewton.py:285-289,
  * Conversion.py:21-49, 361-387, 412-444).  symm: 0 no symmetric transform; 1 the average of

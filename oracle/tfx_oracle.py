@@ -778,14 +778,61 @@ def compute_dogleg_steps(T, Tsize, factors, grad, JT_J_grad, x, y, error, damp, 
     return best_factors, best_x, best_y, best_error
 
 
+def hessian_blocks(factors, P1, P2):
+    """J^T J assembled by blocks: H_ll = P1_l (x) I, H_{l,m} = (P2_{l,m} (x) 1) o [F_l[i, b] F_m[j, a]]
+    at ((a, i), (b, j)).  Follows hessian / compute_blocks (GaussNewton.py:666-710)."""
+    L = len(factors)
+    R = factors[0].shape[1]
+    dims = [f.shape[0] for f in factors]
+    off = np.concatenate([[0], np.cumsum([R * d for d in dims])])
+    H = np.zeros((off[-1], off[-1]))
+    for l in range(L):
+        H[off[l]:off[l + 1], off[l]:off[l + 1]] = np.kron(P1[l], np.identity(dims[l]))
+        vec_m = None
+        for m in range(l):
+            tmp1 = np.kron(P2[l, m], np.ones((dims[l], dims[m])))
+            vec_m = factors[m].flatten('F')[None, :]
+            tmp2 = np.zeros((R * dims[l], R * dims[m]))
+            for r in range(R):
+                tmp = factors[l][:, r][:, None] @ vec_m
+                for rr in range(R):
+                    tmp2[dims[l] * rr:dims[l] * (rr + 1), dims[m] * r:dims[m] * (r + 1)] = \
+                        tmp[:, dims[m] * rr:dims[m] * (rr + 1)]
+            H[off[l]:off[l + 1], off[m]:off[m + 1]] = tmp1 * tmp2
+            H[off[m]:off[m + 1], off[l]:off[l + 1]] = (tmp1 * tmp2).T
+    return H
+
+
+def direct(T, factors, damp):
+    """QR solve of M^2 (H + damp diag(Gamma)) y = M^2 grad (y = 0 on LinAlgError); returns
+    (y, grad, '-', |H y - grad|).  Follows GaussNewton.py:630-663."""
+    from numpy.linalg import LinAlgError, solve
+    dims = [f.shape[0] for f in factors]
+    _, P1, P2 = gramians(factors)
+    Gamma, gamma = regularization(P1, dims)
+    M = precond(Gamma, gamma, damp, dims)
+    grad = -compute_grad(T, factors, P1)
+    H = hessian_blocks(factors, P1, P2)
+    Hd = H + damp * np.diag(Gamma)
+    MHd = ((Hd.T) * (M ** 2)).T
+    Mgrad = (M ** 2) * grad
+    try:
+        q, r = qr(MHd)
+        y = solve(r, q.T @ Mgrad)
+    except LinAlgError:
+        y = np.zeros(grad.size)
+    return y, grad, '-', norm(H @ y - grad)
+
+
 def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
-    """One dGN step.  Follows GaussNewton.py:254-312 for inner methods 'cg' and 'cg_static'."""
+    """One dGN step.  Follows GaussNewton.py:254-312 (inner methods 'cg', 'cg_static', 'als',
+    'direct'; the dogleg only after a cg step, :298-302)."""
     damp, inner_method, cg_maxiter, cg_factor, cg_tol, tol_jump, symm, factors_norm = inner
     if type(inner_method) == list:
         inner_method = inner_method[it]
     if inner_method == 'cg':
         cg_maxiter = 1 + (L - 2) * int(cg_factor * np.random.randint(1 + it ** 0.4, 2 + it ** 0.9))
-    elif inner_method not in ('cg_static', 'als'):
+    elif inner_method not in ('cg_static', 'als', 'direct'):
         sys.exit("Wrong inner method name. Must be 'cg', 'cg_static', 'als' or 'direct'.")
     if inner_method == 'als':
         # GaussNewton.py:273-276: one ALS sweep replaces the step; y is zeroed, the gradient is
@@ -794,6 +841,9 @@ def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
         x = np.concatenate([f.flatten('F') for f in factors])
         y = np.zeros(x.size)
         grad, JT_J_grad, itn, rn = np.array([np.nan]), None, '-', np.nan
+    elif inner_method == 'direct':
+        y, grad, itn, rn = direct(T, factors, damp)
+        JT_J_grad = None
     else:
         y, grad, JT_J_grad, itn, rn = cg(T, factors, damp, cg_maxiter, cg_tol)
     x = x + y
@@ -802,7 +852,7 @@ def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
     error = residual_error(T, factors, Tsize)
     if inner_method == 'als':
         return factors, x, y, grad, itn, Tsize * error, error
-    if it > 3 and error > tol_jump * old_error:
+    if it > 3 and inner_method in ('cg', 'cg_static') and error > tol_jump * old_error:
         x = x - y
         factors, x, y, error = compute_dogleg_steps(T, Tsize, factors, grad, JT_J_grad, x, y, error,
                                                     damp, symm, factors_norm)

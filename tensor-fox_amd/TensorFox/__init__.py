@@ -21,3 +21,4 @@ from .auxiliar import (Output, consistency, gen_rand_tensor, make_lowrank, make_
 from .tensorfox import cpd, tricpd  # noqa: E402
 from .loss import CPDLoss, cpd_loss  # noqa: E402
 from .batched import BatchedCPD, autotune_batched, cpd_batched  # noqa: E402
+from .drivers import foxit, rank, stats  # noqa: E402

@@ -38,6 +38,8 @@ SIGNATURES = {
                _c_int, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_dgn_update': [_c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_vp, _c_dbl, _c_vp, _c_dbl,
                        _c_int, _c_int, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp],
+    'tfx_dgn_direct': [_c_vp, _c_int, _I64P, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_vp,
+                       _c_vp, _c_vp, _c_vp],
     'tfx_als_solve': [_c_vp, _c_int, _c_int, _I64P, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_dogleg_dots': [_c_vp, _c_int, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_axpby': [_c_vp, _c_int, _c_i64, _c_dbl, _c_vp, _c_dbl, _c_vp, _c_vp, _c_vp],

@@ -93,6 +93,10 @@ class Device:
                    _p(y), float(alpha), _p(grad), float(damp), int(bool(eq)), int(symm),
                    float(factors_norm), _p(x_new), _p(F), _p(scal), self.stream())
 
+    def dgn_direct(self, dims, R, F, P1, P2, gamma, damp, grad, y, scal):
+        self._call('tfx_dgn_direct', len(dims), _lib.dims_arg(dims), int(R), _p(F), _p(P1), _p(P2),
+                   _p(gamma), float(damp), _p(grad), _p(y), _p(scal), self.stream())
+
     def als_solve(self, dims, R, mode, P1, N, F, status):
         """F's block of `mode` <- N's block times pinv(P1[mode]); status: int32 device scalar."""
         self._call('tfx_als_solve', dtype_code(F), len(dims), _lib.dims_arg(dims), int(R), int(mode),

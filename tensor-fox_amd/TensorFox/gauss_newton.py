@@ -1,5 +1,5 @@
 """Damped Gauss-Newton on the GPU (GaussNewton.py:29-251 and compute_step :254-312), with the
-inner methods 'cg', 'cg_static' and 'als'.
+inner methods 'cg', 'cg_static', 'als' and 'direct'.
 
 Per iteration the device runs: Gram/Hadamard products and regularization, the fused MTTKRP,
 minus the gradient, the persistent preconditioned CG, the point update with equalization, and
@@ -217,8 +217,8 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
         elif inner == 'cg_static':
             cap = options.cg_maxiter
         elif inner == 'direct':
-            raise NotImplementedError("inner_method 'direct' is not on the MI355X path; use 'cg', "
-                                      "'cg_static' or 'als'")
+            if prob.wdt != torch.float64:
+                raise NotImplementedError("inner_method 'direct' runs in FP64 only")
         elif inner != 'als':
             sys.exit("Wrong inner method name. Must be 'cg', 'cg_static', 'als' or 'direct'.")
         if inner == 'als':
@@ -264,8 +264,13 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
         dev.gramians(dims, R, prob.F, prob.Gr, prob.P1, prob.P2, prob.gamma)
         dev.mttkrp(prob.T, dims, prob.F, R, prob.N, prec=_mttkrp_prec(prob))
         dev.neg_grad(dims, R, prob.F, prob.P1, prob.N, prob.grad)
-        dev.cg(dims, R, prob.F, prob.P1, prob.P2, prob.gamma, damp, prob.grad, cap, options.cg_tol,
-               prob.y, prob.jtjg, prob.scal[_S_CG:_S_CG + 2])
+        if inner == 'direct':
+            # explicit J^T J and a Cholesky solve (GaussNewton.py:630-710)
+            dev.dgn_direct(dims, R, prob.F, prob.P1, prob.P2, prob.gamma, damp, prob.grad, prob.y,
+                           prob.scal[_S_CG:_S_CG + 2])
+        else:
+            dev.cg(dims, R, prob.F, prob.P1, prob.P2, prob.gamma, damp, prob.grad, cap,
+                   options.cg_tol, prob.y, prob.jtjg, prob.scal[_S_CG:_S_CG + 2])
         # --- x = x + y; factors = transform(equalize(x2cpd(x))); error ----------------------------
         dev.dgn_update(dims, R, x, prob.y, 1.0, prob.grad, damp, True,
                        _symm_mode(symm, factors_norm, aliased), factors_norm, x_spare, prob.F,
@@ -280,7 +285,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
         dx2 = float(h[_S_UPD + 1])
         x2 = float(h[_S_UPD + 2])
         gmax = float(h[_S_UPD + 3])
-        if it > 3 and error > tol_jump * old_error:
+        if it > 3 and inner != 'direct' and error > tol_jump * old_error:
             prob.dogleg_calls += 1
             x_back = torch.empty_like(x)
             dev.axpby(1.0, x, -1.0, prob.y, x_back)
@@ -310,7 +315,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
         improv[it] = errors[it] if it == 0 else abs(errors[it] - errors[it - 1])
         if display > 1:
             print_row(display, it, errors[it], step_sizes[it], improv[it], gradients[it],
-                      float(h[_S_CG + 1]), int(h[_S_CG]))
+                      float(h[_S_CG + 1]), '-' if inner == 'direct' else int(h[_S_CG]))
         code = _stop_code(it, errors, step_sizes, improv, gradients, tol, tol_step, tol_improv,
                           tol_grad, const, error, Tsize)
         if code is not None:

@@ -4,8 +4,9 @@ functions for the sparse COO path and the MLSVD-based starting points, where clo
 fix the result: the seeded generator gen_rand_sparse_tensor (Auxiliar.py:575-647, set order and
 the missing-entry-counts-as-one rule), the sparse Gram / multilinear product / error
 (Conversion.py:255-302, MultilinearAlgebra.py:145-175, 524-571, Critical.py:25-53), seeded sparse
-cpd trajectories (TensorFox.py:424-655 with T = [data, idxs, dims]) and the index selection of
-smart / smart_random (Initialization.py:97-358).
+cpd trajectories (TensorFox.py:424-655 with T = [data, idxs, dims]), the index selection of
+smart / smart_random (Initialization.py:97-358) and seeded trajectories of the direct inner
+solver (GaussNewton.py:630-710).
 
 Run once, in the build container only (the reference checkout is not available elsewhere):
 
@@ -140,6 +141,25 @@ def main(ref_root):
             F = ini.smart_random(S, S.shape, R)
             for l, f in enumerate(F):
                 out['%s_smartrand_R%d_%d' % (name, R, l)] = f
+
+    # --- inner_method 'direct' (GaussNewton.py:630-710) on small dense tensors -----------------
+    dcases = [('d1', (6, 5, 4), 2, 1e-4, 11, {'inner_method': 'direct', 'maxiter': 40}),
+              ('d2', (7, 6, 5), 3, 1e-5, 12, {'inner_method': 'direct', 'refine': True, 'maxiter': 30}),
+              ('d3', (6, 6, 5), 3, 1e-5, 13, {'inner_method': ['direct', 'cg'] * 15, 'maxiter': 30})]
+    for name, dims, R, noise, seed, kw in dcases:
+        rs = np.random.RandomState(seed)
+        F = [rs.randn(d, R) for d in dims]
+        T = cnv.cpd2tens(F)
+        T = T + noise * np.linalg.norm(T) * rs.randn(*dims) / np.sqrt(T.size)
+        out[name + '_T'] = T
+        np.random.seed(seed)
+        f, o = tfx.cpd(T, R, _opts(tfx, **kw))
+        out[name + '_errors0'] = np.asarray(o.errors[0], dtype=np.float64)
+        out[name + '_errors1'] = np.asarray(o.errors[1], dtype=np.float64)
+        out[name + '_rel_error'] = np.array([o.rel_error])
+        out[name + '_meta'] = np.array([R, seed])
+        for l in range(len(dims)):
+            out['%s_f%d' % (name, l)] = f[l]
 
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, len(out), 'arrays')

@@ -150,3 +150,36 @@ def test_smart_random_error_formula_brute_force():
     v = S.ravel()[u]
     err2 = (np.sum(S * S) - v @ v) + np.sum(((1 - m) * v) ** 2)
     assert np.sqrt(err2) == pytest.approx(np.linalg.norm(dense), rel=1e-13)
+
+
+def test_hessian_blocks_equal_explicit_jacobian():
+    """The block assembly of GaussNewton.py:666-710 against J^T J from an explicit Jacobian."""
+    rs = np.random.RandomState(4)
+    for dims, R in [((4, 3, 5), 2), ((3, 4, 2, 3), 3)]:
+        F = [rs.randn(d, R) for d in dims]
+        _, P1, P2 = o.gramians(F)
+        H = o.hessian_blocks(F, P1, P2)
+        ref = o.hessian(F)
+        assert np.abs(H - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+DIRECT = ['d1', 'd2', 'd3']
+
+
+@pytest.mark.parametrize('name', DIRECT)
+def test_direct_inner_solver_trajectory_matches_reference(sgolden, name):
+    kw = {'d1': {'inner_method': 'direct', 'maxiter': 40},
+          'd2': {'inner_method': 'direct', 'refine': True, 'maxiter': 30},
+          'd3': {'inner_method': ['direct', 'cg'] * 15, 'maxiter': 30}}[name]
+    T = sgolden[name + '_T']
+    R, seed = [int(v) for v in sgolden[name + '_meta']]
+    np.random.seed(seed)
+    f, out = o.cpd(T, R, make_opts(kw))
+    for st in (0, 1):
+        ref = sgolden['%s_errors%d' % (name, st)]
+        e = np.asarray(out.errors[st])
+        # the core errors fall to 1e-9 and below, where the iterates are decided by roundoff
+        k = int(np.argmax(ref < 1e-6)) if (ref < 1e-6).any() else ref.size
+        assert e.size >= k
+        assert np.allclose(e[:k], ref[:k], rtol=1e-8, atol=1e-11)
+    assert abs(out.rel_error - float(sgolden[name + '_rel_error'][0])) <= 1e-8
