@@ -824,7 +824,7 @@ def direct(T, factors, damp):
     return y, grad, '-', norm(H @ y - grad)
 
 
-def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
+def compute_step(T, Tsize, factors, x, inner, it, old_error, L, fixed=None):
     """One dGN step.  Follows GaussNewton.py:254-312 (inner methods 'cg', 'cg_static', 'als',
     'direct'; the dogleg only after a cg step, :298-302)."""
     damp, inner_method, cg_maxiter, cg_factor, cg_tol, tol_jump, symm, factors_norm = inner
@@ -849,6 +849,8 @@ def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
     x = x + y
     factors = x2cpd(x, factors)
     factors = transform(factors, symm, factors_norm)
+    if fixed is not None and L == 3:
+        factors[fixed[0]] = deepcopy(fixed[1])
     error = residual_error(T, factors, Tsize)
     if inner_method == 'als':
         return factors, x, y, grad, itn, Tsize * error, error
@@ -859,9 +861,22 @@ def compute_step(T, Tsize, factors, x, inner, it, old_error, L):
     return factors, x, y, grad, itn, rn, error
 
 
+def _split_fixed(factors):
+    """A factor given as [F, l] is held fixed (bicpd).  Follows GaussNewton.py:94-102."""
+    fixed = None
+    factors = list(factors)
+    for l in range(len(factors)):
+        if type(factors[l]) == list:
+            fixed = (l, deepcopy(factors[l][0]))
+            factors[l] = factors[l][0]
+    return factors, fixed
+
+
 def dGN(T, factors, R, options, trace=None):
-    """Damped Gauss-Newton.  Follows GaussNewton.py:29-251 (no fixed mode).  If `trace` is a
-    list, per-iteration (error, damp, gain_ratio, itn) tuples are appended to it."""
+    """Damped Gauss-Newton.  Follows GaussNewton.py:29-251, including a fixed mode (a factor
+    given as [F, l], :94-102, restored after every update, :292-295).  If `trace` is a list,
+    per-iteration (error, damp, gain_ratio, itn) tuples are appended to it."""
+    factors, fixed = _split_fixed(factors)
     init_damp = options.init_damp
     maxiter = options.maxiter
     tol, tol_step, tol_improv, tol_grad = options.tol, options.tol_step, options.tol_improv, options.tol_grad
@@ -886,7 +901,8 @@ def dGN(T, factors, R, options, trace=None):
         old_x, old_error = x, error
         inner = (damp, options.inner_method, options.cg_maxiter, options.cg_factor, options.cg_tol,
                  options.tol_jump, options.symm, options.factors_norm)
-        factors, x, y, grad, itn, rn, error = compute_step(T, Tsize, factors, x, inner, it, old_error, L)
+        factors, x, y, grad, itn, rn, error = compute_step(T, Tsize, factors, x, inner, it, old_error, L,
+                                                           fixed)
         gain_ratio = update_gain_ratio(damp, old_error, error, Tsize, old_x, x, grad)
         if trace is not None:
             trace.append((error, damp, gain_ratio, itn))
@@ -932,13 +948,16 @@ def dGN(T, factors, R, options, trace=None):
 # Alternating least squares (Alternating_Least_Squares.py)
 # ----------------------------------------------------------------------------------------------
 
-def als_iteration(Tl, factors):
+def als_iteration(Tl, factors, fix_mode=-1):
     """One Gauss-Seidel sweep over the modes: F_l <- T_(l) pinv(KR_l^T), with KR_l the
     Khatri-Rao product of the other factors in reversed mode order, each mode using the modes
     already updated in this sweep.  `factors` is modified in place and returned.  Follows
-    Alternating_Least_Squares.py:206-231 (no fixed mode)."""
+    Alternating_Least_Squares.py:206-264; with a fixed mode (third order, bicpd) that mode is
+    skipped and the other two are swept in the same order (:233-262)."""
     L = len(factors)
     for l in range(L):
+        if l == fix_mode:
+            continue
         itr = [m for m in reversed(range(L)) if m != l]
         M = factors[itr[0]]
         for ll in range(L - 2):
@@ -949,9 +968,12 @@ def als_iteration(Tl, factors):
 
 def als(T, factors, R, options):
     """ALS iterations with the reference's stopping rules.  Follows
-    Alternating_Least_Squares.py:18-203 (no fixed mode): the step size is the absolute
+    Alternating_Least_Squares.py:18-203 (a factor given as [F, l] is held fixed, :60-67,
+    119-124): the step size is the absolute
     ||x - old_x|| of the unbalanced point taken before `transform`, the 'gradient' is
     |old_error - error| / step size, and the LAST factors are returned, not the best."""
+    factors, fixed = _split_fixed(factors)
+    fix_mode = fixed[0] if fixed is not None else -1
     maxiter = options.maxiter
     tol, tol_step, tol_improv, tol_grad = options.tol, options.tol_step, options.tol_improv, options.tol_grad
     L = len(factors)
@@ -968,9 +990,11 @@ def als(T, factors, R, options):
     it = 0
     for it in range(maxiter):
         old_x, old_error = x, error
-        factors = als_iteration(Tl, factors)
+        factors = als_iteration(Tl, factors, fix_mode)
         x = np.concatenate([f.flatten('F') for f in factors])
         factors = transform(factors, options.symm, options.factors_norm)
+        if fixed is not None and L == 3:
+            factors[fix_mode] = fixed[1].copy()
         error = fastnorm(Tl[0], cpd2unfold1(factors)) / Tsize
         step_sizes[it] = norm(x - old_x)
         errors[it] = error
@@ -1180,13 +1204,213 @@ def tricpd(T, R, options, stages=None):
     return factors, out
 
 
+# ----------------------------------------------------------------------------------------------
+# Tensor-train CPD, method 'ttcpd' (TensorFox.py:252-421, 658-817, Auxiliar.py:365-527)
+# ----------------------------------------------------------------------------------------------
+
+def tt_core(V, dims, r1, r2, l):
+    """One core of the CPD tensor train: Fortran-order reshape to (r1 I_l) x rest, truncated SVD
+    by scikit-learn's randomized_svd with n_iter=0 and all min(shape) components (the sketch is
+    drawn from the global generator, random_state=None), rank r2.  Follows Auxiliar.py:365-383."""
+    from sklearn.utils.extmath import randomized_svd
+    V = V.reshape(r1 * dims[l], int(np.prod(dims[l + 1:])), order='F')
+    low_rank = min(V.shape[0], V.shape[1])
+    U, S, V = randomized_svd(V, low_rank, n_iter=0)
+    U = U[:, :r2]
+    V = (np.diag(S) @ V)[:r2, :]
+    g = U.reshape(dims[l], r2, order='F') if r1 == 1 else U.reshape(r1, dims[l], r2, order='F')
+    return V, g
+
+
+def cpdtt(T, R):
+    """Tensor train dims[0] x R -> R x dims[1] x R -> ... -> R x dims[L-1].  Follows
+    TensorFox.py:1037-1051."""
+    dims = np.array(T.shape)
+    L = len(dims)
+    G = []
+    r1, r2 = 1, R
+    V = T
+    for l in range(L - 1):
+        V, g = tt_core(V, dims, r1, r2, l)
+        r1, r2 = R, R
+        G.append(g)
+    G.append(V)
+    return G
+
+
+def inflate(T, R, dims):
+    """Modes shorter than R padded to R with 1e-12 randn entries (global generator).  Follows
+    Conversion.py:460-480."""
+    L = len(dims)
+    new_dims = [max(int(d), R) for d in dims]
+    new_T = 1e-12 * np.random.randn(*new_dims)
+    new_T[tuple(slice(int(d)) for d in dims)] = T
+    return new_T
+
+
+def deflate(factors, orig_dims, inflate_status):
+    """Follows Conversion.py:483-495."""
+    if not inflate_status:
+        return factors
+    return [factors[l][:orig_dims[l], :] for l in range(len(orig_dims))]
+
+
+def bicpd(T, R, fixed_factor, options):
+    """Third-order CPD with one factor fixed: MLSVD (no mode sorting), random start with the
+    fixed mode's start U_l^T F_fixed, ALS (bi_method_parameters[0], default) or dGN with that
+    mode held, uncompression of the other two, error with the fixed factor.  Follows
+    TensorFox.py:658-817."""
+    bi_method = options.bi_method_parameters[0]
+    Tsize = norm(T)
+    ordering = [0, 1, 2]
+    consistency(R, T.shape, options)
+    S, U, T1, sigmas = mlsvd(T, Tsize, R, options)
+    if options.symm:
+        R_min = min(S.shape)
+        S = S[:R_min, :R_min, :R_min]
+        U = [u[:, :R_min] for u in U]
+    X, Y, Z = starting_point(S, U, R, ordering, options)
+    mode = fixed_factor[1]
+    init = [X, Y, Z]
+    init[mode] = [U[mode].T @ fixed_factor[0], mode]
+    solver = als if bi_method == 'als' else dGN
+    factors, ss, e, im, g, stop = solver(S, init, R, options)
+    X, Y, Z = factors
+    out = [U[0] @ X, U[1] @ Y, U[2] @ Z]
+    out[mode] = factors[mode]
+    full = list(out)
+    full[mode] = fixed_factor[0]
+    rel_error = fastnorm(T1, cpd2unfold1(full)) / Tsize
+    z = np.array([0])
+    output = Output(ss.size, rel_error, [ss, z], [e, z], [im, z], [g, z], [stop, 5], options)
+    return out[0], out[1], out[2], output
+
+
+def cpd_cores(G, max_trials, epochs, R, display, options):
+    """CPDs of the train's cores: tricpd of G[1] (best of max_trials, early exit below 1e-4),
+    then forward (and on odd epochs backward) sweeps of bicpd with the shared factor fixed to
+    pinv of the neighbour's.  Follows Auxiliar.py:424-527."""
+    L = len(G)
+    if epochs % 2 == 0:
+        epochs += 1
+    cpd_list = list(range(L - 2))
+    outputs = list(range(L - 2))
+    best_error = np.inf
+    for _ in range(max_trials):
+        factors, output = tricpd(G[1], R, options)
+        X, Y, Z = factors
+        if output.rel_error < best_error:
+            best_output, best_error = output, output.rel_error
+            best_X, best_Y, best_Z = X, Y, Z
+            if best_error < 1e-4:
+                break
+    outputs[0] = best_output
+    cpd_list[0] = [best_X, best_Y, best_Z]
+    low, upp = 2, L - 2
+    for epoch in range(epochs):
+        if epoch % 2 == 0:
+            for l in range(low, L - 1):
+                best_error = np.inf
+                fixed_X = np.linalg.pinv(best_Z.T)
+                for _ in range(max_trials):
+                    X, Y, Z, output = bicpd(G[l], R, [fixed_X, 0], options)
+                    if output.rel_error < best_error:
+                        best_output, best_error = output, output.rel_error
+                        best_X, best_Y, best_Z = fixed_X, Y, Z
+                        if best_error < 1e-4:
+                            break
+                if epoch == epochs - 1:
+                    outputs[l - 1] = best_output
+                    cpd_list[l - 1] = [fixed_X, best_Y, best_Z]
+        else:
+            if L == 3:
+                low, upp = 1, L - 1
+            for l in reversed(range(1, upp)):
+                best_error = np.inf
+                fixed_Z = np.linalg.pinv(best_X.T)
+                for _ in range(max_trials):
+                    X, Y, Z, output = bicpd(G[l], R, [fixed_Z, 2], options)
+                    if output.rel_error < best_error:
+                        best_output, best_error = output, output.rel_error
+                        best_X, best_Y, best_Z = X, Y, fixed_Z
+                        if best_error < 1e-4:
+                            break
+                if epoch == epochs - 2:
+                    outputs[l - 1] = best_output
+                    cpd_list[l - 1] = [best_X, best_Y, fixed_Z]
+    return cpd_list, outputs, best_Z
+
+
+def highcpd(T, R, options):
+    """CPD of T from the CPDs of its train cores.  Follows TensorFox.py:368-421."""
+    L = T.ndim
+    options.refine = False
+    G = cpdtt(T, R)
+    cpd_list, outputs, best_Z = cpd_cores(G, options.trials, options.epochs, R, options.display, options)
+    factors = [G[0] @ cpd_list[0][0]]
+    for l in range(L - 2):
+        factors.append(cpd_list[l][1])
+    factors.append(G[-1].T @ best_Z)
+    return equalize(factors, R), outputs
+
+
+class FinalOutputs:
+    """Follows make_final_outputs (Auxiliar.py:223-239)."""
+
+    def __init__(self, num_steps, rel_error, accuracy, outputs, options):
+        self.num_steps = num_steps
+        self.rel_error = rel_error
+        self.accuracy = accuracy
+        self.cpd_output = outputs
+        self.options = options
+
+
+def ttcpd(T, R, options, stages=None):
+    """method 'ttcpd' of cpd: sort, MLSVD, inflate the core to R where shorter, highcpd, deflate,
+    U_l W_l, error.  Follows TensorFox.py:256-365 (dense and sparse input)."""
+    sparse_in = type(T) == list
+    if sparse_in:
+        data_orig = np.array(T[0], dtype=np.float64)
+        idxs_orig = np.array(T[1], dtype=np.uint64)
+        dims_orig = np.array(T[2])
+        tol = options.tol_mlsvd[0] if type(options.tol_mlsvd) == list else options.tol_mlsvd
+        Ts, ordering = sparse_sort_dims([data_orig, idxs_orig, dims_orig])
+        Tsize = norm(Ts[0])
+        if tol < 0:
+            options.tol_mlsvd = 1e-16
+        S, U, _ = sparse_mlsvd(Ts, R, options)
+    else:
+        Ts, ordering = sort_dims(np.asarray(T, dtype=np.float64))
+        Tsize = norm(Ts)
+        S, U, T1, _ = mlsvd(Ts, Tsize, R, options)
+    L = S.ndim
+    S_orig_dims = S.shape
+    inflate_status = R > min(S_orig_dims)
+    if inflate_status:
+        S = inflate(S, R, S_orig_dims)
+    options.trunc_dims = 0
+    factors, outputs = highcpd(S, R, options)
+    factors = deflate(factors, S_orig_dims, inflate_status)
+    factors = [U[l] @ factors[l] for l in range(L)]
+    if stages is not None:
+        stages.update(S=S, U=U, ordering=ordering)
+    if sparse_in:
+        factors = unsort_dims(factors, ordering)
+        rel_error = sparse_fastnorm(data_orig, idxs_orig, dims_orig, factors) / Tsize
+    else:
+        rel_error = fastnorm(T1, cpd2unfold1(factors)) / Tsize
+        factors = unsort_dims(factors, ordering)
+    num_steps = sum(o.num_steps for o in outputs)
+    return factors, FinalOutputs(num_steps, rel_error, max(0, 100 * (1 - rel_error)), outputs, options)
+
+
 def cpd(T, R, options=False, stages=None):
     """Drop-in dense CPD.  Follows TensorFox.py:69-365 (dGN and ALS branches, dense input)."""
     dims = T[2] if type(T) == list else T.shape
     options = make_options(options)
     consistency(R, dims, options)
-    if options.method not in ('dGN', 'als'):
-        raise NotImplementedError("the oracle covers methods 'dGN' and 'als' only")
+    if options.method == 'ttcpd':
+        return ttcpd(T, R, options, stages=stages)
     if type(T) == list:
         return sparse_tricpd(T, R, options, stages=stages)
     return tricpd(np.asarray(T, dtype=np.float64), R, options, stages=stages)

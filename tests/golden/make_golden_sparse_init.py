@@ -5,8 +5,8 @@ fix the result: the seeded generator gen_rand_sparse_tensor (Auxiliar.py:575-647
 the missing-entry-counts-as-one rule), the sparse Gram / multilinear product / error
 (Conversion.py:255-302, MultilinearAlgebra.py:145-175, 524-571, Critical.py:25-53), seeded sparse
 cpd trajectories (TensorFox.py:424-655 with T = [data, idxs, dims]), the index selection of
-smart / smart_random (Initialization.py:97-358) and seeded trajectories of the direct inner
-solver (GaussNewton.py:630-710).
+smart / smart_random (Initialization.py:97-358), seeded trajectories of the direct inner
+solver (GaussNewton.py:630-710) and seeded tensor-train CPDs (method 'ttcpd').
 
 Run once, in the build container only (the reference checkout is not available elsewhere):
 
@@ -49,6 +49,7 @@ def main(ref_root):
     warnings.simplefilter('ignore')
     import TensorFox as tfx
     import TensorFox.Initialization as ini
+    import TensorFox.Auxiliar as aux
     import TensorFox.Compression as cmpr
     import TensorFox.Conversion as cnv
     import TensorFox.MultilinearAlgebra as mla
@@ -158,6 +159,43 @@ def main(ref_root):
         out[name + '_errors1'] = np.asarray(o.errors[1], dtype=np.float64)
         out[name + '_rel_error'] = np.array([o.rel_error])
         out[name + '_meta'] = np.array([R, seed])
+        for l in range(len(dims)):
+            out['%s_f%d' % (name, l)] = f[l]
+
+    # --- method 'ttcpd' (TensorFox.py:252-421, 658-817, Auxiliar.py:365-527) -------------------
+    tcases = [('tt1', (10, 9, 8, 7), 3, 1e-6, 21, {'method': 'ttcpd'}),
+              ('tt2', (6, 6, 5, 5, 4), 2, 1e-6, 22, {'method': 'ttcpd'}),
+              ('tt3', (3, 3, 3, 3), 4, 1e-6, 23, {'method': 'ttcpd'}),
+              ('tt4', (9, 8, 7, 6), 3, 1e-6, 24, {'method': 'ttcpd', 'epochs': 2}),
+              ('tt5', (8, 8, 7, 6), 3, 1e-6, 25, {'method': 'ttcpd', 'bi_method': 'cg'})]
+    for name, dims, R, noise, seed, kw in tcases:
+        rs = np.random.RandomState(seed)
+        F = [rs.randn(d, R) for d in dims]
+        T = cnv.cpd2tens(F)
+        T = T + noise * np.linalg.norm(T) * rs.randn(*dims) / np.sqrt(T.size)
+        out[name + '_T'] = T
+        np.random.seed(seed)
+        # the raw options class: cpd runs make_options itself, and a second pass would reset
+        # bi_method_parameters (only 'bi_method' sets them, Auxiliar.py:312-320)
+        raw = type('options', (), dict(kw))
+        f, o = tfx.cpd(T, R, raw)
+        out[name + '_rel_error'] = np.array([o.rel_error])
+        out[name + '_num_steps'] = np.array([o.num_steps])
+        out[name + '_meta'] = np.array([R, seed])
+        # the stages of the same call: sorted-mode MLSVD core, inflated core, train cores
+        np.random.seed(seed)
+        opts = tfx.make_options(raw)
+        Ts, _ = aux.sort_dims(T)
+        S, U, T1, _ = cmpr.mlsvd(Ts, np.linalg.norm(Ts), R, opts)
+        out[name + '_S'] = S
+        if R > min(S.shape):
+            S = cnv.inflate(S, R, S.shape)
+        out[name + '_Sinf'] = S
+        for i, g in enumerate(tfx.cpdtt(S, R)):
+            out['%s_G%d' % (name, i)] = g
+        for i, co in enumerate(o.cpd_output):
+            out['%s_core%d_errors' % (name, i)] = np.asarray(co.errors[0], dtype=np.float64)
+            out['%s_core%d_rel_error' % (name, i)] = np.array([co.rel_error])
         for l in range(len(dims)):
             out['%s_f%d' % (name, l)] = f[l]
 
