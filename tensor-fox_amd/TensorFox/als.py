@@ -15,21 +15,34 @@ from . import _lib
 from .display import print_header, print_row
 
 
-def als_sweep(prob, F):
+def restore_fixed(prob, F, fixed):
+    """Put the held factor back into the flat factors F (bicpd's fixed mode,
+    Alternating_Least_Squares.py:119-124, GaussNewton.py:292-295)."""
+    if fixed is None or len(prob.dims) != 3:
+        return
+    mode, block = fixed
+    off = prob.R * sum(prob.dims[:mode])
+    F[off:off + block.numel()].copy_(block)
+
+
+def als_sweep(prob, F, fix_mode=-1):
     """One ALS sweep over the flat factors F (device, in place): Alternating_Least_Squares.py:
-    206-231 without a fixed mode."""
+    206-264; a fixed mode (bicpd) is skipped, the others keep their order."""
     dev, dims, R = prob.dev, prob.dims, prob.R
     for l in range(len(dims)):
+        if l == fix_mode:
+            continue
         dev.gramians(dims, R, F, prob.Gr, prob.P1, prob.P2, prob.gamma)
         dev.mttkrp(prob.T, dims, F, R, prob.N, mask=1 << l, prec=prob.prec)
         dev.als_solve(dims, R, l, prob.P1, prob.N, F, prob.als_status)
 
 
-def als(prob, F0, options, Tsize):
+def als(prob, F0, options, Tsize, fixed=None):
     """ALS from the flat factors F0 (device).  Returns (last flat factors, step_sizes, errors,
     improv, gradients, stop) like Alternating_Least_Squares.py:18-203: the step size is the
     absolute ||x - old_x|| of the sweep's factors before `transform`, the gradient entry is
-    |old_error - error| / step size, and the factors of the LAST iteration are returned."""
+    |old_error - error| / step size, and the factors of the LAST iteration are returned.
+    fixed: (mode, flat block) of a factor held fixed (bicpd), restored after every sweep."""
     dev, dims, R = prob.dev, prob.dims, prob.R
     maxiter = options.maxiter
     tol, tol_step, tol_improv, tol_grad = options.tol, options.tol_step, options.tol_improv, options.tol_grad
@@ -54,13 +67,14 @@ def als(prob, F0, options, Tsize):
     it = 0
     for it in range(maxiter):
         old_error = error
-        als_sweep(prob, F)
+        als_sweep(prob, F, fixed[0] if fixed is not None else -1)
         x_new.copy_(F)
         if symm or factors_norm > 0:
             # the sweep assigns a new array to every mode, so the symmetric transform averages
             # separate arrays (Conversion.py:432-444)
             dev.dgn_update(dims, R, x_new, zero, 0.0, zero, 0.0, False, 1 if symm else 0,
                            factors_norm, d, F, sc[8:12])
+        restore_fixed(prob, F, fixed)
         dev.residual_sq(prob.T, dims, F, R, sc[6:7])
         dev.axpby(1.0, x_new, -1.0, x_old, d)
         dev.dogleg_dots(d, d, d, sc[0:4])

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .als import als_sweep
+from .als import als_sweep, restore_fixed
 from .display import print_header, print_row
 
 _S_CG, _S_UPD, _S_ERR = 0, 2, 6     # offsets in the per-iteration scalar vector
@@ -175,12 +175,13 @@ def _symm_mode(symm, factors_norm, aliased):
     return 2 if aliased else 1
 
 
-def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
+def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False, fixed=None):
     """Damped Gauss-Newton from the flat factors F0 (device, working dtype).  Returns
     (best flat factors, step_sizes, errors, improv, gradients, stop) like GaussNewton.py:29-251.
     `aliased`: the starting factors came out of a symmetric transform (random initialization
     with symm, Initialization.py:74-83), so the reference's factor list already holds one shared
-    array."""
+    array.  `fixed`: (mode, flat block) of a factor held fixed (bicpd, GaussNewton.py:94-102),
+    put back after every update (:292-295)."""
     dev = prob.dev
     dims, R = prob.dims, prob.R
     L = len(dims)
@@ -224,7 +225,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
         if inner == 'als':
             # --- compute_step: one ALS sweep replaces the step (GaussNewton.py:273-276); y is
             # zeroed, x is the swept factors, then x2cpd + equalize + transform as below -------
-            als_sweep(prob, prob.F)
+            als_sweep(prob, prob.F, fixed[0] if fixed is not None else -1)
             prob.y.zero_()
             x_als = torch.empty_like(x)
             x_als.copy_(prob.F)
@@ -232,6 +233,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
                            _symm_mode(symm, factors_norm, False), factors_norm, x_spare, prob.F,
                            prob.scal[_S_UPD:_S_UPD + 4])
             aliased = True
+            restore_fixed(prob, prob.F, fixed)
             prob.error_sq(prob.F, prob.scal[_S_ERR:_S_ERR + 1])
             d = torch.empty_like(x)
             dev.axpby(1.0, x_spare, -1.0, x, d)
@@ -276,6 +278,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False):
                        _symm_mode(symm, factors_norm, aliased), factors_norm, x_spare, prob.F,
                        prob.scal[_S_UPD:_S_UPD + 4])
         aliased = True
+        restore_fixed(prob, prob.F, fixed)
         prob.error_sq(prob.F, prob.scal[_S_ERR:_S_ERR + 1])
         h = prob.scal[:8].cpu().numpy()
         old_x = x

@@ -116,6 +116,34 @@ def _to_device(dev, T, ordering, tdt):
     return torch.from_numpy(A).to(dev.dev).view(-1)
 
 
+def starting_point(dev, S, S_dims, U, dims, R, ordering, options, wdt):
+    """Flat starting factors on the core (Initialization.py:20-94): the user's factors projected
+    by U_l^T, or 'random' / 'smart_random' / 'smart' followed by clean-up, equalization and
+    transform.  Draws from the global NumPy generator as the reference does."""
+    L = len(S_dims)
+    init = options.initialization
+    if type(init) == list:
+        F0 = dev.empty(R * sum(S_dims), wdt)
+        off = 0
+        for l in range(L):
+            B = dev.upload(np.asarray(init[ordering[l]], dtype=np.float64).flatten('F'), wdt)
+            dev.gemm_small(True, S_dims[l], R, dims[l], U[l], B, F0[off:off + R * S_dims[l]])
+            off += R * S_dims[l]
+        return F0
+    if init in ('random', 'smart_random', 'smart'):
+        if init == 'random':
+            F = [np.random.randn(S_dims[l], R) for l in range(L)]
+        else:
+            # index selection on the core (Initialization.py:97-358), read back once
+            S_host = S.to(torch.float64).cpu().numpy().reshape(S_dims)
+            F = (smart_random if init == 'smart_random' else smart)(S_host, S_dims, R)
+        F = _clean_zeros(F, R)
+        F = _equalize(F, R)
+        F = _transform(F, options.symm, options.factors_norm)
+        return dev.upload(_flat(F), wdt)
+    sys.exit('Error with init parameter initialization.')
+
+
 def _sync():
     torch.cuda.synchronize()
 
@@ -190,26 +218,7 @@ def tricpd(T, R, options):
 
     # initialization (Initialization.py:20-94)
     init = options.initialization
-    if type(init) == list:
-        F0 = dev.empty(R * sum(S_dims), wdt)
-        off = 0
-        for l in range(L):
-            B = dev.upload(np.asarray(init[ordering[l]], dtype=np.float64).flatten('F'), wdt)
-            dev.gemm_small(True, S_dims[l], R, dims[l], U[l], B, F0[off:off + R * S_dims[l]])
-            off += R * S_dims[l]
-    elif init in ('random', 'smart_random', 'smart'):
-        if init == 'random':
-            F = [np.random.randn(S_dims[l], R) for l in range(L)]
-        else:
-            # index selection on the core (Initialization.py:97-358), read back once
-            S_host = S.to(torch.float64).cpu().numpy().reshape(S_dims)
-            F = (smart_random if init == 'smart_random' else smart)(S_host, S_dims, R)
-        F = _clean_zeros(F, R)
-        F = _equalize(F, R)
-        F = _transform(F, options.symm, options.factors_norm)
-        F0 = dev.upload(_flat(F), wdt)
-    else:
-        sys.exit('Error with init parameter initialization.')
+    F0 = starting_point(dev, S, S_dims, U, dims, R, ordering, options, wdt)
     Sstats = dev.zeros(2, torch.float64)
     dev.tensor_stats(S, Sstats)
     sst = Sstats.cpu().numpy()
@@ -218,9 +227,8 @@ def tricpd(T, R, options):
     t2 = time.perf_counter()
     timings['init'] = t2 - t1
 
-    # main stage on the core: dGN, or ALS (TensorFox.py:563-568)
-    if options.method not in ('dGN', 'als'):
-        raise NotImplementedError("method '%s' is not on the MI355X path" % options.method)
+    # main stage on the core: ALS for method 'als', dGN otherwise (TensorFox.py:563-568; the
+    # tensor-train path calls this for its first core with method 'ttcpd')
     prob = Problem(dev, S, S_dims, R, wdt, prec)
     if options.method == 'als':
         Fm, ss_m, e_m, im_m, g_m, stop_m = als(prob, F0, options, Ssize)
@@ -324,5 +332,5 @@ def cpd(T, R, options=False):
         check_supported(dims_orig)
     if options.method in ('dGN', 'als'):
         return tricpd(T, R, options)
-    raise NotImplementedError("method '%s' is not on the MI355X path (dense 'dGN' and 'als' only)"
-                              % options.method)
+    from .tt import ttcpd
+    return ttcpd(T, R, options)
