@@ -21,6 +21,13 @@
 
 int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const double* F,
                        unsigned mask, double* out, hipStream_t stream);
+int tfx_mttkrp_f64_gemm3(tfx_ctx* ctx, const double* T, const Shape& s, const double* F,
+                         unsigned mask, double* out, hipStream_t stream);
+
+// Third-order tensors of at least this many entries take the two-pass GEMM form
+// (mttkrp_f64_gemm.hip), whose workspace holds an (I0 I1) x 64 matrix.
+constexpr int64_t F64_GEMM_MIN_N = int64_t(1) << 22;
+constexpr int64_t F64_GEMM_MAX_W = int64_t(1) << 29;   // doubles of W (4 GiB)
 
 namespace {
 
@@ -182,6 +189,8 @@ int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const doub
                        unsigned mask, double* out, hipStream_t stream) {
   const int L = s.L;
   if (L < 2 || L > TFX_MAXL) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "MTTKRP order %d", L);
+  if (L == 3 && s.N >= F64_GEMM_MIN_N && s.d[0] * s.d[1] * 64 <= F64_GEMM_MAX_W)
+    return tfx_mttkrp_f64_gemm3(ctx, T, s, F, mask, out, stream);
   M64Plan P;
   P.s = s;
   P.stride[L - 1] = 1;

@@ -100,6 +100,28 @@ def test_mttkrp_fp64_large(dev):
         assert _rel(got[l], o.mttkrp(T, F, l)) < 1e-12
 
 
+@pytest.mark.parametrize('dims,R,mask', [((160, 150, 200), 50, 0b111), ((257, 129, 131), 17, 0b111),
+                                         ((100, 90, 500), 70, 0b111), ((64, 64, 1025), 5, 0b111),
+                                         ((200, 160, 150), 33, 0b100), ((150, 170, 190), 9, 0b011),
+                                         ((130, 140, 300), 64, 0b010)])
+def test_mttkrp_fp64_two_pass(dev, dims, R, mask):
+    """Third-order FP64 tensors of at least 2^22 entries take the two-pass GEMM form (W = T_mat
+    F_2 for modes 0 and 1, T_mat^T KR for mode 2): odd sizes, a K not a multiple of the staged
+    chunk, two rank chunks (R = 70), selected-mode subsets (unselected outputs untouched)."""
+    print('mttkrp fp64 two-pass', dims, R, bin(mask), flush=True)
+    T, F = _case(dims, R, 9)
+    Td = dev.upload(T.ravel(), torch.float64)
+    Fd = dev.upload(_flat(F), torch.float64)
+    out = dev.zeros(R * sum(dims), torch.float64)
+    dev.mttkrp(Td, dims, Fd, R, out, mask=mask)
+    got = _unflat(out.cpu().numpy(), dims, R)
+    for l in range(3):
+        if (mask >> l) & 1:
+            assert _rel(got[l], o.mttkrp(T, F, l)) < 1e-12, l
+        else:
+            assert np.all(got[l] == 0), l
+
+
 def test_mttkrp_mode_mask(dev):
     dims, R = (6, 7, 8), 4
     T, F = _case(dims, R, 2)
