@@ -205,6 +205,41 @@ int tfx_gemm_small(tfx_ctx* ctx, int dtype, int transA, int m, int n, int kk, co
 int tfx_convert(tfx_ctx* ctx, int64_t n, const void* src, int src_dtype, void* dst, int dst_dtype,
                 hipStream_t stream);
 
+/* ---- the dGN loop as one call --------------------------------------------------------------
+ * Options of tfx_dgn_run (the reference's dGN options, GaussNewton.py:29-251).  damp0 is the
+ * initial damping (init_damp * mean|T|, or init_damp[0] for a schedule); symm / factors_norm as
+ * in tfx_dgn_update, `aliased` when the start point already came out of a symmetric transform
+ * (the first update then uses the shared-array form). */
+typedef struct tfx_dgn_opts {
+  int maxiter;
+  double tol, tol_step, tol_improv, tol_grad, tol_jump;
+  double cg_tol, cg_factor;
+  int cg_maxiter;
+  double damp0;
+  int symm, aliased;
+  double factors_norm;
+} tfx_dgn_opts;
+
+/* Doubles of device workspace that tfx_dgn_run needs (host-only, no context). */
+int64_t tfx_dgn_run_work(int L, const int64_t* dims, int R);
+
+/* Damped Gauss-Newton with the CG inner methods from the flat FP64 start F0 (GaussNewton.py:
+ * 29-251, compute_step :254-312, dogleg :713-795, stopping codes 0-7): every iteration runs
+ * tfx_gramians, tfx_mttkrp_mixed, tfx_neg_grad, tfx_cg, tfx_dgn_update and
+ * tfx_cpd_residual_mixed on `stream` and the scalar control flow on the host (one 8-double
+ * readback per iteration).  inner[it] = 0 for 'cg' (cap 1 + (L - 2) int(cg_factor randint(1 +
+ * it^0.4, 2 + it^0.9)) drawn from the MT19937 state mt_key / *mt_pos of NumPy's legacy global
+ * generator, advanced in place, one draw per iteration run), 1 for 'cg_static' (cap cg_maxiter);
+ * damp_sched: NULL, or maxiter damping values (an init_damp list).  Tsize = ||T||.  Outputs:
+ * F_best (device, the best factors), hist (host, 4 x maxiter: step sizes, errors, improvements,
+ * gradient norms), info (host: iterations, stop code, dogleg calls).  work: device, at least
+ * tfx_dgn_run_work doubles.  Replaces dGN and compute_step (GaussNewton.py:29-312) for inner
+ * methods 'cg' / 'cg_static'. */
+int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, int L, const int64_t* dims, int R,
+                const double* F0, const tfx_dgn_opts* opts, const int8_t* inner, const double* damp_sched,
+                uint32_t* mt_key, int* mt_pos, double Tsize, double* work, int64_t work_doubles,
+                double* F_best, double* hist, int* info, hipStream_t stream);
+
 /* ---- sparse COO input T = [data, idxs, dims] (TensorFox.py:256-280) ----------------------
  * Entries are handed over as data (nnz, FP64) and coordinates (nnz x L, uint64, row-major, the
  * caller's mode order).  tfx_sparse_pack turns the coordinates into L columns of nnz int32 in

@@ -52,6 +52,8 @@ SIGNATURES = {
     'tfx_sparse_gram': [_c_vp, _c_i64, _c_int, _I64P, _c_vp, _c_vp, _c_int, _c_vp, _c_vp],
     'tfx_sparse_ttm0': [_c_vp, _c_i64, _c_int, _I64P, _c_vp, _c_vp, _c_vp, _c_int, _c_vp, _c_vp],
     'tfx_sparse_residual': [_c_vp, _c_i64, _c_int, _I64P, _c_vp, _c_vp, _c_vp, _c_int, _c_vp, _c_vp],
+    'tfx_dgn_run': [_c_vp, _c_vp, _c_int, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_vp, _c_vp, _c_vp,
+                    _c_vp, _c_vp, _c_dbl, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_batch_mlsvd': [_c_vp, _c_vp, _c_int, _c_int, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp, _c_vp,
                         _c_vp, _c_vp],
     'tfx_batch_dgn': [_c_vp, _c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
@@ -67,7 +69,19 @@ SIGNATURES = {
 HOST_SIGNATURES = {
     'tfx_mttkrp_plan': [_c_int, _c_int, _c_int, _c_int, _I64P, _c_int, _c_uint],
 }
+# host-only entry points returning int64
+HOST_I64_SIGNATURES = {
+    'tfx_dgn_run_work': [_c_int, _I64P, _c_int],
+}
 TFX_PATH_SPLIT16_MFMA, TFX_PATH_FMA, TFX_PATH_F64_MFMA = 0, 1, 2
+
+
+class DgnOpts(ctypes.Structure):
+    """tfx_dgn_opts (include/tfxhip.h)."""
+    _fields_ = [('maxiter', _c_int), ('tol', _c_dbl), ('tol_step', _c_dbl), ('tol_improv', _c_dbl),
+                ('tol_grad', _c_dbl), ('tol_jump', _c_dbl), ('cg_tol', _c_dbl), ('cg_factor', _c_dbl),
+                ('cg_maxiter', _c_int), ('damp0', _c_dbl), ('symm', _c_int), ('aliased', _c_int),
+                ('factors_norm', _c_dbl)]
 
 
 class BatchOpts(ctypes.Structure):
@@ -94,6 +108,10 @@ def load():
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
+        for name, args in HOST_I64_SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int64
         lib.tfx_last_error.argtypes = [_c_vp]
         lib.tfx_last_error.restype = ctypes.c_char_p
         _lib = lib

@@ -93,6 +93,17 @@ class Device:
                    _p(y), float(alpha), _p(grad), float(damp), int(bool(eq)), int(symm),
                    float(factors_norm), _p(x_new), _p(F), _p(scal), self.stream())
 
+    def dgn_run(self, T, prec, dims, R, F0, opts, inner, damp_sched, mt_key, mt_pos, Tsize, work,
+                F_best, hist, info):
+        """tfx_dgn_run: inner (int8 NumPy), damp_sched (float64 NumPy or None), mt_key (uint32
+        NumPy, advanced in place), mt_pos (ctypes.c_int), hist (float64 NumPy 4 x maxiter), info
+        (ctypes int array of 3)."""
+        ptr = lambda a: ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+        self._call('tfx_dgn_run', _p(T), dtype_code(T), int(prec), len(dims), _lib.dims_arg(dims),
+                   int(R), _p(F0), ctypes.byref(opts), ptr(inner), ptr(damp_sched), ptr(mt_key),
+                   ctypes.byref(mt_pos), float(Tsize), _p(work), ctypes.c_int64(work.numel()),
+                   _p(F_best), ptr(hist), info, self.stream())
+
     def dgn_direct(self, dims, R, F, P1, P2, gamma, damp, grad, y, scal):
         self._call('tfx_dgn_direct', len(dims), _lib.dims_arg(dims), int(R), _p(F), _p(P1), _p(P2),
                    _p(gamma), float(damp), _p(grad), _p(y), _p(scal), self.stream())

@@ -5,7 +5,8 @@ Per iteration the device runs: Gram/Hadamard products and regularization, the fu
 minus the gradient, the persistent preconditioned CG, the point update with equalization, and
 the residual.  The host keeps only the scalar control flow of the reference (CG cap draw from the
 global NumPy RNG, gain ratio, damping, best point, stopping codes), reading one small vector of
-FP64 scalars per iteration.
+FP64 scalars per iteration.  For the CG inner methods (the default) the same loop runs inside the
+library as one call (tfx_dgn_run), without interpreter overhead between the kernel launches.
 """
 
 import sys
@@ -18,6 +19,8 @@ from .als import als_sweep, restore_fixed
 from .display import print_header, print_row
 
 _S_CG, _S_UPD, _S_ERR = 0, 2, 6     # offsets in the per-iteration scalar vector
+# the loop below instead of tfx_dgn_run for the CG inner methods (tests compare the two)
+NATIVE_LOOP_OFF = False
 
 
 class Problem:
@@ -175,6 +178,63 @@ def _symm_mode(symm, factors_norm, aliased):
     return 2 if aliased else 1
 
 
+def _native_inner(options, trace, fixed, prob):
+    """The per-iteration inner methods when the whole loop can run as one library call
+    (tfx_dgn_run): CG inner methods only, FP64 factors, no per-iteration display, trace or fixed
+    mode; None otherwise."""
+    if trace is not None or fixed is not None or options.display > 1 or prob.wdt != torch.float64:
+        return None
+    maxiter = options.maxiter
+    inner = options.inner_method
+    inners = inner if type(inner) == list else [inner] * maxiter
+    if len(inners) < maxiter or any(i not in ('cg', 'cg_static') for i in inners[:maxiter]):
+        return None
+    if type(options.init_damp) == list and len(options.init_damp) < maxiter:
+        return None
+    return np.array([0 if i == 'cg' else 1 for i in inners[:maxiter]], dtype=np.int8)
+
+
+def _dgn_native(prob, F0, options, Tsize, mean_abs, aliased, inner):
+    """dGN as one tfx_dgn_run call: the same kernels, arguments and host arithmetic as the loop
+    below, with the CG caps drawn from the global NumPy generator's MT19937 state inside the
+    library (the state is read before and written back after, advanced by one draw per 'cg'
+    iteration run, as the reference advances it)."""
+    import ctypes
+    dev, dims, R = prob.dev, prob.dims, prob.R
+    maxiter = options.maxiter
+    init_damp = options.init_damp
+    if type(init_damp) == list:
+        damp0 = init_damp[0]
+        sched = np.ascontiguousarray(np.asarray(init_damp[:maxiter], dtype=np.float64))
+    else:
+        damp0 = init_damp * mean_abs
+        sched = None
+    opts = _lib.DgnOpts(int(maxiter), float(options.tol), float(options.tol_step),
+                        float(options.tol_improv), float(options.tol_grad), float(options.tol_jump),
+                        float(options.cg_tol), float(options.cg_factor), int(options.cg_maxiter),
+                        float(damp0), int(bool(options.symm)), int(bool(aliased)),
+                        float(options.factors_norm))
+    st = np.random.get_state()
+    key = np.ascontiguousarray(st[1], dtype=np.uint32).copy()
+    pos = ctypes.c_int(int(st[2]))
+    need = int(dev.lib.tfx_dgn_run_work(len(dims), _lib.dims_arg(dims), int(R)))
+    if getattr(prob, 'run_work', None) is None or prob.run_work.numel() < need:
+        prob.run_work = dev.empty(need, torch.float64)
+    F_best = dev.empty(prob.n, torch.float64)
+    hist = np.zeros(4 * maxiter)
+    info = (ctypes.c_int * 3)()
+    try:
+        dev.dgn_run(prob.T, prob.prec, dims, R, F0, opts, inner, sched, key, pos, Tsize,
+                    prob.run_work, F_best, hist, info)
+    finally:
+        np.random.set_state((st[0], key, pos.value, st[3], st[4]))
+    k, stop = int(info[0]), int(info[1])
+    prob.dogleg_calls += int(info[2])
+    m = maxiter
+    return (F_best, hist[0:k].copy(), hist[m:m + k].copy(), hist[2 * m:2 * m + k].copy(),
+            hist[3 * m:3 * m + k].copy(), stop)
+
+
 def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False, fixed=None):
     """Damped Gauss-Newton from the flat factors F0 (device, working dtype).  Returns
     (best flat factors, step_sizes, errors, improv, gradients, stop) like GaussNewton.py:29-251.
@@ -182,6 +242,9 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False, fixed=Non
     with symm, Initialization.py:74-83), so the reference's factor list already holds one shared
     array.  `fixed`: (mode, flat block) of a factor held fixed (bicpd, GaussNewton.py:94-102),
     put back after every update (:292-295)."""
+    native = _native_inner(options, trace, fixed, prob)
+    if native is not None and not NATIVE_LOOP_OFF:
+        return _dgn_native(prob, F0, options, Tsize, mean_abs, aliased, native)
     dev = prob.dev
     dims, R = prob.dims, prob.R
     L = len(dims)

@@ -31,7 +31,21 @@ def test_library_loads_and_exports_every_declared_symbol():
     # the ctypes prototypes cover every declared compute symbol
     for name in _declared_symbols():
         if name != 'tfx_last_error':
-            assert name in _lib.SIGNATURES or name in _lib.HOST_SIGNATURES, name
+            assert (name in _lib.SIGNATURES or name in _lib.HOST_SIGNATURES
+                    or name in _lib.HOST_I64_SIGNATURES), name
+
+
+def test_dgn_run_workspace_size():
+    """tfx_dgn_run_work (host-only): 18 aligned vectors of R * sum(dims) doubles plus the Gram
+    blocks, and -1 for invalid shapes."""
+    from TensorFox import _lib
+    lib = _lib.load()
+    dims = (50, 40, 30)
+    need = lib.tfx_dgn_run_work(3, _lib.dims_arg(dims), 7)
+    n = 7 * sum(dims)
+    assert need >= 18 * n + 3 * 3 * 49
+    assert lib.tfx_dgn_run_work(1, _lib.dims_arg(dims), 7) == -1
+    assert lib.tfx_dgn_run_work(3, _lib.dims_arg(dims), 0) == -1
 
 
 def test_make_options_matches_oracle():
