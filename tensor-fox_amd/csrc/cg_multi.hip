@@ -25,7 +25,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CT = 16;        // c / r tile of the R x R products
 constexpr int ICH = 128;      // i per workgroup in K1 (split of the reduction over I_m)
-constexpr int KI = 32;        // i staged per step in K1
+constexpr int KI = 64;        // i staged per step in K1 (a 50-row mode in one step)
 constexpr int ZI = 64;        // rows i per workgroup in K2
 constexpr int ZC = 64;        // c per LDS chunk in K2
 constexpr int NTH = 256;
@@ -281,7 +281,6 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   __shared__ double Ts[ZC][CT];
   __shared__ double P1s[ZC][CT];
   const Shape& s = P.s;
-  if (!raw && B.sc[SC_DONE] != 0.0) return;
   int l = 0;
   while (l + 1 < s.L && (int)blockIdx.x >= P.k2first[l + 1]) ++l;
   const int nrb = P.nrb;
@@ -298,6 +297,22 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   // wave w owns rows i0 + 16 w .. + 15 of the 64 x 16 output block, as one FP64 MFMA tile:
   // register j of lane (li, lg) holds row 16 w + 4 j + lg, column r0 + li
   const int wv = t >> 6, li = t & 15, lg = (t >> 4) & 3;
+  // the epilogue's operands, loaded with the stop flag (before the staging loads' results)
+  double eg = 0.0, eM = 0.0, ep[4], er[4];
+  {
+    const int rq = r0 + li < R ? r0 + li : 0;
+    eg = (double)B.gamma[l * R + rq];
+    eM = B.Mv[l * R + rq];
+    const S* rjp = B.r[j & 1];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t i = i0 + 16 * wv + 4 * a + lg;
+      const int64_t e = (i < I && r0 + li < R) ? off + (int64_t)(r0 + li) * I + i : 0;
+      ep[a] = (double)pj[e];
+      er[a] = (double)rjp[e];
+    }
+  }
+  if (!raw && B.sc[SC_DONE] != 0.0) return;
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   const S* qsrc = raw ? B.grad : pj;
   for (int cc0 = 0; cc0 < R; cc0 += ZC) {
@@ -389,13 +404,13 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       if (raw) {
         out[e] = (S)acc[a];
       } else {
-        const double g = (double)B.gamma[l * R + rr];
-        const double M = B.Mv[l * R + rr];
-        const double q = M * (double)pj[e];
+        const double g = eg;
+        const double M = eM;
+        const double q = M * ep[a];
         const S zv = (S)(M * (acc[a] + P.damp * g * q));
         out[e] = zv;
-        const double zd = (double)zv, rd = (double)rj[e];
-        dpz += (double)pj[e] * zd;
+        const double zd = (double)zv, rd = er[a];
+        dpz += ep[a] * zd;
         drz += rd * zd;
         dzz += zd * zd;
         drr += rd * rd;
