@@ -200,9 +200,9 @@ def test_native_batch_draws_match_numpy_stream():
 
 
 def test_mttkrp_planner_routes_large_last_dims_to_fma():
-    """The split16 kernel addresses 64-row tiles with 32-bit byte offsets: a last dimension K with
-    64 K 4 >= 2^31 (or K % 4 != 0) goes to the FMA path; FP64 tensors use the FP64 matrix cores
-    unless an unfolding has 2^31 columns or more."""
+    """The split16 kernel addresses each slab T[o] (J x K) with 32-bit byte offsets of rows up to
+    J + 32: shapes with (J + 32) K 4 > 2^31 - 16 (or K % 4 != 0) go to the FMA path; FP64 tensors
+    use the FP64 matrix cores unless an unfolding has 2^31 columns or more."""
     from TensorFox import _lib
     lib = _lib.load()
     S16, FMA, F64 = _lib.TFX_PATH_SPLIT16_MFMA, _lib.TFX_PATH_FMA, _lib.TFX_PATH_F64_MFMA
@@ -210,9 +210,10 @@ def test_mttkrp_planner_routes_large_last_dims_to_fma():
     def plan(dims, t=_lib.TFX_F32, f=_lib.TFX_F64, prec=_lib.TFX_PREC_SPLIT16, mask=7):
         return lib.tfx_mttkrp_plan(t, f, prec, len(dims), _lib.dims_arg(dims), 50, mask)
     assert plan((400, 400, 400)) == S16
-    assert plan((4, 4, 8388604)) == S16                 # 64 * K * 4 = 2^31 - 1024
-    assert plan((4, 4, 8388608)) == FMA                 # 64 * K * 4 = 2^31
-    assert plan((4, 4, 10000000)) == FMA
+    assert plan((4, 4, 14913080)) == S16                # (4 + 32) K 4 = 2^31 - 128
+    assert plan((4, 4, 14913084)) == FMA                # (4 + 32) K 4 > 2^31 - 16
+    assert plan((2, 100000, 5000)) == S16
+    assert plan((2, 110000, 5000)) == FMA
     assert plan((400, 400, 402)) == FMA                 # K % 4 != 0
     assert plan((30, 20, 10), prec=_lib.TFX_PREC_NATIVE) == FMA
     assert plan((30, 20, 10), t=_lib.TFX_F64, prec=_lib.TFX_PREC_NATIVE) == F64

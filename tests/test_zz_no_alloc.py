@@ -63,7 +63,7 @@ def test_noalloc_mttkrp_refuses_oversized_scratch():
     from TensorFox import _lib
     from TensorFox.device import get_device
     dev = get_device(0)
-    dims, R = (4, 256, 65536), 16        # K-partials alone: 4 j-blocks x 65536 x 64 x 4 B = 64 MiB
+    dims, R = (4, 256, 65536), 16        # K-partials alone: 8 j-blocks x 65536 x 64 x 4 B = 128 MiB
     Td = torch.zeros(int(np.prod(dims)), dtype=torch.float32, device=dev.dev)
     Fd = torch.zeros(R * sum(dims), dtype=torch.float64, device=dev.dev)
     out = torch.zeros_like(Fd)
