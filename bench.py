@@ -31,9 +31,10 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per all-mode split16 MTTKRP call measured with rocprofv3 --pmc FETCH_SIZE and
 # --pmc WRITE_SIZE (separate runs of `bench.py --kernel-only`), each scaled by the counts of a
-# plain 256 MB device copy in the same run (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
-# reads 262.4 MB + writes 12.2 MB (brick partials) + the partial sums (6.1 MB read).
-MEASURED_TRAFFIC = {((400, 400, 400), 50): 280.7e6}
+# plain 256 MB device copy in the same run (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), summed over the
+# call's three kernels: k_mttkrp_mfma reads 261.9 MB (T once) and writes 16.7 MB (partials),
+# k_sum_parts reads 16.7 MB and writes 1.9 MB, k_mttkrp_prep reads 1.0 MB and writes 0.4 MB.
+MEASURED_TRAFFIC = {((400, 400, 400), 50): 298.7e6}
 TRAFFIC_SOURCE = ('rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --kernel-only` '
                   '(profiles/mttkrp_split16_400x400x400_r50.md), not measured in this run')
 FP64_MFMA_PEAK_TFLOPS = 78.6
