@@ -403,7 +403,7 @@ def test_dgn_update(dev, dt, symm, fnorm):
 
 
 GRAM_CASES = [(40, 30, 20), (5, 6, 7, 8), (64, 64, 1), (3, 100, 2), (300, 40, 24), (130, 7, 9),
-              (37, 45, 29), (96, 24, 24, 8)]
+              (37, 45, 29), (96, 24, 24, 8), (400, 64, 40)]
 
 
 @pytest.mark.parametrize('dt', [torch.float64, torch.float32])
