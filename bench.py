@@ -563,8 +563,11 @@ def run(args, pool):
         T0 = tfx.cpd2tens(Ftrue)
         T0n = float(np.linalg.norm(T0))
 
+        cold = None
         for w in range(args.warmup):
             t, _, out = time_cpd(tfx, Tdev, R, 'mixed', seed)
+            if w == 0:
+                cold = t
             log('warmup %d: %.3f s rel_error %.3e' % (w, t, out.rel_error))
         times, outs, facs = [], [], []
         torch.cuda.synchronize()
@@ -609,6 +612,9 @@ def run(args, pool):
             'main_iterations': n_main,
             'main_seconds_per_iteration': round(out.timings['main'] / n_main, 6),
             'stage_seconds': {a: round(b, 5) for a, b in out.timings.items()},
+            'cold_seconds': None if cold is None else round(cold, 5),
+            'cold_note': 'first cpd call of the process (the first warm-up): code-object loading, '
+                         'context and workspace allocation, first launches; `value` is warm',
         }
         if base is not None:
             result['oracle'] = {
