@@ -18,8 +18,14 @@ struct tfx_ctx {
   char err[512];
   void* ws;           // device workspace, grown on demand
   size_t ws_size;
-  double* dscal;      // small device scratch for internal reductions
+  double* dscal;      // device scratch for cross-workgroup sums (TFX_DSCAL_BYTES, zeroed at
+                      // creation; calls that use it on one context are stream-ordered)
 };
+
+// Context scratch: 8 doubles of counters (word 0: the rank-update kernel's arrival counter, reset
+// by the kernel itself), then the per-rank partials of tfx_dgn_update (4 doubles per rank).
+constexpr size_t TFX_DSCAL_BYTES = (size_t)64 << 10;
+constexpr int TFX_UPD_MAX_RANKS = (int)((TFX_DSCAL_BYTES / 8 - 8) / 4);
 
 int tfx_set_error(tfx_ctx* ctx, int code, const char* fmt, ...);
 

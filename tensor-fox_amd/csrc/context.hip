@@ -77,9 +77,15 @@ extern "C" int tfx_ctx_create(int device, tfx_ctx** out) {
   ctx->cu_count = prop.multiProcessorCount;
   ctx->arch_major = prop.major;
   ctx->arch_minor = prop.minor;
-  if (hipMalloc(&ctx->dscal, 4096) != hipSuccess) {
+  if (hipMalloc(&ctx->dscal, TFX_DSCAL_BYTES) != hipSuccess) {
     free(ctx);
     return TFX_E_NOMEM;
+  }
+  if (hipMemset(ctx->dscal, 0, TFX_DSCAL_BYTES) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
+    (void)hipFree(ctx->dscal);
+    free(ctx);
+    return TFX_E_HIP;
   }
   *out = ctx;
   return TFX_OK;

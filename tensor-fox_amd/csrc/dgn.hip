@@ -568,6 +568,216 @@ __global__ __launch_bounds__(1024) void k_dgn_update(Shape s, const S* __restric
   }
 }
 
+
+// k_dgn_update with one workgroup per rank r (factors_norm == 0): every step of the update --
+// x_new = x + alpha y, F = x2cpd(x_new), the column norms, equalize (Conversion.py:361-387) and
+// the symmetric transforms (Conversion.py:412-444) -- mixes only the L columns (l, r) of one
+// rank, so a workgroup does all of it for its rank without exchanging data with the others.
+// The four step scalars are summed over the ranks by the last workgroup to finish (an atomic
+// counter in the context's scratch, reset by that workgroup; no workgroup waits for another),
+// in rank order, so the result does not depend on scheduling.  Columns of up to 4 x 256 rows
+// keep their new values in registers for the scaling; longer ones re-read F.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+constexpr int UPD_NT = 256;
+constexpr int UPD_VC = 4;
+
+template <typename S>
+__global__ __launch_bounds__(UPD_NT) void k_dgn_update_rk(
+    Shape s, const S* __restrict__ x, const S* __restrict__ y, double alpha,
+    const S* __restrict__ g, double damp, int eq, int symm, S* __restrict__ xn,
+    S* __restrict__ F, double* __restrict__ scal, unsigned* __restrict__ counter,
+    double* __restrict__ part) {
+  const int r = blockIdx.x;
+  const int L = s.L, R = s.R;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  __shared__ double red[UPD_NT / 64][TFX_MAXL + 4];
+  __shared__ double cn[TFX_MAXL], sc[TFX_MAXL];
+  __shared__ int is_last;
+  int64_t maxd = 0;
+  for (int l = 0; l < L; ++l) maxd = s.d[l] > maxd ? s.d[l] : maxd;
+  const bool cached = maxd <= (int64_t)UPD_NT * UPD_VC;
+  double gain = 0.0, dy2 = 0.0, x2 = 0.0, gmax = 0.0;
+  double ss[TFX_MAXL];
+  S vc[TFX_MAXL][UPD_VC];
+#pragma unroll
+  for (int l = 0; l < TFX_MAXL; ++l) {
+    ss[l] = 0.0;
+#pragma unroll
+    for (int u = 0; u < UPD_VC; ++u) vc[l][u] = S(0);
+  }
+#pragma unroll
+  for (int l = 0; l < TFX_MAXL; ++l) {
+    if (l < L) {
+      const int64_t I = s.d[l], base = s.off[l] + (int64_t)r * I;
+      for (int64_t i0 = 0; i0 < I; i0 += (int64_t)UPD_NT * UPD_VC) {
+        S xv[UPD_VC], yv[UPD_VC], gv[UPD_VC];
+#pragma unroll
+        for (int u = 0; u < UPD_VC; ++u) {
+          const int64_t i = i0 + t + UPD_NT * u;
+          const int64_t e = i < I ? base + i : base;
+          xv[u] = x[e];
+          yv[u] = y[e];
+          gv[u] = g[e];
+        }
+#pragma unroll
+        for (int u = 0; u < UPD_VC; ++u) {
+          const int64_t i = i0 + t + UPD_NT * u;
+          if (i < I) {
+            const int64_t e = base + i;
+            const double xo = (double)xv[u];
+            const S v = xv[u] + (S)alpha * yv[u];
+            xn[e] = v;
+            F[e] = v;
+            vc[l][u] = v;
+            const double dy = (double)(v - xv[u]);   // x_new - x_old as the reference forms it
+            gain += dy * ((double)gv[u] + damp * dy);
+            dy2 += dy * dy;
+            x2 += xo * xo;
+            const double ag = fabs((double)gv[u]);
+            gmax = ag > gmax ? ag : gmax;
+            ss[l] += (double)v * (double)v;
+          }
+        }
+      }
+    }
+  }
+  // column norms and this rank's scalars: wave sums, then the waves in order
+#pragma unroll
+  for (int l = 0; l < TFX_MAXL; ++l)
+    if (l < L) ss[l] = wave_sum(ss[l]);
+  gain = wave_sum(gain);
+  dy2 = wave_sum(dy2);
+  x2 = wave_sum(x2);
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_xor(gmax, o, 64);
+    gmax = u > gmax ? u : gmax;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int l = 0; l < TFX_MAXL; ++l)
+      if (l < L) red[wave][l] = ss[l];
+    red[wave][TFX_MAXL + 0] = gain;
+    red[wave][TFX_MAXL + 1] = dy2;
+    red[wave][TFX_MAXL + 2] = x2;
+    red[wave][TFX_MAXL + 3] = gmax;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double q[4] = {0.0, 0.0, 0.0, red[0][TFX_MAXL + 3]};
+    for (int w = 0; w < UPD_NT / 64; ++w) {
+      q[0] += red[w][TFX_MAXL + 0];
+      q[1] += red[w][TFX_MAXL + 1];
+      q[2] += red[w][TFX_MAXL + 2];
+      q[3] = red[w][TFX_MAXL + 3] > q[3] ? red[w][TFX_MAXL + 3] : q[3];
+    }
+    for (int l = 0; l < L; ++l) {
+      double a = 0.0;
+      for (int w = 0; w < UPD_NT / 64; ++w) a += red[w][l];
+      cn[l] = sqrt(a);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) part[(int64_t)r * 4 + k] = q[k];
+    // scales: equalize (symm != 2) or the shared-array rule on the last mode (symm == 2)
+    for (int l = 0; l < L; ++l) {
+      double prod = 1.0;
+      if (symm == 2) {
+        const double nr = cn[L - 1];
+        for (int k = 0; k < L; ++k) prod *= nr;
+        sc[l] = prod != 0.0 ? pow(prod, 1.0 / L) / nr : 0.0;
+      } else {
+        for (int k = 0; k < L; ++k) prod *= cn[k];
+        sc[l] = prod != 0.0 ? pow(prod, 1.0 / L) / cn[l] : 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (eq && symm != 2) {
+#pragma unroll
+    for (int l = 0; l < TFX_MAXL; ++l) {
+      if (l < L) {
+        const double f = sc[l];
+        const int64_t I = s.d[l], base = s.off[l] + (int64_t)r * I;
+        if (f != 0.0) {
+          if (cached) {
+#pragma unroll
+            for (int u = 0; u < UPD_VC; ++u) {
+              const int64_t i = t + UPD_NT * u;
+              if (i < I) F[base + i] = (S)(f * (double)vc[l][u]);
+            }
+          } else {
+            for (int64_t i = t; i < I; i += UPD_NT) F[base + i] = (S)(f * (double)F[base + i]);
+          }
+        }
+      }
+    }
+    if (symm == 1) __syncthreads();
+  }
+  if (symm == 1) {
+    // transform on separate factor arrays (Conversion.py:432-437), this rank's rows
+    const int64_t I = s.d[0];
+    for (int64_t i = t; i < I; i += UPD_NT) {
+      const int64_t e = (int64_t)r * I + i;
+      S acc = F[e];
+      for (int l = 1; l < L; ++l) acc += F[s.off[l] + e];
+      acc = acc / (S)L;
+      for (int l = 0; l < L; ++l) F[s.off[l] + e] = acc;
+    }
+  } else if (symm == 2) {
+    // the shared-array rule of k_dgn_update (Conversion.py:39-44, 378-385, 433-438)
+    const int64_t I = s.d[0];
+    const int64_t last = s.off[L - 1];
+    const double f = sc[0];
+    const double dbl = (double)(1 << (L - 1));
+    for (int64_t i = t; i < I; i += UPD_NT) {
+      const int64_t e = (int64_t)r * I + i;
+      S v = F[last + e];
+      if (eq && f != 0.0)
+        for (int k = 0; k < L; ++k) v = (S)(f * (double)v);
+      v = (S)((double)v * dbl) / (S)L;
+      for (int l = 0; l < L; ++l) F[s.off[l] + e] = v;
+    }
+  }
+  // the last workgroup sums the ranks' scalars in rank order
+  if (t == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    is_last = prev == (unsigned)(gridDim.x - 1);
+  }
+  __syncthreads();
+  if (is_last) {
+    // all threads load a chunk of ranks' partials at once, then thread 0 adds them in order
+    __shared__ double pbuf[UPD_NT][4];
+    __threadfence();
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r0 = 0; r0 < R; r0 += UPD_NT) {
+      if (r0 + t < R) {
+        const f64x2* pr = (const f64x2*)(part + (int64_t)(r0 + t) * 4);
+        const f64x2 a = __builtin_nontemporal_load(pr), b = __builtin_nontemporal_load(pr + 1);
+        pbuf[t][0] = a[0];
+        pbuf[t][1] = a[1];
+        pbuf[t][2] = b[0];
+        pbuf[t][3] = b[1];
+      }
+      __syncthreads();
+      if (t == 0) {
+        const int n = R - r0 < UPD_NT ? R - r0 : UPD_NT;
+        for (int u = 0; u < n; ++u) {
+          q[0] += pbuf[u][0];
+          q[1] += pbuf[u][1];
+          q[2] += pbuf[u][2];
+          q[3] = (r0 == 0 && u == 0) || pbuf[u][3] > q[3] ? pbuf[u][3] : q[3];
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) scal[k] = q[k];
+      *counter = 0u;
+    }
+  }
+}
+
 template <typename S>
 __global__ __launch_bounds__(1024) void k_dogleg_dots(int64_t n, const S* __restrict__ y,
                                                        const S* __restrict__ g,
@@ -731,6 +941,24 @@ extern "C" int tfx_dgn_update(tfx_ctx* ctx, int dtype, int L, const int64_t* dim
   if (symm)
     for (int l = 1; l < L; ++l)
       if (dims[l] != dims[0]) return tfx_set_error(ctx, TFX_E_ARG, "symm needs equal dims");
+  if (factors_norm <= 0.0 && R <= TFX_UPD_MAX_RANKS) {
+    // per-rank form: scalars summed through the context scratch (counter + R x 4 partials)
+    unsigned* counter = (unsigned*)ctx->dscal;
+    double* part = ctx->dscal + 8;
+    if (dtype == TFX_F32)
+      hipLaunchKernelGGL(k_dgn_update_rk<float>, dim3(R), dim3(UPD_NT), 0, stream, s,
+                         (const float*)x_flat, (const float*)y_flat, alpha, (const float*)grad_flat,
+                         damp, eq, symm, (float*)x_new_flat, (float*)F_flat, scal, counter, part);
+    else if (dtype == TFX_F64)
+      hipLaunchKernelGGL(k_dgn_update_rk<double>, dim3(R), dim3(UPD_NT), 0, stream, s,
+                         (const double*)x_flat, (const double*)y_flat, alpha,
+                         (const double*)grad_flat, damp, eq, symm, (double*)x_new_flat,
+                         (double*)F_flat, scal, counter, part);
+    else
+      return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+    TFX_CHECK_LAUNCH(ctx);
+    return TFX_OK;
+  }
   const size_t lds = sizeof(double) * 2 * (size_t)L * R;
   if (lds > 120 * 1024) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "L*R too large");
   if (lds > 64 * 1024) {
