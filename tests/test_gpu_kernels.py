@@ -64,7 +64,11 @@ def test_mttkrp_all_modes(dev, dims, R, dt):
 
 
 SPLIT_CASES = [((64, 100, 96), 50), ((130, 70, 400), 50), ((20, 30, 40, 48), 24), ((40, 96, 96), 70),
-               ((300, 128), 20), ((10, 20, 30), 5), ((3, 5, 4, 2, 8), 3)]
+               ((300, 128), 20), ((10, 20, 30), 5), ((3, 5, 4, 2, 8), 3),
+               # K % 8 == 4 (the upper half of a lane's last 8-column group is past the row), a
+               # ragged 36-row j range, and an order-4 tensor with 1200 outer indices (several o
+               # ranges per block group)
+               ((50, 36, 100), 17), ((30, 40, 36, 20), 9)]
 
 
 @pytest.mark.parametrize('dims,R', SPLIT_CASES)
