@@ -230,9 +230,10 @@ __global__ void k_mttkrp_prep(const double* __restrict__ F, MPlan P, float* __re
   const int L = s.L, R = s.R;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < P.NO * RC) {
-    const int64_t o = e / RC;
-    const int ln = (int)(e % RC) >> 2, cb = (int)e & 3;
-    const int r = P.rc0 + 16 * cb + ln;
+    // consecutive threads take consecutive o of one rank: contiguous reads of the factor columns
+    const int64_t o = e % P.NO;
+    const int q = (int)(e / P.NO), ln = q & 15, cb = q >> 4;
+    const int r = P.rc0 + q;
     float f = 0.f;
     if (r < R) {
       double fd = 1.0;
@@ -244,7 +245,7 @@ __global__ void k_mttkrp_prep(const double* __restrict__ F, MPlan P, float* __re
       }
       f = (float)fd;
     }
-    fo[e] = f;
+    fo[o * RC + ln * 4 + cb] = f;
     return;
   }
   e -= P.NO * RC;
