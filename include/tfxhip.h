@@ -114,6 +114,14 @@ int tfx_gramians(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, con
 int tfx_neg_grad(tfx_ctx* ctx, int dtype, int L, const int64_t* dims, int R, const void* F_flat,
                  const void* P1, const void* N_flat, void* grad_flat, hipStream_t stream);
 
+/* tfx_mttkrp_mixed over every mode followed by tfx_neg_grad, with the same results: N_flat =
+ * T_(l) KR_l and grad_flat = N_flat - F_l P1_l.  For FP64 tensors on the FP64 matrix-core
+ * MTTKRP the gradient is formed in the MTTKRP's final sum (one launch fewer).
+ * Replaces compute_grad (GaussNewton.py:401-427) with the sign flip of cg (GaussNewton.py:339). */
+int tfx_mttkrp_neg_grad(tfx_ctx* ctx, const void* T, int t_dtype, int L, const int64_t* dims,
+                        const void* F_flat, int f_dtype, int R, const void* P1, void* N_flat,
+                        void* grad_flat, int prec, hipStream_t stream);
+
 /* Split-preconditioned CG on (J^T J + damp Gamma) y = grad, one persistent workgroup.
  * Replaces cg + cg_iterations + matvec + precond (GaussNewton.py:315-398, 551-581, 609-627),
  * including the aliasing of P and residual_cg in the first iteration (GaussNewton.py:349-392).
@@ -225,7 +233,7 @@ int64_t tfx_dgn_run_work(int L, const int64_t* dims, int R);
 
 /* Damped Gauss-Newton with the CG inner methods from the flat FP64 start F0 (GaussNewton.py:
  * 29-251, compute_step :254-312, dogleg :713-795, stopping codes 0-7): every iteration runs
- * tfx_gramians, tfx_mttkrp_mixed, tfx_neg_grad, tfx_cg, tfx_dgn_update and
+ * tfx_gramians, tfx_mttkrp_neg_grad, tfx_cg, tfx_dgn_update and
  * tfx_cpd_residual_mixed on `stream` and the scalar control flow on the host (one 8-double
  * readback per iteration).  inner[it] = 0 for 'cg' (cap 1 + (L - 2) int(cg_factor randint(1 +
  * it^0.4, 2 + it^0.9)) drawn from the MT19937 state mt_key / *mt_pos of NumPy's legacy global

@@ -34,6 +34,8 @@ SIGNATURES = {
     'tfx_tensor_stats': [_c_vp, _c_vp, _c_int, _c_i64, _c_vp, _c_vp],
     'tfx_gramians': [_c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_neg_grad': [_c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp],
+    'tfx_mttkrp_neg_grad': [_c_vp, _c_vp, _c_int, _c_int, _I64P, _c_vp, _c_int, _c_int, _c_vp, _c_vp,
+                            _c_vp, _c_int, _c_vp],
     'tfx_cg': [_c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_vp,
                _c_int, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp],
     'tfx_dgn_update': [_c_vp, _c_int, _c_int, _I64P, _c_int, _c_vp, _c_vp, _c_dbl, _c_vp, _c_dbl,

@@ -79,6 +79,11 @@ class Device:
         self._call('tfx_gramians', dtype_code(F), len(dims), _lib.dims_arg(dims), int(R), _p(F),
                    _p(Gr), _p(P1), _p(P2), _p(gamma), self.stream())
 
+    def mttkrp_neg_grad(self, T, dims, F, R, P1, N, grad, prec=_lib.TFX_PREC_NATIVE):
+        """N = the MTTKRP of every mode, grad = N - F_l P1_l (minus the gradient)."""
+        self._call('tfx_mttkrp_neg_grad', _p(T), dtype_code(T), len(dims), _lib.dims_arg(dims), _p(F),
+                   dtype_code(F), int(R), _p(P1), _p(N), _p(grad), int(prec), self.stream())
+
     def neg_grad(self, dims, R, F, P1, N, grad):
         self._call('tfx_neg_grad', dtype_code(F), len(dims), _lib.dims_arg(dims), int(R), _p(F),
                    _p(P1), _p(N), _p(grad), self.stream())

@@ -327,8 +327,7 @@ def dGN(prob, F0, options, Tsize, mean_abs, trace=None, aliased=False, fixed=Non
             continue
         # --- compute_step: cg ------------------------------------------------------------------
         dev.gramians(dims, R, prob.F, prob.Gr, prob.P1, prob.P2, prob.gamma)
-        dev.mttkrp(prob.T, dims, prob.F, R, prob.N, prec=_mttkrp_prec(prob))
-        dev.neg_grad(dims, R, prob.F, prob.P1, prob.N, prob.grad)
+        dev.mttkrp_neg_grad(prob.T, dims, prob.F, R, prob.P1, prob.N, prob.grad, prec=_mttkrp_prec(prob))
         if inner == 'direct':
             # explicit J^T J and a Cholesky solve (GaussNewton.py:630-710)
             dev.dgn_direct(dims, R, prob.F, prob.P1, prob.P2, prob.gamma, damp, prob.grad, prob.y,

@@ -244,8 +244,7 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
       cap = o->cg_maxiter;
     }
     RUN(tfx_gramians(ctx, TFX_F64, L, dims, R, b.F, b.Gr, b.P1, b.P2, b.gamma, st));
-    RUN(tfx_mttkrp_mixed(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, (1u << L) - 1, b.N, prec, st));
-    RUN(tfx_neg_grad(ctx, TFX_F64, L, dims, R, b.F, b.P1, b.N, b.grad, st));
+    RUN(tfx_mttkrp_neg_grad(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, b.P1, b.N, b.grad, prec, st));
     RUN(tfx_cg(ctx, TFX_F64, L, dims, R, b.F, b.P1, b.P2, b.gamma, damp, b.grad, cap, o->cg_tol, b.y, b.jtjg,
                b.scal, st));
     RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, x, b.y, 1.0, b.grad, damp, 1, symm_now, o->factors_norm, xs, b.F,

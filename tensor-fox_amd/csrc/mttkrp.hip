@@ -516,6 +516,31 @@ extern "C" int tfx_mttkrp_mixed(tfx_ctx* ctx, const void* T, int t_dtype, int L,
                                        (double*)out_flat, stream);
 }
 
+int tfx_mttkrp_f64_uses_gemm3(const Shape& s);
+int tfx_mttkrp_f64mfma_g(tfx_ctx* ctx, const double* T, const Shape& s, const double* F,
+                         unsigned mask, double* out, const double* P1, double* grad,
+                         hipStream_t stream);
+
+extern "C" int tfx_mttkrp_neg_grad(tfx_ctx* ctx, const void* T, int t_dtype, int L,
+                                   const int64_t* dims, const void* F_flat, int f_dtype, int R,
+                                   const void* P1, void* N_flat, void* grad_flat, int prec,
+                                   hipStream_t stream) {
+  if (!ctx || !T || !F_flat || !P1 || !N_flat || !grad_flat)
+    return tfx_set_error(ctx, TFX_E_ARG, "null argument");
+  Shape s;
+  int rc = tfx_make_shape(ctx, L, dims, R, &s);
+  if (rc) return rc;
+  const unsigned all = (1u << L) - 1u;
+  if (t_dtype == TFX_F64 && f_dtype == TFX_F64 &&
+      mttkrp_path(s, t_dtype, f_dtype, all, prec) == TFX_PATH_F64_MFMA &&
+      !tfx_mttkrp_f64_uses_gemm3(s))
+    return tfx_mttkrp_f64mfma_g(ctx, (const double*)T, s, (const double*)F_flat, all,
+                                (double*)N_flat, (const double*)P1, (double*)grad_flat, stream);
+  rc = tfx_mttkrp_mixed(ctx, T, t_dtype, L, dims, F_flat, f_dtype, R, all, N_flat, prec, stream);
+  if (rc) return rc;
+  return tfx_neg_grad(ctx, f_dtype, L, dims, R, F_flat, P1, N_flat, grad_flat, stream);
+}
+
 extern "C" int tfx_mttkrp(tfx_ctx* ctx, const void* T, int dtype, int L, const int64_t* dims,
                           const void* F_flat, int R, unsigned mode_mask, void* out_flat, int prec,
                           hipStream_t stream) {

@@ -47,6 +47,9 @@ struct M64Plan {
   int ngr[TFX_MAXL];             // slice groups (workgroups) per row block
   int64_t slice[TFX_MAXL];       // c per slice (multiple of 16)
   int64_t stride[TFX_MAXL];      // C-order strides of T
+  const double* F;               // with P1 and grad: the sum also writes grad = N - F_l P1_l
+  const double* P1;
+  double* grad;                  // nullptr: N only
 };
 
 template <int L>
@@ -174,7 +177,25 @@ __global__ __launch_bounds__(256) void k_m64_sum(M64Plan P, const double* __rest
     for (int u = 0; u < 8; ++u) acc += v[u];
   }
   for (; g < ng; ++g) acc += src[(int64_t)g * M64_PART];
-  out[s.off[l] + (int64_t)r * s.d[l] + i] = acc;
+  const int64_t I = s.d[l];
+  const int64_t e_out = s.off[l] + (int64_t)r * I + i;
+  out[e_out] = acc;
+  if (P.grad) {
+    // grad = N - F_l P1_l with tfx_neg_grad's arithmetic: four partial sums combined in a fixed
+    // order, so the fused and the separate forms give the same bits
+    const double* Fl = P.F + s.off[l];
+    const double* P1l = P.P1 + (int64_t)l * s.R * s.R;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int c = 0;
+    for (; c + 4 <= s.R; c += 4) {
+      a0 += Fl[(int64_t)c * I + i] * P1l[(int64_t)c * s.R + r];
+      a1 += Fl[(int64_t)(c + 1) * I + i] * P1l[(int64_t)(c + 1) * s.R + r];
+      a2 += Fl[(int64_t)(c + 2) * I + i] * P1l[(int64_t)(c + 2) * s.R + r];
+      a3 += Fl[(int64_t)(c + 3) * I + i] * P1l[(int64_t)(c + 3) * s.R + r];
+    }
+    for (; c < s.R; ++c) a0 += Fl[(int64_t)c * I + i] * P1l[(int64_t)c * s.R + r];
+    P.grad[e_out] = acc - ((a0 + a1) + (a2 + a3));
+  }
 }
 
 template <int L>
@@ -185,14 +206,28 @@ void launch_m64(const double* T, const M64Plan& P, const double* F, double* part
 
 }  // namespace
 
-int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const double* F,
-                       unsigned mask, double* out, hipStream_t stream) {
+// True when tfx_mttkrp_f64mfma uses the two-pass GEMM form for this shape (it then cannot fuse
+// the gradient).
+int tfx_mttkrp_f64_uses_gemm3(const Shape& s) {
+  return s.L == 3 && s.N >= F64_GEMM_MIN_N && s.d[0] * s.d[1] * 64 <= F64_GEMM_MAX_W;
+}
+
+// grad != nullptr (with P1): every mode must be selected; the slice sum also writes
+// grad = N - F_l P1_l.
+int tfx_mttkrp_f64mfma_g(tfx_ctx* ctx, const double* T, const Shape& s, const double* F,
+                         unsigned mask, double* out, const double* P1, double* grad,
+                         hipStream_t stream) {
   const int L = s.L;
   if (L < 2 || L > TFX_MAXL) return tfx_set_error(ctx, TFX_E_UNSUPPORTED, "MTTKRP order %d", L);
-  if (L == 3 && s.N >= F64_GEMM_MIN_N && s.d[0] * s.d[1] * 64 <= F64_GEMM_MAX_W)
+  if (tfx_mttkrp_f64_uses_gemm3(s)) {
+    if (grad) return tfx_set_error(ctx, TFX_E_ARG, "gradient fusion not available for this shape");
     return tfx_mttkrp_f64_gemm3(ctx, T, s, F, mask, out, stream);
+  }
   M64Plan P;
   P.s = s;
+  P.F = F;
+  P.P1 = P1;
+  P.grad = grad;
   P.stride[L - 1] = 1;
   for (int q = L - 2; q >= 0; --q) P.stride[q] = P.stride[q + 1] * s.d[q + 1];
   P.nsel = 0;
@@ -225,6 +260,7 @@ int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const doub
     nwg += (int)(nib * ng);
   }
   P.first[P.nsel] = nwg;
+  if (grad && P.nsel != L) return tfx_set_error(ctx, TFX_E_ARG, "gradient fusion needs every mode");
   void* wsp;
   int rc = tfx_workspace(ctx, sizeof(double) * (size_t)nwg * M64_PART + 256, stream, &wsp);
   if (rc) return rc;
@@ -248,4 +284,9 @@ int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const doub
     TFX_CHECK_LAUNCH(ctx);
   }
   return TFX_OK;
+}
+
+int tfx_mttkrp_f64mfma(tfx_ctx* ctx, const double* T, const Shape& s, const double* F,
+                       unsigned mask, double* out, hipStream_t stream) {
+  return tfx_mttkrp_f64mfma_g(ctx, T, s, F, mask, out, nullptr, nullptr, stream);
 }

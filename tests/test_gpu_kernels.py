@@ -612,3 +612,36 @@ def test_tensor_stats(dev):
     dev.tensor_stats(dev.upload(X.ravel(), torch.float32), out)
     X32 = X.astype(np.float32).astype(np.float64)
     assert out.cpu().numpy() == pytest.approx([np.abs(X32).sum(), (X32 ** 2).sum()], rel=1e-12)
+
+
+@pytest.mark.parametrize('dims,R,t32', [((50, 50, 50), 50, False), ((12, 9, 14, 7), 6, False),
+                                        ((40, 36, 100), 17, True), ((170, 160, 160), 8, False)])
+def test_mttkrp_neg_grad_equals_separate_calls(dev, dims, R, t32):
+    """tfx_mttkrp_neg_grad (gradient formed in the FP64 matrix-core MTTKRP's final sum, or the two
+    calls for the other paths: split16 on an FP32 tensor, the two-pass form of a large FP64 third-
+    order tensor) gives the same bits as tfx_mttkrp_mixed followed by tfx_neg_grad."""
+    from TensorFox import _lib
+    print('mttkrp+neg_grad', dims, R, t32, flush=True)
+    T, F = _case(dims, R, 31)
+    L = len(dims)
+    prec = _lib.TFX_PREC_SPLIT16 if t32 else _lib.TFX_PREC_NATIVE
+    Td = dev.upload(T.astype(np.float32 if t32 else np.float64).ravel(),
+                    torch.float32 if t32 else torch.float64)
+    Fd = dev.upload(_flat(F), torch.float64)
+    n = R * sum(dims)
+    Gr, P1 = dev.empty((L, R, R), torch.float64), dev.empty((L, R, R), torch.float64)
+    P2, gamma = dev.empty((L, L, R, R), torch.float64), dev.empty((L, R), torch.float64)
+    dev.gramians(dims, R, Fd, Gr, P1, P2, gamma)
+    N1, g1 = dev.zeros(n, torch.float64), dev.zeros(n, torch.float64)
+    dev.mttkrp(Td, dims, Fd, R, N1, prec=prec)
+    dev.neg_grad(dims, R, Fd, P1, N1, g1)
+    N2, g2 = dev.zeros(n, torch.float64), dev.zeros(n, torch.float64)
+    dev.mttkrp_neg_grad(Td, dims, Fd, R, P1, N2, g2, prec=prec)
+    torch.cuda.synchronize()
+    assert torch.equal(N1, N2)
+    assert torch.equal(g1, g2)
+    # and the gradient itself against the oracle's compute_grad restatement (FP64 cases)
+    if not t32:
+        P1ref = o.gramians(F)[1]
+        ref = -o.compute_grad(T, F, P1ref)
+        assert _rel(g2.cpu().numpy(), ref) < 1e-11
