@@ -6,8 +6,10 @@
 // C = F_{L-1}: a (NR x R) (R x K) product, formed 16 x 16 at a time by v_mfma_f64_16x16x4_f64
 // (A = W from registers, B = C from LDS), compared with T in registers and never stored.
 // A workgroup owns 64 columns of the last mode and a run of 16-row tiles; its C chunk is staged
-// once in LDS.  Accumulation of (T - M)^2 is FP64; the per-workgroup partials are summed in a
-// fixed order.
+// once in LDS.  Accumulation of (T - M)^2 is FP64.  The per-workgroup partials are summed by the
+// last workgroup to finish (an arrival counter in the context scratch, reset by that workgroup;
+// no workgroup waits for another), 256 threads over the partials in index order, so the result
+// does not depend on scheduling.
 #include "common.h"
 
 namespace {
@@ -24,11 +26,21 @@ __device__ inline dbl4 mfma64(double a, double b, dbl4 c) {
 
 // NQ: MFMA k steps per tile, a compile-time count (R rounded up to a multiple of 8, ranks past R
 // masked), so the step loop unrolls without branches.
+// Sum of the n partials by one workgroup of NT threads (strided accumulation, then block_sum).
+__device__ inline void sum_partials(const double* part, int n, double* red, double* out) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) acc += __builtin_nontemporal_load(part + i);
+  const double tot = block_sum(acc, red);
+  if (threadIdx.x == 0) *out = tot;
+}
+
 template <typename TS, int LM1, int NQ>
 __global__ __launch_bounds__(NT) void k_residual_mfma(const TS* __restrict__ T, Shape s,
                                                        const double* __restrict__ F,
                                                        int tiles_per_wave,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part,
+                                                       unsigned* __restrict__ counter,
+                                                       double* __restrict__ out) {
   __shared__ double Cs[64 * CP];
   __shared__ double red[NT / 64];
   const int R = s.R;
@@ -112,22 +124,27 @@ __global__ __launch_bounds__(NT) void k_residual_mfma(const TS* __restrict__ T, 
     }
   }
   const double tot = block_sum(acc2, red);
-  if (t == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+  __shared__ int is_last;
+  const unsigned nwg = gridDim.x * gridDim.y;
+  if (t == 0) {
+    part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+    __threadfence();
+    is_last = atomicAdd(counter, 1u) == nwg - 1;
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    sum_partials(part, (int)nwg, red, out);
+    if (t == 0) *counter = 0u;
+  }
 }
 
-__global__ void k_sum_parts_res(const double* __restrict__ part, int n, double* __restrict__ out) {
-  __shared__ double red[4];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
-  const double tot = block_sum(acc, red);
-  if (threadIdx.x == 0) *out = tot;
-}
 
 template <typename TS, int LM1, int NQ>
 void launch_nq(const TS* T, const Shape& s, const double* F, int kchunks, int groups, int tpw,
-               double* part, hipStream_t st) {
+               double* part, unsigned* counter, double* out, hipStream_t st) {
   hipLaunchKernelGGL((k_residual_mfma<TS, LM1, NQ>), dim3((unsigned)kchunks, (unsigned)groups),
-                     dim3(NT), 0, st, T, s, F, tpw, part);
+                     dim3(NT), 0, st, T, s, F, tpw, part, counter, out);
 }
 
 template <typename TS, int LM1>
@@ -151,18 +168,17 @@ int launch(tfx_ctx* ctx, const TS* T, const Shape& s, const double* F, double* o
   double* part = (double*)wsp;
   const int nq8 = ((s.R + 3) / 4 + 1) / 2;   // k steps in pairs: 2 nq8 steps, 8 nq8 >= R
   const int kc = (int)kchunks, gr = (int)groups, tp = (int)tpw;
+  unsigned* counter = (unsigned*)(ctx->dscal + TFX_DSCAL_RES_COUNTER);
   switch (nq8) {
-    case 1: launch_nq<TS, LM1, 2>(T, s, F, kc, gr, tp, part, st); break;
-    case 2: launch_nq<TS, LM1, 4>(T, s, F, kc, gr, tp, part, st); break;
-    case 3: launch_nq<TS, LM1, 6>(T, s, F, kc, gr, tp, part, st); break;
-    case 4: launch_nq<TS, LM1, 8>(T, s, F, kc, gr, tp, part, st); break;
-    case 5: launch_nq<TS, LM1, 10>(T, s, F, kc, gr, tp, part, st); break;
-    case 6: launch_nq<TS, LM1, 12>(T, s, F, kc, gr, tp, part, st); break;
-    case 7: launch_nq<TS, LM1, 14>(T, s, F, kc, gr, tp, part, st); break;
-    default: launch_nq<TS, LM1, 16>(T, s, F, kc, gr, tp, part, st); break;
+    case 1: launch_nq<TS, LM1, 2>(T, s, F, kc, gr, tp, part, counter, out, st); break;
+    case 2: launch_nq<TS, LM1, 4>(T, s, F, kc, gr, tp, part, counter, out, st); break;
+    case 3: launch_nq<TS, LM1, 6>(T, s, F, kc, gr, tp, part, counter, out, st); break;
+    case 4: launch_nq<TS, LM1, 8>(T, s, F, kc, gr, tp, part, counter, out, st); break;
+    case 5: launch_nq<TS, LM1, 10>(T, s, F, kc, gr, tp, part, counter, out, st); break;
+    case 6: launch_nq<TS, LM1, 12>(T, s, F, kc, gr, tp, part, counter, out, st); break;
+    case 7: launch_nq<TS, LM1, 14>(T, s, F, kc, gr, tp, part, counter, out, st); break;
+    default: launch_nq<TS, LM1, 16>(T, s, F, kc, gr, tp, part, counter, out, st); break;
   }
-  TFX_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_sum_parts_res, dim3(1), dim3(256), 0, st, part, (int)nparts, out);
   TFX_CHECK_LAUNCH(ctx);
   return 1;
 }
