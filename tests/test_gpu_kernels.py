@@ -229,6 +229,45 @@ def test_gramians_and_grad(dev, dims, R, dt):
     assert _rel(g.cpu().double().numpy(), ref) < TOL[dt] * 10
 
 
+@pytest.mark.parametrize('dt', [torch.float64, torch.float32])
+@pytest.mark.parametrize('dims,R', [((50, 50, 50), 50), ((12, 9, 14, 7), 6), ((60, 70, 80), 64),
+                                    ((40, 30, 20), 90)])
+def test_gramians_products_exact(dev, dims, R, dt):
+    """P1, P2 and gamma from the device's own Gr, bitwise: each entry is a product in mode order
+    and gamma = max|P1_{L-1}| * sqrt|P1_l[r, r]| (GaussNewton.py:584-606), all correctly rounded
+    IEEE operations, so any reordering of the products (a dropped or transposed mode, a wrong
+    diagonal, the max over another mode) fails the comparison.  Two seeds per shape, with the
+    outputs NaN-filled before each call."""
+    print('gram products', dims, R, dt, flush=True)
+    L = len(dims)
+    for seed in (6, 7):
+        _, F = _case(dims, R, seed)
+        Fd = dev.upload(_flat(F), dt)
+        Gr = dev.empty((L, R, R), dt)
+        P1, P2 = torch.full_like(Gr, float('nan')), dev.empty((L, L, R, R), dt).fill_(float('nan'))
+        gamma = dev.empty((L, R), dt).fill_(float('nan'))
+        dev.gramians(dims, R, Fd, Gr, P1, P2, gamma)
+        g = Gr.cpu().numpy()
+        one = np.ones((R, R), dtype=g.dtype)
+        for l in range(L):
+            p = one
+            for k in range(L):
+                if k != l:
+                    p = p * g[k]
+            assert np.array_equal(P1[l].cpu().numpy(), p)
+            for m in range(L):
+                q = one
+                if m != l:
+                    for k in range(L):
+                        if k != l and k != m:
+                            q = q * g[k]
+                assert np.array_equal(P2[l, m].cpu().numpy(), q)
+        p1 = P1.cpu().numpy()
+        mx = np.abs(p1[L - 1]).max()
+        ref = mx * np.sqrt(np.abs(np.stack([np.diag(p1[l]) for l in range(L)])))
+        assert np.array_equal(gamma.cpu().numpy(), ref.astype(g.dtype))
+
+
 @pytest.mark.parametrize('maxiter', [1, 2, 3, 6, 40])
 def test_cg_matches_reference_vectors(dev, golden, maxiter):
     """CG against the reference's own output (tests/golden), FP64."""
