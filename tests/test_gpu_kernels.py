@@ -355,7 +355,14 @@ def test_cg_large_matches_oracle(dev, dims, R, maxiter, tol):
     ((130, 70, 40), 33, 8, 1e-16, 'f64'),        # R not a multiple of the 16-wide tiles
     ((300, 20, 10, 6), 40, 6, 1e-16, 'f64'),     # order 4, I > one 128-row split
     ((200, 60, 50), 32, 40, 1e-2, 'f64'),        # stops on the tolerance
-    ((400, 80, 60), 40, 6, 1e-16, 'f32')])
+    ((400, 80, 60), 40, 6, 1e-16, 'f32'),
+    # cores with every mode in one 64-row block of the z kernel:
+    ((50, 50, 50), 50, 40, 1e-3, 'f64'),         # the core problem, stops on the tolerance
+    ((64, 64, 40), 40, 12, 1e-16, 'f64'),        # 64-row modes, ragged rank tile
+    ((64, 60), 60, 10, 1e-16, 'f64'),            # order 2 (P2 = ones), four rank tiles
+    ((64, 64, 30, 20), 32, 9, 1e-16, 'f64'),     # order 4
+    ((64, 64, 64), 48, 7, 1e-16, 'f32'),         # FP32
+    ((64, 60, 12), 52, 5, 1e-16, 'f64')])        # R above a mode size
 def test_cg_multi_matches_oracle(dev, dims, R, maxiter, tol, dtn):
     """CG of problems whose operands exceed one workgroup's LDS (the multi-kernel CG) against
     the oracle's CG body from a given right-hand side."""
