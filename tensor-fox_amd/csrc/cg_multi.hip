@@ -156,16 +156,14 @@ __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
 }
 
 // K1: A_part[m][split][c][r] = sum_{i in split} q_m[i][c] F_m[i][r].
-//   raw: q = grad (J^T J grad); else q = M p_j.  For j >= 1 every workgroup first forms the
-//   scalars of iteration j - 1 (stopping the CG when rn_{j-1} <= tol, j >= 2), then updates its
-//   elements: the r-block-0 workgroups store y, r_j and p_j.  Block 0 of the raw pass clears
-//   the stop flag and fills Mv.
+//   raw: q = grad (J^T J grad); else q = M p_j.  For j >= 1 every wave forms the scalars of
+//   iteration j - 1 itself (stopping the CG when rn_{j-1} <= tol, j >= 2) while its staging loads
+//   are in flight, then updates its elements: the r-block-0 workgroups store y, r_j and p_j.
+//   Block 0 of the raw pass clears the stop flag and fills Mv.
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, int j) {
   __shared__ double qs[KI][CT + 1];
   __shared__ double fs[KI][CT + 1];
-  __shared__ double slots[D_N];
-  __shared__ int stop_s;
   const Shape& s = P.s;
   if (raw && blockIdx.x == 0 && threadIdx.x == 0) {
     B.sc[SC_DONE] = 0.0;
@@ -173,26 +171,16 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   }
   if (raw && blockIdx.x == 0)
     for (int e = threadIdx.x; e < s.L * s.R; e += NTH) B.Mv[e] = inv_m(B.gamma[e], P.damp);
-  if (!raw) {
-    // workgroups of this kernel may set the flag: read it once for the whole workgroup
-    if (threadIdx.x == 0) stop_s = B.sc[SC_DONE] != 0.0;
-    __syncthreads();
-    if (stop_s) return;
+  const bool scal = !raw && j >= 1;
+  const int lane = threadIdx.x & 63;
+  // K2(j - 1)'s dot-product partials (lanes < nk2; the rest after the staging loads)
+  double dq[D_N] = {0.0, 0.0, 0.0, 0.0};
+  if (scal && lane < P.nk2) {
+#pragma unroll
+    for (int q = 0; q < D_N; ++q) dq[q] = B.dots[(j - 1) & 1][q * P.nk2 + lane];
   }
   double alpha = 0.0, beta = 0.0;
-  if (!raw && j >= 1) {
-    const IterScalars v = iter_scalars(P, B, j - 1, slots);
-    if (j >= 2 && v.rn <= P.tol) {
-      if (threadIdx.x == 0) {
-        B.sc[SC_DONE] = 1.0;
-        B.sc[SC_ITN] = (double)(j - 2);
-      }
-      return;
-    }
-    alpha = v.alpha;
-    beta = v.rn_next / v.rn;
-  }
-  const S aS = (S)alpha;
+  S aS = S(0);
   int m = 0;
   while (m + 1 < s.L && (int)blockIdx.x >= P.k1first[m + 1]) ++m;
   const int nrb = P.nrb;
@@ -220,6 +208,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
     // first (clamped indices), values after
     constexpr int NK1 = KI * CT / NTH;
     S v1[NK1], vo[NK1], vz[NK1], vy[NK1], vf[NK1];
+    double vm[NK1];
 #pragma unroll
     for (int k = 0; k < NK1; ++k) {
       const int u = t + NTH * k;
@@ -232,6 +221,37 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
       vz[k] = B.z[e];
       vy[k] = B.yc[e];
       vf[k] = Fm[(i < ie && rr < R) ? (int64_t)rr * I + i : 0];
+      vm[k] = raw ? 0.0 : B.Mv[m * R + (c < R ? c : 0)];
+    }
+    if (i0 == ib && !raw) {
+      // the stop flag and the scalars, every wave alike: workgroups of this kernel may set the
+      // flag, but only when every wave of every workgroup also finds rn_{j-1} <= tol (sums of the
+      // same partials in iter_scalars' order)
+      if (B.sc[SC_DONE] != 0.0) return;
+      if (scal) {
+        double tq[D_N];
+#pragma unroll
+        for (int q = 0; q < D_N; ++q) {
+          double v = 0.0 + dq[q];
+          for (int i = lane + 64; i < P.nk2; i += 64) v += B.dots[(j - 1) & 1][q * P.nk2 + i];
+          tq[q] = wave_sum(v);
+        }
+        double rn = tq[D_RR];
+        if (j - 1 == 0 && rn == 0.0) rn = 1e-6;
+        double pz = tq[D_PZ];
+        if (pz == 0.0) pz = 1e-6;
+        alpha = rn / pz;
+        const double rn_next = rn - 2.0 * alpha * tq[D_RZ] + alpha * alpha * tq[D_ZZ];
+        if (j >= 2 && rn <= P.tol) {
+          if (threadIdx.x == 0) {
+            B.sc[SC_DONE] = 1.0;
+            B.sc[SC_ITN] = (double)(j - 2);
+          }
+          return;
+        }
+        beta = rn_next / rn;
+        aS = (S)alpha;
+      }
     }
 #pragma unroll
     for (int k = 0; k < NK1; ++k) {
@@ -258,7 +278,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
               pnew[e] = pv;
             }
           }
-          qv = B.Mv[m * R + c] * (double)pv;
+          qv = vm[k] * (double)pv;
         }
       }
       qs[ii][cc] = qv;
@@ -297,7 +317,8 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   // wave w owns rows i0 + 16 w .. + 15 of the 64 x 16 output block, as one FP64 MFMA tile:
   // register j of lane (li, lg) holds row 16 w + 4 j + lg, column r0 + li
   const int wv = t >> 6, li = t & 15, lg = (t >> 4) & 3;
-  // the epilogue's operands, loaded with the stop flag (before the staging loads' results)
+  // the epilogue's operands and the first chunk's staging operands are loaded before the stop
+  // flag is read (K1 of this iteration sets it, so every wave reads the same value)
   double eg = 0.0, eM = 0.0, ep[4], er[4];
   {
     const int rq = r0 + li < R ? r0 + li : 0;
@@ -312,7 +333,6 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       er[a] = (double)rjp[e];
     }
   }
-  if (!raw && B.sc[SC_DONE] != 0.0) return;
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   const S* qsrc = raw ? B.grad : pj;
   for (int cc0 = 0; cc0 < R; cc0 += ZC) {
@@ -331,9 +351,11 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       fv[k] = B.F[e];
       qv[k] = qsrc[e];
     }
-    // tmp_l[c][r] = sum_{m != l} P2[l][m][c][r] sum_split A_m[split][c][r], and P1_l
+    // tmp_l[c][r] = sum_{m != l} P2[l][m][c][r] sum_split A_m[split][c][r], and P1_l.
+    // Modes 0..3: split 0 of A_m and P2[l][m] of the modes m != l, loaded with F and q;
+    // further splits and modes >= 4 follow in the same summation order
     constexpr int NT = ZC * CT / NTH;
-    double tv[NT], pv[NT];
+    double tv[NT], pv[NT], a0[NT][4], p2[NT][4];
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
       const int u = t + NTH * k;
@@ -341,22 +363,29 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       const int c = cc0 + cc, rr = r0 + rq;
       const bool ok = c < R && rr < R;
       const int64_t cr = ok ? (int64_t)c * R + rr : 0;
-      // modes 0..3: split 0 of A_m and P2[l][m] loaded together (mode 0 stands in for the
-      // inactive ones); further splits and modes >= 4 follow in the same summation order
-      double a0[4], p2[4];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        const int mc = (m < L && m != l) ? m : 0;
-        a0[m] = B.A[P.aoff[mc] + cr];
-        p2[m] = (double)B.P2[((int64_t)l * L + mc) * RR + cr];
+        const bool act = m < L && m != l;   // uniform in the workgroup
+        a0[k][m] = act ? B.A[P.aoff[m] + cr] : 0.0;
+        p2[k][m] = act ? (double)B.P2[((int64_t)l * L + m) * RR + cr] : 0.0;
       }
+      pv[k] = ok ? (double)B.P1[(int64_t)l * RR + cr] : 0.0;
+    }
+    if (cc0 == 0 && !raw && B.sc[SC_DONE] != 0.0) return;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const int u = t + NTH * k;
+      const int cc = u / CT, rq = u % CT;
+      const int c = cc0 + cc, rr = r0 + rq;
+      const bool ok = c < R && rr < R;
+      const int64_t cr = ok ? (int64_t)c * R + rr : 0;
       double acc2 = 0.0;
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         if (m >= L || m == l) continue;   // uniform in the workgroup
-        double a = a0[m];
+        double a = a0[k][m];
         for (int sp = 1; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + (int64_t)sp * RR + cr];
-        acc2 += p2[m] * a;
+        acc2 += p2[k][m] * a;
       }
       for (int m = 4; m < L; ++m) {
         if (m == l) continue;
@@ -365,7 +394,6 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
         acc2 += (double)B.P2[((int64_t)l * L + m) * RR + cr] * a;
       }
       tv[k] = ok ? acc2 : 0.0;
-      pv[k] = ok ? (double)B.P1[(int64_t)l * RR + cr] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
