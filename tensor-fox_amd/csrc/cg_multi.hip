@@ -341,6 +341,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
     // F_l and q_l rows i0..i0+63, columns cc0..cc0+63 (coalesced along i)
     constexpr int NFQ = ZI * ZC / NTH;
     S fv[NFQ], qv[NFQ];
+    double mq[NFQ];   // M of column c (wave-uniform: c = cc0 + w + 4 k)
 #pragma unroll
     for (int k = 0; k < NFQ; ++k) {
       const int u = t + NTH * k;
@@ -350,6 +351,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       const int64_t e = (i < I && c < R) ? off + (int64_t)c * I + i : 0;
       fv[k] = B.F[e];
       qv[k] = qsrc[e];
+      mq[k] = raw ? 0.0 : B.Mv[l * R + (c < R ? c : 0)];
     }
     // tmp_l[c][r] = sum_{m != l} P2[l][m][c][r] sum_split A_m[split][c][r], and P1_l.
     // Modes 0..3: split 0 of A_m and P2[l][m] of the modes m != l, loaded with F and q;
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       const int c = cc0 + cc;
       const bool ok = i < I && c < R;
       Fs[ii][cc] = ok ? (double)fv[k] : 0.0;
-      Qs[ii][cc] = ok ? (raw ? (double)qv[k] : B.Mv[l * R + c] * (double)qv[k]) : 0.0;
+      Qs[ii][cc] = ok ? (raw ? (double)qv[k] : mq[k] * (double)qv[k]) : 0.0;
     }
     __syncthreads();
     // z[i][r] += sum_c F[i][c] tmp[c][r] + q[i][c] P1[c][r] (columns past R are zeros)
