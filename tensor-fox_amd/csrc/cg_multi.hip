@@ -201,8 +201,10 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   const bool writer = upd && rbk == 0;
   const S* src1 = raw ? B.grad : (j == 0 ? B.p[0] : B.r[(j + 1) & 1]);
   const int t = threadIdx.x;
-  const int tc = t >> 4, tr = t & 15;
-  double acc = 0.0;
+  // wave 0 forms the 16 x 16 block on FP64 MFMA: lane (lg, li) supplies q[4 k + lg][c0 + li] and
+  // F[4 k + lg][r0 + li] at step k; register jj holds c = c0 + 4 jj + lg, r = r0 + li
+  const int li = t & 15, lg = (t >> 4) & 3;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   for (int64_t i0 = ib; i0 < ie; i0 += KI) {
     // stage q[i0..i0+31][c0..c0+15] and F[i0..i0+31][r0..r0+15] (coalesced along i); loads
     // first (clamped indices), values after
@@ -285,12 +287,21 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
       fs[ii][cc] = (i < ie && rr < R) ? (double)vf[k] : 0.0;
     }
     __syncthreads();
-#pragma unroll 8
-    for (int ii = 0; ii < KI; ++ii) acc += qs[ii][tc] * fs[ii][tr];
+    if (t < 64) {
+#pragma unroll
+      for (int k = 0; k < KI / 4; ++k)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qs[4 * k + lg][li], fs[4 * k + lg][li], acc, 0, 0, 0);
+    }
     __syncthreads();
   }
-  const int c = c0 + tc, rr = r0 + tr;
-  if (c < R && rr < R) B.A[P.aoff[m] + ((int64_t)split * R + c) * R + rr] = acc;
+  if (t < 64) {
+    const int rr = r0 + li;
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int c = c0 + 4 * jj + lg;
+      if (c < R && rr < R) B.A[P.aoff[m] + ((int64_t)split * R + c) * R + rr] = acc[jj];
+    }
+  }
 }
 
 // K2: z_l[i][r] for rows i0..i0+63 and ranks r0..r0+15 of mode l; c in chunks of ZC.
