@@ -146,8 +146,15 @@ int stop_code(int it, const double* errors, const double* steps, const double* i
   return -1;
 }
 
+// Gram/Hadamard products and gradient of one point; two sets, so that the next iteration's can
+// be formed while the current iteration's are still in use
+struct GradSet {
+  double *Gr, *P1, *P2, *gamma, *N, *grad;
+};
+
 struct Bufs {
-  double *Gr, *P1, *P2, *gamma, *N, *grad, *y, *jtjg, *xa, *xb, *F, *best;
+  GradSet gs[2];
+  double *y, *jtjg, *xa, *xb, *F, *best;
   double *dx[2], *dy[2], *dbx, *dby, *dbF, *d, *sx, *sF, *scal, *tscal;
 };
 
@@ -164,11 +171,15 @@ int64_t carve(int L, const int64_t* dims, int R, double* base, Bufs* b) {
   };
   int64_t used = 0;
   auto add = [&](int64_t cnt) { used += (cnt + 31) & ~int64_t(31); };
-  b->Gr = take(L * rr), add(L * rr);
-  b->P1 = take(L * rr), add(L * rr);
-  b->P2 = take((int64_t)L * L * rr), add((int64_t)L * L * rr);
-  b->gamma = take((int64_t)L * R), add((int64_t)L * R);
-  double** vecs[] = {&b->N, &b->grad, &b->y, &b->jtjg, &b->xa, &b->xb, &b->F, &b->best, &b->dx[0], &b->dx[1],
+  for (GradSet& g : b->gs) {
+    g.Gr = take(L * rr), add(L * rr);
+    g.P1 = take(L * rr), add(L * rr);
+    g.P2 = take((int64_t)L * L * rr), add((int64_t)L * L * rr);
+    g.gamma = take((int64_t)L * R), add((int64_t)L * R);
+    g.N = take(nal), add(nal);
+    g.grad = take(nal), add(nal);
+  }
+  double** vecs[] = {&b->y, &b->jtjg, &b->xa, &b->xb, &b->F, &b->best, &b->dx[0], &b->dx[1],
                      &b->dy[0], &b->dy[1], &b->dbx, &b->dby, &b->dbF, &b->d, &b->sx, &b->sF};
   for (double** v : vecs) *v = take(nal), add(nal);
   b->scal = take(16), add(16);
@@ -213,10 +224,17 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
   int symm_now = o->symm ? (o->factors_norm > 0 ? 1 : (o->aliased ? 2 : 1)) : 0;
   double* pin = nullptr;
   if (hipHostMalloc((void**)&pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) return TFX_E_NOMEM;
+  hipEvent_t ev_scal = nullptr;
+  if (hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming) != hipSuccess) {
+    hipHostFree(pin);
+    return TFX_E_HIP;
+  }
   Mt mt{mt_key, *mt_pos};
   int rc = TFX_OK;
   double error = 1.0, best_error = INFINITY, damp = o->damp0;
   int stop = 5, it = 0, dogleg_calls = 0;
+  int gcur = 0;          // the gradient set of the current iteration
+  bool ahead = false;    // its Gram products and gradient were formed during the previous one
   double* x = b.xa;
   double* xs = b.xb;
 #define RUN(call)            \
@@ -243,16 +261,32 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
     } else {
       cap = o->cg_maxiter;
     }
-    RUN(tfx_gramians(ctx, TFX_F64, L, dims, R, b.F, b.Gr, b.P1, b.P2, b.gamma, st));
-    RUN(tfx_mttkrp_neg_grad(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, b.P1, b.N, b.grad, prec, st));
-    RUN(tfx_cg(ctx, TFX_F64, L, dims, R, b.F, b.P1, b.P2, b.gamma, damp, b.grad, cap, o->cg_tol, b.y, b.jtjg,
+    const GradSet& g = b.gs[gcur];
+    if (!ahead) {
+      RUN(tfx_gramians(ctx, TFX_F64, L, dims, R, b.F, g.Gr, g.P1, g.P2, g.gamma, st));
+      RUN(tfx_mttkrp_neg_grad(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, g.P1, g.N, g.grad, prec, st));
+    }
+    ahead = false;
+    RUN(tfx_cg(ctx, TFX_F64, L, dims, R, b.F, g.P1, g.P2, g.gamma, damp, g.grad, cap, o->cg_tol, b.y, b.jtjg,
                b.scal, st));
-    RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, x, b.y, 1.0, b.grad, damp, 1, symm_now, o->factors_norm, xs, b.F,
+    RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, x, b.y, 1.0, g.grad, damp, 1, symm_now, o->factors_norm, xs, b.F,
                        b.scal + 2, st));
     symm_now = symm1;
     RUN(tfx_cpd_residual_mixed(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, b.scal + 6, st));
     HIPRUN(hipMemcpyAsync(pin, b.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPRUN(hipStreamSynchronize(st));
+    HIPRUN(hipEventRecord(ev_scal, st));
+    // The next iteration's Gram products and gradient depend only on the updated factors: they
+    // go into the other set now, so the device works while the host reads this iteration's
+    // scalars.  A dogleg step changes the factors (they are formed again after it); after a stop
+    // they are left unused.
+    const GradSet& gn = b.gs[1 - gcur];
+    if (it + 1 < maxiter) {
+      RUN(tfx_gramians(ctx, TFX_F64, L, dims, R, b.F, gn.Gr, gn.P1, gn.P2, gn.gamma, st));
+      RUN(tfx_mttkrp_neg_grad(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, gn.P1, gn.N, gn.grad, prec, st));
+      ahead = true;
+    }
+    gcur = 1 - gcur;
+    HIPRUN(hipEventSynchronize(ev_scal));   // the scalars only, not the work queued after them
     double* old_x = x;
     std::swap(x, xs);
     error = sqrt(pin[6]) / Tsize;
@@ -271,14 +305,14 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
       int count = 0;
       while (gain > 0) {
         const double dold = derr;
-        RUN(tfx_dogleg_dots(ctx, TFX_F64, n, b.dy[cur], b.grad, b.jtjg, b.tscal, st));
+        RUN(tfx_dogleg_dots(ctx, TFX_F64, n, b.dy[cur], g.grad, b.jtjg, b.tscal, st));
         HIPRUN(hipMemcpyAsync(pin + 16, b.tscal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPRUN(hipStreamSynchronize(st));
         double ca, cb;
         dogleg_coeffs(pin + 16, delta, &ca, &cb);
         const int nx = 1 - cur;
-        RUN(tfx_axpby(ctx, TFX_F64, n, ca, b.dy[cur], cb, b.grad, b.dy[nx], st));
-        RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, b.dx[cur], b.dy[nx], 1.0, b.grad, damp, 0, symm1,
+        RUN(tfx_axpby(ctx, TFX_F64, n, ca, b.dy[cur], cb, g.grad, b.dy[nx], st));
+        RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, b.dx[cur], b.dy[nx], 1.0, g.grad, damp, 0, symm1,
                            o->factors_norm, b.dx[nx], b.F, b.tscal + 4, st));
         RUN(tfx_cpd_residual_mixed(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, b.tscal + 8, st));
         HIPRUN(hipMemcpyAsync(pin + 24, b.tscal + 4, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -301,12 +335,13 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
       HIPRUN(hipMemcpyAsync(b.y, b.dby, vb, hipMemcpyDeviceToDevice, st));
       // x - old_x statistics for this iteration's gain ratio and step size
       RUN(tfx_axpby(ctx, TFX_F64, n, 1.0, x, -1.0, old_x, b.d, st));
-      RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, old_x, b.d, 1.0, b.grad, damp, 0, 0, 0.0, b.sx, b.sF,
+      RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, old_x, b.d, 1.0, g.grad, damp, 0, 0, 0.0, b.sx, b.sF,
                          b.tscal, st));
       HIPRUN(hipMemcpyAsync(pin + 32, b.tscal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
       HIPRUN(hipStreamSynchronize(st));
       gain_den = pin[32];
       dx2 = pin[33];
+      ahead = false;   // the factors moved: the next iteration forms its products again
     }
     const double gr = gain_ratio(old_error, error, Tsize, gain_den);
     damp = update_damp(damp, damp_sched, gr, it);
@@ -336,5 +371,6 @@ done:
 #undef HIPRUN
   *mt_pos = mt.pos;
   hipHostFree(pin);
+  hipEventDestroy(ev_scal);
   return rc;
 }
