@@ -23,6 +23,25 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// EL consecutive elements of type S loaded or stored as one vector (EL = 2: 16-byte accesses,
+// which need an even element offset)
+template <typename S, int EL>
+struct Lanes {
+  typedef S T __attribute__((ext_vector_type(EL)));
+};
+template <typename S>
+struct Lanes<S, 1> {
+  typedef S T;
+};
+template <typename S, int EL>
+__device__ inline S lane_at(const typename Lanes<S, EL>::T& v, int h) {
+  if constexpr (EL == 1) {
+    return v;
+  } else {
+    return v[h];
+  }
+}
+
 constexpr int CT = 16;        // c / r tile of the R x R products
 constexpr int ICH = 128;      // i per workgroup in K1 (split of the reduction over I_m)
 constexpr int KI = 64;        // i staged per step in K1 (a 50-row mode in one step)
@@ -160,7 +179,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
 //   iteration j - 1 itself (stopping the CG when rn_{j-1} <= tol, j >= 2) while its staging loads
 //   are in flight, then updates its elements: the r-block-0 workgroups store y, r_j and p_j.
 //   Block 0 of the raw pass clears the stop flag and fills Mv.
-template <typename S>
+template <typename S, int EL>
 __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, int j) {
   __shared__ double qs[KI][CT + 1];
   __shared__ double fs[KI][CT + 1];
@@ -205,24 +224,27 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   // F[4 k + lg][r0 + li] at step k; register jj holds c = c0 + 4 jj + lg, r = r0 + li
   const int li = t & 15, lg = (t >> 4) & 3;
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  // EL consecutive rows per load (EL = 2 when every mode size is even, so that element offsets
+  // are even and a row range [i, i + 2) is either inside [ib, ie) or outside it)
+  typedef typename Lanes<S, EL>::T SV;
   for (int64_t i0 = ib; i0 < ie; i0 += KI) {
-    // stage q[i0..i0+31][c0..c0+15] and F[i0..i0+31][r0..r0+15] (coalesced along i); loads
+    // stage q[i0..i0+63][c0..c0+15] and F[i0..i0+63][r0..r0+15] (coalesced along i); loads
     // first (clamped indices), values after
-    constexpr int NK1 = KI * CT / NTH;
-    S v1[NK1], vo[NK1], vz[NK1], vy[NK1], vf[NK1];
+    constexpr int NK1 = KI * CT / NTH / EL;
+    SV v1[NK1], vo[NK1], vz[NK1], vy[NK1], vf[NK1];
     double vm[NK1];
 #pragma unroll
     for (int k = 0; k < NK1; ++k) {
       const int u = t + NTH * k;
-      const int cc = u / KI, ii = u % KI;
+      const int cc = u / (KI / EL), ii = u % (KI / EL) * EL;
       const int64_t i = i0 + ii;
       const int c = c0 + cc, rr = r0 + cc;
       const int64_t e = (i < ie && c < R) ? off + (int64_t)c * I + i : 0;
-      v1[k] = src1[e];
-      vo[k] = pold[e];
-      vz[k] = B.z[e];
-      vy[k] = B.yc[e];
-      vf[k] = Fm[(i < ie && rr < R) ? (int64_t)rr * I + i : 0];
+      v1[k] = *(const SV*)(src1 + e);
+      vo[k] = *(const SV*)(pold + e);
+      vz[k] = *(const SV*)(B.z + e);
+      vy[k] = *(const SV*)(B.yc + e);
+      vf[k] = *(const SV*)(Fm + ((i < ie && rr < R) ? (int64_t)rr * I + i : 0));
       vm[k] = raw ? 0.0 : B.Mv[m * R + (c < R ? c : 0)];
     }
     if (i0 == ib && !raw) {
@@ -258,33 +280,44 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
 #pragma unroll
     for (int k = 0; k < NK1; ++k) {
       const int u = t + NTH * k;
-      const int cc = u / KI, ii = u % KI;
+      const int cc = u / (KI / EL), ii = u % (KI / EL) * EL;
       const int64_t i = i0 + ii;
       const int c = c0 + cc, rr = r0 + cc;
-      const bool okq = i < ie && c < R;
-      double qv = 0.0;
-      if (okq) {
-        if (raw) {
-          qv = (double)v1[k];
-        } else {
-          S pv;
-          if (j == 0) {
-            pv = v1[k];
+      const bool okq = i < ie && c < R;   // the same for all EL rows of the slot
+      SV ry, rr_, rp;
+#pragma unroll
+      for (int h = 0; h < EL; ++h) {
+        double qv = 0.0;
+        if (okq) {
+          const S a1 = lane_at<S, EL>(v1[k], h), ao = lane_at<S, EL>(vo[k], h);
+          if (raw) {
+            qv = (double)a1;
           } else {
-            const S rv = v1[k] - aS * vz[k];
-            pv = (S)((double)rv + beta * (double)(j == 1 ? rv : vo[k]));
-            if (writer) {
-              const int64_t e = off + (int64_t)c * I + i;
-              B.yc[e] = vy[k] + aS * vo[k];
-              rnew[e] = rv;
-              pnew[e] = pv;
+            S pv;
+            if (j == 0) {
+              pv = a1;
+            } else {
+              const S rv = a1 - aS * lane_at<S, EL>(vz[k], h);
+              pv = (S)((double)rv + beta * (double)(j == 1 ? rv : ao));
+              const S yv = lane_at<S, EL>(vy[k], h) + aS * ao;
+              if constexpr (EL == 1) {
+                ry = yv, rr_ = rv, rp = pv;
+              } else {
+                ry[h] = yv, rr_[h] = rv, rp[h] = pv;
+              }
             }
+            qv = vm[k] * (double)pv;
           }
-          qv = vm[k] * (double)pv;
         }
+        qs[ii + h][cc] = qv;
+        fs[ii + h][cc] = (i < ie && rr < R) ? (double)lane_at<S, EL>(vf[k], h) : 0.0;
       }
-      qs[ii][cc] = qv;
-      fs[ii][cc] = (i < ie && rr < R) ? (double)vf[k] : 0.0;
+      if (okq && writer) {
+        const int64_t e = off + (int64_t)c * I + i;
+        *(SV*)(B.yc + e) = ry;
+        *(SV*)(rnew + e) = rr_;
+        *(SV*)(pnew + e) = rp;
+      }
     }
     __syncthreads();
     if (t < 64) {
@@ -305,7 +338,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
 }
 
 // K2: z_l[i][r] for rows i0..i0+63 and ranks r0..r0+15 of mode l; c in chunks of ZC.
-template <typename S>
+template <typename S, int EL>
 __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, int j, S* out) {
   __shared__ double Fs[ZI][ZC + 1];
   __shared__ double Qs[ZI][ZC + 1];
@@ -350,18 +383,20 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
     // Loads first, values after: every loop below has a compile-time trip count and
     // clamped indices, so all of a thread's loads are in flight together.
     // F_l and q_l rows i0..i0+63, columns cc0..cc0+63 (coalesced along i)
-    constexpr int NFQ = ZI * ZC / NTH;
-    S fv[NFQ], qv[NFQ];
-    double mq[NFQ];   // M of column c (wave-uniform: c = cc0 + w + 4 k)
+    // (EL consecutive rows per load, as in K1)
+    typedef typename Lanes<S, EL>::T SV;
+    constexpr int NFQ = ZI * ZC / NTH / EL;
+    SV fv[NFQ], qv[NFQ];
+    double mq[NFQ];   // M of column c
 #pragma unroll
     for (int k = 0; k < NFQ; ++k) {
       const int u = t + NTH * k;
-      const int cc = u / ZI, ii = u % ZI;
+      const int cc = u / (ZI / EL), ii = u % (ZI / EL) * EL;
       const int64_t i = i0 + ii;
       const int c = cc0 + cc;
       const int64_t e = (i < I && c < R) ? off + (int64_t)c * I + i : 0;
-      fv[k] = B.F[e];
-      qv[k] = qsrc[e];
+      fv[k] = *(const SV*)(B.F + e);
+      qv[k] = *(const SV*)(qsrc + e);
       mq[k] = raw ? 0.0 : B.Mv[l * R + (c < R ? c : 0)];
     }
     // tmp_l[c][r] = sum_{m != l} P2[l][m][c][r] sum_split A_m[split][c][r], and P1_l.
@@ -417,12 +452,16 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
 #pragma unroll
     for (int k = 0; k < NFQ; ++k) {
       const int u = t + NTH * k;
-      const int cc = u / ZI, ii = u % ZI;
+      const int cc = u / (ZI / EL), ii = u % (ZI / EL) * EL;
       const int64_t i = i0 + ii;
       const int c = cc0 + cc;
       const bool ok = i < I && c < R;
-      Fs[ii][cc] = ok ? (double)fv[k] : 0.0;
-      Qs[ii][cc] = ok ? (raw ? (double)qv[k] : mq[k] * (double)qv[k]) : 0.0;
+#pragma unroll
+      for (int h = 0; h < EL; ++h) {
+        const double fh = (double)lane_at<S, EL>(fv[k], h), qh = (double)lane_at<S, EL>(qv[k], h);
+        Fs[ii + h][cc] = ok ? fh : 0.0;
+        Qs[ii + h][cc] = ok ? (raw ? qh : mq[k] * qh) : 0.0;
+      }
     }
     __syncthreads();
     // z[i][r] += sum_c F[i][c] tmp[c][r] + q[i][c] P1[c][r] (columns past R are zeros)
@@ -590,17 +629,22 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   B.rq0 = (double*)w; w += align256(sizeof(double) * (size_t)nbv);
   B.sc = (double*)w;
 
+  // paired-row loads when every mode size is even (FP64)
+  bool even = sizeof(S) == 8;
+  for (int m = 0; m < s.L; ++m) even = even && s.d[m] % 2 == 0;
+  const auto kA = even ? k_cgm_A<S, 2> : k_cgm_A<S, 1>;
+  const auto kZ = even ? k_cgm_Z<S, 2> : k_cgm_Z<S, 1>;
   // J^T J grad for the dogleg (also clears the stop flag and fills Mv)
-  hipLaunchKernelGGL(k_cgm_A<S>, dim3(k1), dim3(NTH), 0, st, P, B, 1, 0);
+  hipLaunchKernelGGL(kA, dim3(k1), dim3(NTH), 0, st, P, B, 1, 0);
   TFX_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_cgm_Z<S>, dim3(k2), dim3(NTH), 0, st, P, B, 1, 0, jtjg);
+  hipLaunchKernelGGL(kZ, dim3(k2), dim3(NTH), 0, st, P, B, 1, 0, jtjg);
   TFX_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_cgm_start<S>, dim3(nbv), dim3(NTH), 0, st, P, B);
   TFX_CHECK_LAUNCH(ctx);
   for (int jj = 0; jj < maxiter; ++jj) {
-    hipLaunchKernelGGL(k_cgm_A<S>, dim3(k1), dim3(NTH), 0, st, P, B, 0, jj);
+    hipLaunchKernelGGL(kA, dim3(k1), dim3(NTH), 0, st, P, B, 0, jj);
     TFX_CHECK_LAUNCH(ctx);
-    hipLaunchKernelGGL(k_cgm_Z<S>, dim3(k2), dim3(NTH), 0, st, P, B, 0, jj, B.z);
+    hipLaunchKernelGGL(kZ, dim3(k2), dim3(NTH), 0, st, P, B, 0, jj, B.z);
     TFX_CHECK_LAUNCH(ctx);
   }
   hipLaunchKernelGGL(k_cgm_end<S>, dim3(nbv), dim3(NTH), 0, st, P, B, maxiter, y, scal);

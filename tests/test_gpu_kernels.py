@@ -362,7 +362,8 @@ def test_cg_large_matches_oracle(dev, dims, R, maxiter, tol):
     ((64, 60), 60, 10, 1e-16, 'f64'),            # order 2 (P2 = ones), four rank tiles
     ((64, 64, 30, 20), 32, 9, 1e-16, 'f64'),     # order 4
     ((64, 64, 64), 48, 7, 1e-16, 'f32'),         # FP32
-    ((64, 60, 12), 52, 5, 1e-16, 'f64')])        # R above a mode size
+    ((64, 60, 12), 52, 5, 1e-16, 'f64'),         # R above a mode size
+    ((51, 49, 47), 50, 8, 1e-16, 'f64')])        # odd mode sizes (one-row loads)
 def test_cg_multi_matches_oracle(dev, dims, R, maxiter, tol, dtn):
     """CG of problems whose operands exceed one workgroup's LDS (the multi-kernel CG) against
     the oracle's CG body from a given right-hand side."""
