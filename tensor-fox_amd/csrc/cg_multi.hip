@@ -403,51 +403,57 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
     // Modes 0..3: split 0 of A_m and P2[l][m] of the modes m != l, loaded with F and q;
     // further splits and modes >= 4 follow in the same summation order
     constexpr int NT = ZC * CT / NTH;
-    double tv[NT], pv[NT], a0[NT][4], p2[NT][4];
+    // (EL consecutive ranks r per load: R is even when EL = 2)
+    typedef typename Lanes<double, EL>::T DV;
+    constexpr int NTS = NT / EL;
+    DV a0[NTS][4];
+    SV p2[NTS][4], pv[NTS];
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
+    for (int k = 0; k < NTS; ++k) {
       const int u = t + NTH * k;
-      const int cc = u / CT, rq = u % CT;
+      const int cc = u / (CT / EL), rq = u % (CT / EL) * EL;
       const int c = cc0 + cc, rr = r0 + rq;
-      const bool ok = c < R && rr < R;
+      const bool ok = c < R && rr < R;   // the same for the EL ranks of the slot
       const int64_t cr = ok ? (int64_t)c * R + rr : 0;
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        const bool act = m < L && m != l;   // uniform in the workgroup
-        a0[k][m] = act ? B.A[P.aoff[m] + cr] : 0.0;
-        p2[k][m] = act ? (double)B.P2[((int64_t)l * L + m) * RR + cr] : 0.0;
+        if (m < L && m != l) {   // uniform in the workgroup
+          a0[k][m] = *(const DV*)(B.A + P.aoff[m] + cr);
+          p2[k][m] = *(const SV*)(B.P2 + ((int64_t)l * L + m) * RR + cr);
+        } else {
+          a0[k][m] = DV(0.0);
+          p2[k][m] = SV(0);
+        }
       }
-      pv[k] = ok ? (double)B.P1[(int64_t)l * RR + cr] : 0.0;
+      pv[k] = *(const SV*)(B.P1 + (int64_t)l * RR + cr);
     }
     if (cc0 == 0 && !raw && B.sc[SC_DONE] != 0.0) return;
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
+    for (int k = 0; k < NTS; ++k) {
       const int u = t + NTH * k;
-      const int cc = u / CT, rq = u % CT;
+      const int cc = u / (CT / EL), rq = u % (CT / EL) * EL;
       const int c = cc0 + cc, rr = r0 + rq;
       const bool ok = c < R && rr < R;
-      const int64_t cr = ok ? (int64_t)c * R + rr : 0;
-      double acc2 = 0.0;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        if (m >= L || m == l) continue;   // uniform in the workgroup
-        double a = a0[k][m];
-        for (int sp = 1; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + (int64_t)sp * RR + cr];
-        acc2 += p2[k][m] * a;
-      }
-      for (int m = 4; m < L; ++m) {
-        if (m == l) continue;
-        double a = 0.0;
-        for (int sp = 0; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + (int64_t)sp * RR + cr];
-        acc2 += (double)B.P2[((int64_t)l * L + m) * RR + cr] * a;
-      }
-      tv[k] = ok ? acc2 : 0.0;
-    }
+      for (int h = 0; h < EL; ++h) {
+        const int64_t cr = ok ? (int64_t)c * R + rr + h : 0;
+        double acc2 = 0.0;
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
-      const int u = t + NTH * k;
-      Ts[u / CT][u % CT] = tv[k];
-      P1s[u / CT][u % CT] = pv[k];
+        for (int m = 0; m < 4; ++m) {
+          if (m >= L || m == l) continue;   // uniform in the workgroup
+          double a = lane_at<double, EL>(a0[k][m], h);
+          for (int sp = 1; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + (int64_t)sp * RR + cr];
+          acc2 += (double)lane_at<S, EL>(p2[k][m], h) * a;
+        }
+        for (int m = 4; m < L; ++m) {
+          if (m == l) continue;
+          double a = 0.0;
+          for (int sp = 0; sp < P.nsplit[m]; ++sp) a += B.A[P.aoff[m] + (int64_t)sp * RR + cr];
+          acc2 += (double)B.P2[((int64_t)l * L + m) * RR + cr] * a;
+        }
+        Ts[cc][rq + h] = ok ? acc2 : 0.0;
+        P1s[cc][rq + h] = ok ? (double)lane_at<S, EL>(pv[k], h) : 0.0;
+      }
     }
 #pragma unroll
     for (int k = 0; k < NFQ; ++k) {
@@ -629,8 +635,8 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   B.rq0 = (double*)w; w += align256(sizeof(double) * (size_t)nbv);
   B.sc = (double*)w;
 
-  // paired-row loads when every mode size is even (FP64)
-  bool even = sizeof(S) == 8;
+  // paired loads when every mode size and R are even (FP64)
+  bool even = sizeof(S) == 8 && s.R % 2 == 0;
   for (int m = 0; m < s.L; ++m) even = even && s.d[m] % 2 == 0;
   const auto kA = even ? k_cgm_A<S, 2> : k_cgm_A<S, 1>;
   const auto kZ = even ? k_cgm_Z<S, 2> : k_cgm_Z<S, 1>;
