@@ -33,6 +33,17 @@ template <typename S>
 struct Lanes<S, 1> {
   typedef S T;
 };
+// a[i] for a run-time i, from an array read with constant indices.  For the kernel-argument
+// arrays this reads every entry in one burst of scalar loads at kernel entry, where a[i] would be
+// one more dependent scalar-load round trip (each costs a miss on the argument block).
+template <typename T, int N>
+__device__ inline T pick(const T* a, int i) {
+  T v = a[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) v = i == k ? a[k] : v;
+  return v;
+}
+
 template <typename S, int EL>
 __device__ inline S lane_at(const typename Lanes<S, EL>::T& v, int h) {
   if constexpr (EL == 1) {
@@ -192,33 +203,38 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
     for (int e = threadIdx.x; e < s.L * s.R; e += NTH) B.Mv[e] = inv_m(B.gamma[e], P.damp);
   const bool scal = !raw && j >= 1;
   const int lane = threadIdx.x & 63;
+  // block -> mode, and every argument used below, with constant indices (see pick)
+  int m = 0;
+#pragma unroll
+  for (int q = 1; q < TFX_MAXL; ++q) m += (q < s.L && (int)blockIdx.x >= P.k1first[q]) ? 1 : 0;
+  const int nrb = P.nrb, nk2 = P.nk2;
+  const int64_t I = pick<int64_t, TFX_MAXL>(s.d, m);
+  const int64_t off = pick<int64_t, TFX_MAXL>(s.off, m);
+  const int64_t aoffm = pick<int64_t, TFX_MAXL>(P.aoff, m);
+  const int local = (int)blockIdx.x - pick<int, TFX_MAXL>(P.k1first, m);
+  const int odd = j & 1;
+  const S* pold = odd ? B.p[0] : B.p[1];
+  S* pnew = odd ? B.p[1] : B.p[0];
+  S* rnew = odd ? B.r[1] : B.r[0];
+  const S* src1 = raw ? B.grad : (j == 0 ? B.p[0] : (odd ? B.r[0] : B.r[1]));
+  const double* dprev = odd ? B.dots[0] : B.dots[1];
   // K2(j - 1)'s dot-product partials (lanes < nk2; the rest after the staging loads)
   double dq[D_N] = {0.0, 0.0, 0.0, 0.0};
-  if (scal && lane < P.nk2) {
+  if (scal && lane < nk2) {
 #pragma unroll
-    for (int q = 0; q < D_N; ++q) dq[q] = B.dots[(j - 1) & 1][q * P.nk2 + lane];
+    for (int q = 0; q < D_N; ++q) dq[q] = dprev[q * nk2 + lane];
   }
   double alpha = 0.0, beta = 0.0;
   S aS = S(0);
-  int m = 0;
-  while (m + 1 < s.L && (int)blockIdx.x >= P.k1first[m + 1]) ++m;
-  const int nrb = P.nrb;
-  const int local = (int)blockIdx.x - P.k1first[m];
   const int split = local / (nrb * nrb);
   const int cb = (local / nrb) % nrb, rbk = local % nrb;
   const int c0 = cb * CT, r0 = rbk * CT;
-  const int64_t I = s.d[m];
   const int64_t ib = (int64_t)split * ICH;
   const int64_t ie = ib + ICH < I ? ib + ICH : I;
   const int R = s.R;
-  const S* Fm = B.F + s.off[m];
-  const int64_t off = s.off[m];
-  const S* pold = B.p[(j + 1) & 1];
-  S* pnew = B.p[j & 1];
-  S* rnew = B.r[j & 1];
+  const S* Fm = B.F + off;
   const bool upd = !raw && j >= 1;
   const bool writer = upd && rbk == 0;
-  const S* src1 = raw ? B.grad : (j == 0 ? B.p[0] : B.r[(j + 1) & 1]);
   const int t = threadIdx.x;
   // wave 0 forms the 16 x 16 block on FP64 MFMA: lane (lg, li) supplies q[4 k + lg][c0 + li] and
   // F[4 k + lg][r0 + li] at step k; register jj holds c = c0 + 4 jj + lg, r = r0 + li
@@ -257,7 +273,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
 #pragma unroll
         for (int q = 0; q < D_N; ++q) {
           double v = 0.0 + dq[q];
-          for (int i = lane + 64; i < P.nk2; i += 64) v += B.dots[(j - 1) & 1][q * P.nk2 + i];
+          for (int i = lane + 64; i < nk2; i += 64) v += dprev[q * nk2 + i];
           tq[q] = wave_sum(v);
         }
         double rn = tq[D_RR];
@@ -332,7 +348,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
       const int c = c0 + 4 * jj + lg;
-      if (c < R && rr < R) B.A[P.aoff[m] + ((int64_t)split * R + c) * R + rr] = acc[jj];
+      if (c < R && rr < R) B.A[aoffm + ((int64_t)split * R + c) * R + rr] = acc[jj];
     }
   }
 }
@@ -345,18 +361,21 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   __shared__ double Ts[ZC][CT];
   __shared__ double P1s[ZC][CT];
   const Shape& s = P.s;
+  // block -> mode, and the arguments used below, with constant indices (see pick)
   int l = 0;
-  while (l + 1 < s.L && (int)blockIdx.x >= P.k2first[l + 1]) ++l;
+#pragma unroll
+  for (int q = 1; q < TFX_MAXL; ++q) l += (q < s.L && (int)blockIdx.x >= P.k2first[q]) ? 1 : 0;
   const int nrb = P.nrb;
-  const int local = (int)blockIdx.x - P.k2first[l];
+  const int local = (int)blockIdx.x - pick<int, TFX_MAXL>(P.k2first, l);
   const int ibk = local / nrb, rbk = local % nrb;
-  const int64_t I = s.d[l];
+  const int64_t I = pick<int64_t, TFX_MAXL>(s.d, l);
   const int64_t i0 = (int64_t)ibk * ZI;
   const int r0 = rbk * CT;
   const int R = s.R, L = s.L;
   const int64_t RR = (int64_t)R * R;
-  const int64_t off = s.off[l];
-  const S* pj = B.p[j & 1];
+  const int64_t off = pick<int64_t, TFX_MAXL>(s.off, l);
+  const S* pj = j & 1 ? B.p[1] : B.p[0];
+  double* dout = j & 1 ? B.dots[1] : B.dots[0];
   const int t = threadIdx.x;
   // wave w owns rows i0 + 16 w .. + 15 of the 64 x 16 output block, as one FP64 MFMA tile:
   // register j of lane (li, lg) holds row 16 w + 4 j + lg, column r0 + li
@@ -368,7 +387,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
     const int rq = r0 + li < R ? r0 + li : 0;
     eg = (double)B.gamma[l * R + rq];
     eM = B.Mv[l * R + rq];
-    const S* rjp = B.r[j & 1];
+    const S* rjp = j & 1 ? B.r[1] : B.r[0];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const int64_t i = i0 + 16 * wv + 4 * a + lg;
@@ -521,7 +540,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   if (threadIdx.x < D_N) {
     double tot = 0.0;
     for (int q = 0; q < NTH / 64; ++q) tot += red4[threadIdx.x][q];
-    B.dots[j & 1][threadIdx.x * P.nk2 + blockIdx.x] = tot;
+    dout[threadIdx.x * P.nk2 + blockIdx.x] = tot;
   }
 }
 
