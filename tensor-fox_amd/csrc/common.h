@@ -55,6 +55,11 @@ struct Shape {
 
 int tfx_make_shape(tfx_ctx* ctx, int L, const int64_t* dims, int R, Shape* s);
 
+// Up to four copies src[k] -> dst[k] of n[k] doubles in one launch on `stream` (library-internal;
+// a destination may be pinned host memory).
+int tfx_copies(tfx_ctx* ctx, int np, const double* const* src, double* const* dst, const int64_t* n,
+               hipStream_t stream);
+
 static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Sum over the 64 lanes of a wavefront, returned in every lane; all lanes must be active.  DPP

@@ -811,6 +811,22 @@ __global__ void k_axpby(int64_t n, double a, const S* __restrict__ y, double b,
   if (e < n) out[e] = (S)(a * (double)y[e] + b * (double)g[e]);
 }
 
+// Up to four copies of n doubles in one launch (pairs past `np` are unused); a destination may be
+// pinned host memory (the per-iteration scalars of the dGN loop).
+struct Copies {
+  const double* src[4];
+  double* dst[4];
+  int64_t n[4];
+  int np;
+};
+
+__global__ void k_copies(Copies C) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < C.np && e < C.n[k]) C.dst[k][e] = C.src[k][e];
+}
+
 template <typename A, typename B>
 __global__ void k_convert(int64_t n, const A* __restrict__ src, B* __restrict__ dst) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1010,6 +1026,24 @@ extern "C" int tfx_axpby(tfx_ctx* ctx, int dtype, int64_t n, double a, const voi
                        (const double*)g, (double*)out);
   else
     return tfx_set_error(ctx, TFX_E_ARG, "bad dtype %d", dtype);
+  TFX_CHECK_LAUNCH(ctx);
+  return TFX_OK;
+}
+
+int tfx_copies(tfx_ctx* ctx, int np, const double* const* src, double* const* dst, const int64_t* n,
+               hipStream_t stream) {
+  if (np < 1 || np > 4) return tfx_set_error(ctx, TFX_E_ARG, "copies: 1 to 4 pairs");
+  Copies C{};
+  int64_t nmax = 0;
+  for (int k = 0; k < np; ++k) {
+    C.src[k] = src[k];
+    C.dst[k] = dst[k];
+    C.n[k] = n[k];
+    nmax = n[k] > nmax ? n[k] : nmax;
+  }
+  C.np = np;
+  if (nmax == 0) return TFX_OK;
+  hipLaunchKernelGGL(k_copies, dim3((unsigned)((nmax + 255) / 256)), dim3(256), 0, stream, C);
   TFX_CHECK_LAUNCH(ctx);
   return TFX_OK;
 }

@@ -17,8 +17,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <initializer_list>
+#include <utility>
 
 #include "tfxhip.h"
+
+// library-internal (dgn.hip): up to four copies of n[k] doubles in one launch
+int tfx_copies(tfx_ctx* ctx, int np, const double* const* src, double* const* dst, const int64_t* n,
+               hipStream_t stream);
 
 namespace {
 
@@ -210,7 +216,6 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
     n += (int64_t)R * dims[l];
     sdims += dims[l];
   }
-  const size_t vb = sizeof(double) * (size_t)n;
   const int maxiter = o->maxiter;
   double* steps = hist;
   double* errors = hist + maxiter;
@@ -223,7 +228,8 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
   const int symm1 = o->symm ? (o->factors_norm > 0 ? 1 : 2) : 0;   // the transform on shared arrays
   int symm_now = o->symm ? (o->factors_norm > 0 ? 1 : (o->aliased ? 2 : 1)) : 0;
   double* pin = nullptr;
-  if (hipHostMalloc((void**)&pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) return TFX_E_NOMEM;
+  // coherent pinned memory: the copy kernel writes the scalars straight into it
+  if (hipHostMalloc((void**)&pin, 64 * sizeof(double), hipHostMallocCoherent) != hipSuccess) return TFX_E_NOMEM;
   hipEvent_t ev_scal = nullptr;
   if (hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming) != hipSuccess) {
     hipHostFree(pin);
@@ -249,9 +255,21 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
       goto done;                          \
     }                                     \
   } while (0)
-  HIPRUN(hipMemcpyAsync(x, F0, vb, hipMemcpyDeviceToDevice, st));
-  HIPRUN(hipMemcpyAsync(b.F, F0, vb, hipMemcpyDeviceToDevice, st));
-  HIPRUN(hipMemcpyAsync(b.best, F0, vb, hipMemcpyDeviceToDevice, st));
+  // copies in one launch each (hipMemcpyAsync costs a runtime blit per call)
+  auto copies = [&](std::initializer_list<std::pair<const double*, double*>> pairs, int64_t cnt) {
+    const double* src[4];
+    double* dst[4];
+    int64_t nn[4];
+    int np = 0;
+    for (const auto& pr : pairs) {
+      src[np] = pr.first;
+      dst[np] = pr.second;
+      nn[np] = cnt;
+      ++np;
+    }
+    return tfx_copies(ctx, np, src, dst, nn, st);
+  };
+  RUN(copies({{F0, x}, {F0, b.F}, {F0, b.best}}, n));
   for (it = 0; it < maxiter; ++it) {
     const double old_error = error;
     int cap;
@@ -273,7 +291,7 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
                        b.scal + 2, st));
     symm_now = symm1;
     RUN(tfx_cpd_residual_mixed(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, b.scal + 6, st));
-    HIPRUN(hipMemcpyAsync(pin, b.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    RUN(copies({{b.scal, pin}}, 8));
     HIPRUN(hipEventRecord(ev_scal, st));
     // The next iteration's Gram products and gradient depend only on the updated factors: they
     // go into the other set now, so the device works while the host reads this iteration's
@@ -297,16 +315,13 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
       ++dogleg_calls;
       int cur = 0;
       RUN(tfx_axpby(ctx, TFX_F64, n, 1.0, x, -1.0, b.y, b.dx[0], st));
-      HIPRUN(hipMemcpyAsync(b.dy[0], b.y, vb, hipMemcpyDeviceToDevice, st));
-      HIPRUN(hipMemcpyAsync(b.dbx, b.dx[0], vb, hipMemcpyDeviceToDevice, st));
-      HIPRUN(hipMemcpyAsync(b.dby, b.dy[0], vb, hipMemcpyDeviceToDevice, st));
-      HIPRUN(hipMemcpyAsync(b.dbF, b.F, vb, hipMemcpyDeviceToDevice, st));
+      RUN(copies({{b.y, b.dy[0]}, {b.dx[0], b.dbx}, {b.y, b.dby}, {b.F, b.dbF}}, n));
       double derr = error, dbest = error, gain = 1.0, delta = 1.0;
       int count = 0;
       while (gain > 0) {
         const double dold = derr;
         RUN(tfx_dogleg_dots(ctx, TFX_F64, n, b.dy[cur], g.grad, b.jtjg, b.tscal, st));
-        HIPRUN(hipMemcpyAsync(pin + 16, b.tscal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        RUN(copies({{b.tscal, pin + 16}}, 4));
         HIPRUN(hipStreamSynchronize(st));
         double ca, cb;
         dogleg_coeffs(pin + 16, delta, &ca, &cb);
@@ -315,14 +330,12 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
         RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, b.dx[cur], b.dy[nx], 1.0, g.grad, damp, 0, symm1,
                            o->factors_norm, b.dx[nx], b.F, b.tscal + 4, st));
         RUN(tfx_cpd_residual_mixed(ctx, T, t_dtype, L, dims, b.F, TFX_F64, R, b.tscal + 8, st));
-        HIPRUN(hipMemcpyAsync(pin + 24, b.tscal + 4, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+        RUN(copies({{b.tscal + 4, pin + 24}}, 5));
         HIPRUN(hipStreamSynchronize(st));
         derr = sqrt(pin[28]) / Tsize;
         gain = gain_ratio(dold, derr, Tsize, pin[24]);
         if (derr < dold) {
-          HIPRUN(hipMemcpyAsync(b.dbx, b.dx[nx], vb, hipMemcpyDeviceToDevice, st));
-          HIPRUN(hipMemcpyAsync(b.dby, b.dy[nx], vb, hipMemcpyDeviceToDevice, st));
-          HIPRUN(hipMemcpyAsync(b.dbF, b.F, vb, hipMemcpyDeviceToDevice, st));
+          RUN(copies({{b.dx[nx], b.dbx}, {b.dy[nx], b.dby}, {b.F, b.dbF}}, n));
           dbest = derr;
         }
         delta = update_delta(delta, gain, sqrt(pin[25]));
@@ -330,14 +343,12 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
         if (++count > 10) break;
       }
       error = dbest;
-      HIPRUN(hipMemcpyAsync(b.F, b.dbF, vb, hipMemcpyDeviceToDevice, st));
-      HIPRUN(hipMemcpyAsync(x, b.dbx, vb, hipMemcpyDeviceToDevice, st));
-      HIPRUN(hipMemcpyAsync(b.y, b.dby, vb, hipMemcpyDeviceToDevice, st));
+      RUN(copies({{b.dbF, b.F}, {b.dbx, x}, {b.dby, b.y}}, n));
       // x - old_x statistics for this iteration's gain ratio and step size
       RUN(tfx_axpby(ctx, TFX_F64, n, 1.0, x, -1.0, old_x, b.d, st));
       RUN(tfx_dgn_update(ctx, TFX_F64, L, dims, R, old_x, b.d, 1.0, g.grad, damp, 0, 0, 0.0, b.sx, b.sF,
                          b.tscal, st));
-      HIPRUN(hipMemcpyAsync(pin + 32, b.tscal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+      RUN(copies({{b.tscal, pin + 32}}, 4));
       HIPRUN(hipStreamSynchronize(st));
       gain_den = pin[32];
       dx2 = pin[33];
@@ -347,7 +358,7 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
     damp = update_damp(damp, damp_sched, gr, it);
     if (error < best_error) {
       best_error = error;
-      HIPRUN(hipMemcpyAsync(b.best, b.F, vb, hipMemcpyDeviceToDevice, st));
+      RUN(copies({{b.F, b.best}}, n));
     }
     errors[it] = error;
     steps[it] = sqrt(dx2) / sqrt(x2);
@@ -361,7 +372,7 @@ extern "C" int tfx_dgn_run(tfx_ctx* ctx, const void* T, int t_dtype, int prec, i
     }
   }
   if (it >= maxiter) it = maxiter - 1;
-  HIPRUN(hipMemcpyAsync(F_best, b.best, vb, hipMemcpyDeviceToDevice, st));
+  RUN(copies({{b.best, F_best}}, n));
   HIPRUN(hipStreamSynchronize(st));
   info[0] = it + 1;
   info[1] = stop;
