@@ -94,10 +94,21 @@ struct CgBufs {
   double* sc;
 };
 
-__device__ inline int mode_of_e(const Shape& s, int64_t e, int64_t* within) {
+// Mode of flat element e, its offset within the mode and the mode's size, compared with
+// constant indices into the shape (scalar loads at entry rather than per-element vector loads
+// of the kernel arguments).
+__device__ inline int mode_of_e(const Shape& s, int64_t e, int64_t* within, int64_t* dim) {
   int l = 0;
-  while (l + 1 < s.L && e >= s.off[l + 1]) ++l;
-  *within = e - s.off[l];
+  int64_t o = s.off[0], d = s.d[0];
+#pragma unroll
+  for (int q = 1; q < TFX_MAXL; ++q) {
+    const bool in = q < s.L && e >= s.off[q];
+    l = in ? q : l;
+    o = in ? s.off[q] : o;
+    d = in ? s.d[q] : d;
+  }
+  *within = e - o;
+  *dim = d;
   return l;
 }
 
@@ -171,9 +182,9 @@ __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
   const Shape& s = P.s;
   double acc = 0.0;
   for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
-    int64_t w;
-    const int l = mode_of_e(s, e, &w);
-    const int rk = (int)(w / s.d[l]);
+    int64_t w, I;
+    const int l = mode_of_e(s, e, &w, &I);
+    const int rk = (int)(w / I);
     const double M = B.Mv[l * s.R + rk];
     const S rr = (S)(M * (double)B.grad[e]);
     B.r[0][e] = rr;
@@ -577,9 +588,9 @@ __global__ __launch_bounds__(NTH) void k_cgm_end(CgPlan P, CgBufs<S> B, int maxi
   }
   const S* plast = B.p[(maxiter - 1) & 1];
   for (int64_t e = (int64_t)blockIdx.x * NTH + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * NTH) {
-    int64_t w;
-    const int l = mode_of_e(s, e, &w);
-    const int rk = (int)(w / s.d[l]);
+    int64_t w, I;
+    const int l = mode_of_e(s, e, &w, &I);
+    const int rk = (int)(w / I);
     const S yv = last_upd ? B.yc[e] + aS * plast[e] : B.yc[e];
     y[e] = (S)(B.Mv[l * s.R + rk] * (double)yv);
   }

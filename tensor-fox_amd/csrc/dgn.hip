@@ -21,10 +21,24 @@ int cg_lds_try(tfx_ctx* ctx, const Shape& s, const double* F, const double* P1, 
 
 namespace {
 
-__device__ inline int mode_of(const Shape& s, int64_t e, int64_t* within) {
+// Mode of flat element e, with its offset within the mode and (optionally) the mode's size and
+// offset.  Every comparison uses a constant index into the shape, so the shape is read with
+// scalar loads at kernel entry; a per-element index would read the kernel arguments with
+// dependent vector loads.
+__device__ inline int mode_of(const Shape& s, int64_t e, int64_t* within, int64_t* dim = nullptr,
+                              int64_t* offset = nullptr) {
   int l = 0;
-  while (l + 1 < s.L && e >= s.off[l + 1]) ++l;
-  *within = e - s.off[l];
+  int64_t o = s.off[0], d = s.d[0];
+#pragma unroll
+  for (int q = 1; q < TFX_MAXL; ++q) {
+    const bool in = q < s.L && e >= s.off[q];
+    l = in ? q : l;
+    o = in ? s.off[q] : o;
+    d = in ? s.d[q] : d;
+  }
+  *within = e - o;
+  if (dim) *dim = d;
+  if (offset) *offset = o;
   return l;
 }
 
@@ -127,11 +141,11 @@ __global__ void k_neg_grad(Shape s, const S* __restrict__ F, const S* __restrict
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= s.n) return;
   int64_t w;
-  const int l = mode_of(s, e, &w);
-  const int64_t I = s.d[l];
+  int64_t I, offl;
+  const int l = mode_of(s, e, &w, &I, &offl);
   const int r = (int)(w / I);
   const int64_t i = w % I;
-  const S* Fl = F + s.off[l];
+  const S* Fl = F + offl;
   const S* P = P1 + (int64_t)l * s.R * s.R;
   // four partial sums (independent loads and FMA chains), combined in a fixed order
   S a0 = S(0), a1 = S(0), a2 = S(0), a3 = S(0);
@@ -274,12 +288,12 @@ __global__ __launch_bounds__(SMALL_NT) void k_cg_small(
   // stage F (row-major per mode) and P1
   for (int64_t e = t; e < n; e += blockDim.x) {
     int64_t w;
-    const int l = mode_of(s, e, &w);
-    const int64_t I = s.d[l];
+    int64_t I, offl;
+    const int l = mode_of(s, e, &w, &I, &offl);
     const int r = (int)(w / I);
     const int64_t i = w % I;
-    F[s.off[l] + i * R + r] = Fg[e];
-    q[s.off[l] + i * R + r] = grad[e];   // J^T J grad first (no preconditioner)
+    F[offl + i * R + r] = Fg[e];
+    q[offl + i * R + r] = grad[e];   // J^T J grad first (no preconditioner)
   }
   for (int64_t e = t; e < LRR; e += blockDim.x) P1[e] = P1g[e];
   __syncthreads();
