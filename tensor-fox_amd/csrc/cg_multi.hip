@@ -56,7 +56,7 @@ __device__ inline S lane_at(const typename Lanes<S, EL>::T& v, int h) {
 constexpr int CT = 16;        // c / r tile of the R x R products
 constexpr int ICH = 128;      // i per workgroup in K1 (split of the reduction over I_m)
 constexpr int KI = 64;        // i staged per step in K1 (a 50-row mode in one step)
-constexpr int ZI = 64;        // rows i per workgroup in K2
+constexpr int ZI = 16;        // rows i per workgroup in K2 (one MFMA row tile)
 constexpr int ZC = 64;        // c per LDS chunk in K2
 constexpr int NTH = 256;
 
@@ -364,7 +364,9 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   }
 }
 
-// K2: z_l[i][r] for rows i0..i0+63 and ranks r0..r0+15 of mode l; c in chunks of ZC.
+// K2: z_l[i][r] for rows i0..i0+15 and ranks r0..r0+15 of mode l; c in chunks of ZC.  The four
+// waves take c slices 16 w .. 16 w + 15 of each chunk (8 MFMAs each), and wave 0 adds their
+// tiles in wave order and finishes the block.
 template <typename S, int EL>
 __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, int j, S* out) {
   __shared__ double Fs[ZI][ZC + 1];
@@ -388,20 +390,20 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
   const S* pj = j & 1 ? B.p[1] : B.p[0];
   double* dout = j & 1 ? B.dots[1] : B.dots[0];
   const int t = threadIdx.x;
-  // wave w owns rows i0 + 16 w .. + 15 of the 64 x 16 output block, as one FP64 MFMA tile:
-  // register j of lane (li, lg) holds row 16 w + 4 j + lg, column r0 + li
-  const int wv = t >> 6, li = t & 15, lg = (t >> 4) & 3;
-  // the epilogue's operands and the first chunk's staging operands are loaded before the stop
+  // the 16 x 16 output block as one FP64 MFMA tile: register a of lane (li, lg) holds row
+  // i0 + 4 a + lg, column r0 + li
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6), li = t & 15, lg = (t >> 4) & 3;
+  // wave 0's epilogue operands and the first chunk's staging operands are loaded before the stop
   // flag is read (K1 of this iteration sets it, so every wave reads the same value)
-  double eg = 0.0, eM = 0.0, ep[4], er[4];
-  {
+  double eg = 0.0, eM = 0.0, ep[4] = {0.0, 0.0, 0.0, 0.0}, er[4] = {0.0, 0.0, 0.0, 0.0};
+  if (wv == 0) {
     const int rq = r0 + li < R ? r0 + li : 0;
     eg = (double)B.gamma[l * R + rq];
     eM = B.Mv[l * R + rq];
     const S* rjp = j & 1 ? B.r[1] : B.r[0];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const int64_t i = i0 + 16 * wv + 4 * a + lg;
+      const int64_t i = i0 + 4 * a + lg;
       const int64_t e = (i < I && r0 + li < R) ? off + (int64_t)(r0 + li) * I + i : 0;
       ep[a] = (double)pj[e];
       er[a] = (double)rjp[e];
@@ -500,21 +502,30 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
       }
     }
     __syncthreads();
-    // z[i][r] += sum_c F[i][c] tmp[c][r] + q[i][c] P1[c][r] (columns past R are zeros)
+    // z[i][r] += sum_c F[i][c] tmp[c][r] + q[i][c] P1[c][r] over this wave's c slice (columns
+    // past R are zeros)
 #pragma unroll
-    for (int k4 = 0; k4 < ZC / 4; ++k4) {
-      const int cc = 4 * k4 + lg;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Fs[16 * wv + li][cc], Ts[cc][li], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Qs[16 * wv + li][cc], P1s[cc][li], acc, 0, 0, 0);
+    for (int k4 = 0; k4 < ZC / 16; ++k4) {
+      const int cc = 16 * wv + 4 * k4 + lg;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Fs[li][cc], Ts[cc][li], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Qs[li][cc], P1s[cc][li], acc, 0, 0, 0);
     }
     __syncthreads();
   }
+  // the waves' tiles, added by wave 0 in wave order
+  __shared__ double Zp[NTH / 64][4][64];
+  const int lane = t & 63;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) Zp[wv][a][lane] = acc[a];
+  __syncthreads();
+  if (wv != 0) return;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) acc[a] = ((Zp[0][a][lane] + Zp[1][a][lane]) + Zp[2][a][lane]) + Zp[3][a][lane];
   double dpz = 0.0, drz = 0.0, dzz = 0.0, drr = 0.0;
-  const S* rj = B.r[j & 1];
   const int rr = r0 + li;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int64_t i = i0 + 16 * wv + 4 * a + lg;
+    const int64_t i = i0 + 4 * a + lg;
     if (i < I && rr < R) {
       const int64_t e = off + (int64_t)rr * I + i;
       if (raw) {
@@ -534,24 +545,16 @@ __global__ __launch_bounds__(NTH) void k_cgm_Z(CgPlan P, CgBufs<S> B, int raw, i
     }
   }
   if (raw) return;
-  // the four block sums in one pass (fixed order)
+  // the block's four dot products (wave 0 alone)
   dpz = wave_sum(dpz);
   drz = wave_sum(drz);
   dzz = wave_sum(dzz);
   drr = wave_sum(drr);
-  __shared__ double red4[D_N][NTH / 64];
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red4[D_PZ][w] = dpz;
-    red4[D_RZ][w] = drz;
-    red4[D_ZZ][w] = dzz;
-    red4[D_RR][w] = drr;
-  }
-  __syncthreads();
-  if (threadIdx.x < D_N) {
-    double tot = 0.0;
-    for (int q = 0; q < NTH / 64; ++q) tot += red4[threadIdx.x][q];
-    dout[threadIdx.x * P.nk2 + blockIdx.x] = tot;
+  if (lane == 0) {
+    dout[D_PZ * P.nk2 + blockIdx.x] = dpz;
+    dout[D_RZ * P.nk2 + blockIdx.x] = drz;
+    dout[D_ZZ * P.nk2 + blockIdx.x] = dzz;
+    dout[D_RR * P.nk2 + blockIdx.x] = drr;
   }
 }
 
