@@ -247,9 +247,11 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   const bool upd = !raw && j >= 1;
   const bool writer = upd && rbk == 0;
   const int t = threadIdx.x;
-  // wave 0 forms the 16 x 16 block on FP64 MFMA: lane (lg, li) supplies q[4 k + lg][c0 + li] and
-  // F[4 k + lg][r0 + li] at step k; register jj holds c = c0 + 4 jj + lg, r = r0 + li
+  // the 16 x 16 block on FP64 MFMA, wave w taking steps 4 w .. 4 w + 3 of each 64-row chunk:
+  // lane (lg, li) supplies q[4 k + lg][c0 + li] and F[4 k + lg][r0 + li] at step k; register jj
+  // holds c = c0 + 4 jj + lg, r = r0 + li; wave 0 adds the waves' blocks in wave order
   const int li = t & 15, lg = (t >> 4) & 3;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   // EL consecutive rows per load (EL = 2 when every mode size is even, so that element offsets
   // are even and a row range [i, i + 2) is either inside [ib, ie) or outside it)
@@ -347,19 +349,22 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
       }
     }
     __syncthreads();
-    if (t < 64) {
 #pragma unroll
-      for (int k = 0; k < KI / 4; ++k)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qs[4 * k + lg][li], fs[4 * k + lg][li], acc, 0, 0, 0);
-    }
+    for (int k = 4 * wv; k < 4 * wv + 4; ++k)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qs[4 * k + lg][li], fs[4 * k + lg][li], acc, 0, 0, 0);
     __syncthreads();
   }
-  if (t < 64) {
+  __shared__ double Ap[NTH / 64][4][64];
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) Ap[wv][jj][lane] = acc[jj];
+  __syncthreads();
+  if (wv == 0) {
     const int rr = r0 + li;
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
       const int c = c0 + 4 * jj + lg;
-      if (c < R && rr < R) B.A[aoffm + ((int64_t)split * R + c) * R + rr] = acc[jj];
+      const double v = ((Ap[0][jj][lane] + Ap[1][jj][lane]) + Ap[2][jj][lane]) + Ap[3][jj][lane];
+      if (c < R && rr < R) B.A[aoffm + ((int64_t)split * R + c) * R + rr] = v;
     }
   }
 }
