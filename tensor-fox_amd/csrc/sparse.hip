@@ -15,8 +15,10 @@
 //
 // Layout: coordinates are L columns of nnz int32 (column l = sorted mode l), data is nnz FP64 in
 // the same entry order.  The first sorted mode is the longest, so contracting it first gives the
-// smallest dense intermediate.  Gram and first-mode product accumulate with FP64 atomics into
-// device-zeroed outputs; their summation order (and last-bit result) depends on scheduling.
+// smallest dense intermediate.  Gram and first-mode product sum in a fixed order (entries sorted
+// by column with stable radix sorts, one thread per output row or per column run), so their
+// results do not depend on scheduling; the truncation that follows compares roundoff-sized
+// singular values, so this keeps whole runs reproducible.
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -78,26 +80,62 @@ __global__ __launch_bounds__(kThreads) void k_col_keys(const int32_t* __restrict
   }
 }
 
-// One thread per sorted position s: the pairs (s, t), t >= s, inside the run of equal keys.
-// The diagonal pair adds d^2 once, the others add d_s d_t to both (i, j) and (j, i).
-__global__ __launch_bounds__(kThreads) void k_gram_pairs(const uint64_t* __restrict__ key,
-                                                         const int32_t* __restrict__ ent,
-                                                         const int32_t* __restrict__ rowc,
-                                                         const double* __restrict__ data, int64_t nnz,
-                                                         int64_t I, double* __restrict__ G) {
+// Row keys of the column-sorted entries: rkey[s] = row of the entry at column-sorted position s,
+// rpos[s] = s (sorted next by row, stably, so each row lists its entries in column order).
+__global__ __launch_bounds__(kThreads) void k_row_keys(const int32_t* __restrict__ ent,
+                                                       const int32_t* __restrict__ rowc, int64_t nnz,
+                                                       uint32_t* __restrict__ rkey, int32_t* __restrict__ rpos) {
   for (int64_t s = blockIdx.x * (int64_t)kThreads + threadIdx.x; s < nnz; s += (int64_t)gridDim.x * kThreads) {
-    const uint64_t k = key[s];
-    const int32_t e = ent[s];
-    const int64_t i = rowc[e];
-    const double d = data[e];
-    unsafeAtomicAdd(&G[i * I + i], d * d);
-    for (int64_t t = s + 1; t < nnz && key[t] == k; ++t) {
-      const int32_t f = ent[t];
-      const int64_t j = rowc[f];
-      const double v = d * data[f];
-      unsafeAtomicAdd(&G[i * I + j], v);
-      unsafeAtomicAdd(&G[j * I + i], v);
+    rkey[s] = (uint32_t)rowc[ent[s]];
+    rpos[s] = (int32_t)s;
+  }
+}
+
+// One thread per row i of G: for each entry e of row i (column order) and each entry f in the
+// run of e's column (column-sorted order), G[i][row f] += d_e d_f where row f >= i.  Each
+// unordered pair of rows is summed by the smaller row's thread alone, in a fixed order (pairs
+// inside row i come in both orders, so duplicates add up as (sum d)^2); k_gram_mirror copies the
+// upper triangle down.
+__global__ __launch_bounds__(kThreads) void k_gram_rows(const uint64_t* __restrict__ key,
+                                                        const int32_t* __restrict__ ent,
+                                                        const uint32_t* __restrict__ rkey,
+                                                        const int32_t* __restrict__ rpos,
+                                                        const int32_t* __restrict__ rowc,
+                                                        const double* __restrict__ data, int64_t nnz,
+                                                        int64_t I, double* __restrict__ G) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < I; i += (int64_t)gridDim.x * kThreads) {
+    // the range of row i in the row-sorted keys (lower and upper bound)
+    int64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if ((int64_t)rkey[mid] < i) lo = mid + 1; else hi = mid;
     }
+    int64_t end = lo;
+    hi = nnz;
+    while (end < hi) {
+      const int64_t mid = (end + hi) / 2;
+      if ((int64_t)rkey[mid] <= i) end = mid + 1; else hi = mid;
+    }
+    double* Gi = G + i * I;
+    for (int64_t q = lo; q < end; ++q) {
+      const int64_t s = rpos[q];
+      const uint64_t k = key[s];
+      const double d = data[ent[s]];
+      int64_t t = s;
+      while (t > 0 && key[t - 1] == k) --t;
+      for (; t < nnz && key[t] == k; ++t) {
+        const int32_t f = ent[t];
+        const int64_t j = rowc[f];
+        if (j >= i) Gi[j] += d * data[f];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_gram_mirror(int64_t I, double* __restrict__ G) {
+  for (int64_t g = blockIdx.x * (int64_t)kThreads + threadIdx.x; g < I * I; g += (int64_t)gridDim.x * kThreads) {
+    const int64_t r = g / I, c = g - r * I;
+    if (r > c) G[g] = G[c * I + r];
   }
 }
 
@@ -108,19 +146,38 @@ struct Ttm0Args {
   int64_t stride[TFX_MAXL];   // C-order strides of modes 1..L-1 inside C (stride[0] unused)
 };
 
-// Flat index g = e * k + s over (entry, output row): Y[s, rest(e)] += d_e U[c_0e, s].
-__global__ __launch_bounds__(kThreads) void k_sparse_ttm0(const int32_t* __restrict__ idx,
+// rest(e): C-order linear index of the coordinates of modes 1..L-1 (the output column).
+__global__ __launch_bounds__(kThreads) void k_rest_keys(const int32_t* __restrict__ idx, int64_t nnz,
+                                                        Ttm0Args a, uint64_t* __restrict__ key,
+                                                        int32_t* __restrict__ ent) {
+  for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kThreads) {
+    uint64_t rest = 0;
+    for (int l = 1; l < a.L; ++l) rest += (uint64_t)idx[(int64_t)l * nnz + e] * (uint64_t)a.stride[l];
+    key[e] = rest;
+    ent[e] = (int32_t)e;
+  }
+}
+
+// Flat index g = p * k + s over (rest-sorted position, output row): the position that starts a
+// run of equal rest sums d_e U[c_0e, s] over the run in sorted order into Y[s, rest].
+__global__ __launch_bounds__(kThreads) void k_sparse_ttm0(const uint64_t* __restrict__ key,
+                                                          const int32_t* __restrict__ ent,
+                                                          const int32_t* __restrict__ idx0,
                                                           const double* __restrict__ data, int64_t nnz,
                                                           const double* __restrict__ U, Ttm0Args a,
                                                           double* __restrict__ Y) {
   const int64_t total = nnz * a.k;
   for (int64_t g = blockIdx.x * (int64_t)kThreads + threadIdx.x; g < total; g += (int64_t)gridDim.x * kThreads) {
-    const int64_t e = g / a.k;
-    const int s = (int)(g - e * a.k);
-    int64_t rest = 0;
-    for (int l = 1; l < a.L; ++l) rest += (int64_t)idx[(int64_t)l * nnz + e] * a.stride[l];
-    const int64_t i0 = idx[e];
-    unsafeAtomicAdd(&Y[(int64_t)s * a.C + rest], data[e] * U[(int64_t)s * a.I0 + i0]);
+    const int64_t p = g / a.k;
+    const int s = (int)(g - p * a.k);
+    const uint64_t k = key[p];
+    if (p > 0 && key[p - 1] == k) continue;
+    double acc = 0.0;
+    for (int64_t t = p; t < nnz && key[t] == k; ++t) {
+      const int32_t e = ent[t];
+      acc += data[e] * U[(int64_t)s * a.I0 + idx0[e]];
+    }
+    Y[(int64_t)s * a.C + (int64_t)k] = acc;
   }
 }
 
@@ -221,25 +278,40 @@ extern "C" int tfx_sparse_gram(tfx_ctx* ctx, int64_t nnz, int L, const int64_t* 
   unsigned end_bit = 1;
   while (end_bit < 64 && (uint64_t(1) << end_bit) < s) ++end_bit;
 
-  size_t temp = 0;
+  unsigned row_bits = 1;
+  while (row_bits < 32 && (int64_t(1) << row_bits) < I) ++row_bits;
+  size_t temp = 0, temp2 = 0;
   TFX_HIP(ctx, rocprim::radix_sort_pairs(nullptr, temp, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                          (int32_t*)nullptr, (int32_t*)nullptr, (size_t)nnz, 0u, end_bit, stream));
+  TFX_HIP(ctx, rocprim::radix_sort_pairs(nullptr, temp2, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                         (int32_t*)nullptr, (int32_t*)nullptr, (size_t)nnz, 0u, row_bits, stream));
+  if (temp2 > temp) temp = temp2;
   const size_t kb = align256(sizeof(uint64_t) * nnz), vb = align256(sizeof(int32_t) * nnz);
   void* wsp;
-  rc = tfx_workspace(ctx, 2 * kb + 2 * vb + align256(temp), stream, &wsp);
+  rc = tfx_workspace(ctx, 2 * kb + 6 * vb + align256(temp), stream, &wsp);
   if (rc) return rc;
   char* w = (char*)wsp;
   uint64_t* k_in = (uint64_t*)w;
   uint64_t* k_out = (uint64_t*)(w + kb);
   int32_t* v_in = (int32_t*)(w + 2 * kb);
   int32_t* v_out = (int32_t*)(w + 2 * kb + vb);
-  void* tmp = w + 2 * kb + 2 * vb;
+  uint32_t* rk_in = (uint32_t*)(w + 2 * kb + 2 * vb);
+  uint32_t* rk_out = (uint32_t*)(w + 2 * kb + 3 * vb);
+  int32_t* rp_in = (int32_t*)(w + 2 * kb + 4 * vb);
+  int32_t* rp_out = (int32_t*)(w + 2 * kb + 5 * vb);
+  void* tmp = w + 2 * kb + 6 * vb;
+  const int32_t* rowc = idx + (int64_t)mode * nnz;
   const int nb = grid_for(ctx, nnz);
   hipLaunchKernelGGL(k_col_keys, dim3(nb), dim3(kThreads), 0, stream, idx, nnz, ka, k_in, v_in);
   TFX_CHECK_LAUNCH(ctx);
   TFX_HIP(ctx, rocprim::radix_sort_pairs(tmp, temp, k_in, k_out, v_in, v_out, (size_t)nnz, 0u, end_bit, stream));
-  hipLaunchKernelGGL(k_gram_pairs, dim3(nb), dim3(kThreads), 0, stream, k_out, v_out,
-                     idx + (int64_t)mode * nnz, data, nnz, I, G);
+  hipLaunchKernelGGL(k_row_keys, dim3(nb), dim3(kThreads), 0, stream, v_out, rowc, nnz, rk_in, rp_in);
+  TFX_CHECK_LAUNCH(ctx);
+  TFX_HIP(ctx, rocprim::radix_sort_pairs(tmp, temp, rk_in, rk_out, rp_in, rp_out, (size_t)nnz, 0u, row_bits, stream));
+  hipLaunchKernelGGL(k_gram_rows, dim3(grid_for(ctx, I)), dim3(kThreads), 0, stream, k_out, v_out, rk_out,
+                     rp_out, rowc, data, nnz, I, G);
+  TFX_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_gram_mirror, dim3(grid_for(ctx, I * I)), dim3(kThreads), 0, stream, I, G);
   TFX_CHECK_LAUNCH(ctx);
   return TFX_OK;
 }
@@ -265,7 +337,28 @@ extern "C" int tfx_sparse_ttm0(tfx_ctx* ctx, int64_t nnz, int L, const int64_t* 
   hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(ctx, C * k)), dim3(kThreads), 0, stream, Y, C * k);
   TFX_CHECK_LAUNCH(ctx);
   if (nnz == 0) return TFX_OK;
-  hipLaunchKernelGGL(k_sparse_ttm0, dim3(grid_for(ctx, nnz * k)), dim3(kThreads), 0, stream, idx, data, nnz, U, a, Y);
+  // entries sorted (stably) by output column; one sum per (column run, output row)
+  unsigned end_bit = 1;
+  while (end_bit < 64 && (uint64_t(1) << end_bit) < (uint64_t)C) ++end_bit;
+  size_t temp = 0;
+  TFX_HIP(ctx, rocprim::radix_sort_pairs(nullptr, temp, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                         (int32_t*)nullptr, (int32_t*)nullptr, (size_t)nnz, 0u, end_bit, stream));
+  const size_t kb = align256(sizeof(uint64_t) * nnz), vb = align256(sizeof(int32_t) * nnz);
+  void* wsp;
+  rc = tfx_workspace(ctx, 2 * kb + 2 * vb + align256(temp), stream, &wsp);
+  if (rc) return rc;
+  char* w = (char*)wsp;
+  uint64_t* k_in = (uint64_t*)w;
+  uint64_t* k_out = (uint64_t*)(w + kb);
+  int32_t* v_in = (int32_t*)(w + 2 * kb);
+  int32_t* v_out = (int32_t*)(w + 2 * kb + vb);
+  void* tmp = w + 2 * kb + 2 * vb;
+  const int nb = grid_for(ctx, nnz);
+  hipLaunchKernelGGL(k_rest_keys, dim3(nb), dim3(kThreads), 0, stream, idx, nnz, a, k_in, v_in);
+  TFX_CHECK_LAUNCH(ctx);
+  TFX_HIP(ctx, rocprim::radix_sort_pairs(tmp, temp, k_in, k_out, v_in, v_out, (size_t)nnz, 0u, end_bit, stream));
+  hipLaunchKernelGGL(k_sparse_ttm0, dim3(grid_for(ctx, nnz * k)), dim3(kThreads), 0, stream, k_out, v_out, idx,
+                     data, nnz, U, a, Y);
   TFX_CHECK_LAUNCH(ctx);
   return TFX_OK;
 }

@@ -75,6 +75,32 @@ def test_sparse_gram_matches_oracle(sgolden, kind, spec):
         assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), l
 
 
+@pytest.mark.parametrize('kind,spec', [CASES[2], CASES[4]])
+def test_sparse_gram_and_first_product_are_reproducible(sgolden, kind, spec):
+    """The Gram and the first-mode product sum in a fixed order: two calls give the same bits,
+    and the Gram is exactly symmetric.  The truncation after them compares roundoff-sized
+    singular values (the golden g3b case keeps a core size that roundoff decides), so anything
+    less makes whole runs differ from one call to the next."""
+    import torch
+    data, idxs, dims = _case(sgolden, kind, spec)
+    dev, X, ordering = _packed(data, idxs, dims)
+    for l in range(X.L):
+        I = X.dims[l]
+        G1, G2 = dev.empty((I, I), torch.float64), dev.empty((I, I), torch.float64)
+        dev.sparse_gram(X.nnz, X.dims, X.idx, X.data, l, G1)
+        dev.sparse_gram(X.nnz, X.dims, X.idx, X.data, l, G2)
+        g1 = G1.cpu().numpy()
+        assert np.array_equal(g1, G2.cpu().numpy()), l
+        assert np.array_equal(g1, g1.T), l
+    k = min(X.dims[0], 5)
+    U = dev.upload(np.random.RandomState(4).randn(X.dims[0] * k), torch.float64)
+    n = k * int(np.prod(X.dims[1:]))
+    S1, S2 = dev.empty(n, torch.float64), dev.empty(n, torch.float64)
+    dev.sparse_ttm0(X.nnz, X.dims, X.idx, X.data, U, k, S1)
+    dev.sparse_ttm0(X.nnz, X.dims, X.idx, X.data, U, k, S2)
+    assert np.array_equal(S1.cpu().numpy(), S2.cpu().numpy())
+
+
 @pytest.mark.parametrize('kind,spec', CASES)
 def test_sparse_core_and_residual_match_oracle(sgolden, kind, spec):
     import torch
