@@ -175,7 +175,8 @@ __device__ inline IterScalars iter_scalars(const CgPlan& P, const CgBufs<S>& B, 
   return v;
 }
 
-// K0: r_0 = M grad, p_0 = r_0, y = 0, partials of r_0 . r_0.
+// K0 (launched only when maxiter is 0; otherwise K1(0) forms the start): r_0 = M grad, p_0 = r_0,
+// y = 0, partials of r_0 . r_0.
 template <typename S>
 __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
   __shared__ double red[NTH / 64];
@@ -197,9 +198,11 @@ __global__ __launch_bounds__(NTH) void k_cgm_start(CgPlan P, CgBufs<S> B) {
 }
 
 // K1: A_part[m][split][c][r] = sum_{i in split} q_m[i][c] F_m[i][r].
-//   raw: q = grad (J^T J grad); else q = M p_j.  For j >= 1 every wave forms the scalars of
-//   iteration j - 1 itself (stopping the CG when rn_{j-1} <= tol, j >= 2) while its staging loads
-//   are in flight, then updates its elements: the r-block-0 workgroups store y, r_j and p_j.
+//   raw: q = grad (J^T J grad); else q = M p_j.  At j = 0, p_0 = r_0 = M grad is formed from grad
+//   and the r-block-0 workgroups store r_0, p_0 and y = 0.  For j >= 1 every wave forms the
+//   scalars of iteration j - 1 itself (stopping the CG when rn_{j-1} <= tol, j >= 2) while its
+//   staging loads are in flight, then updates its elements: the r-block-0 workgroups store y,
+//   r_j and p_j.
 //   Block 0 of the raw pass clears the stop flag and fills Mv.
 template <typename S, int EL>
 __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, int j) {
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   const S* pold = odd ? B.p[0] : B.p[1];
   S* pnew = odd ? B.p[1] : B.p[0];
   S* rnew = odd ? B.r[1] : B.r[0];
-  const S* src1 = raw ? B.grad : (j == 0 ? B.p[0] : (odd ? B.r[0] : B.r[1]));
+  const S* src1 = (raw || j == 0) ? B.grad : (odd ? B.r[0] : B.r[1]);
   const double* dprev = odd ? B.dots[0] : B.dots[1];
   // K2(j - 1)'s dot-product partials (lanes < nk2; the rest after the staging loads)
   double dq[D_N] = {0.0, 0.0, 0.0, 0.0};
@@ -244,8 +247,9 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
   const int64_t ie = ib + ICH < I ? ib + ICH : I;
   const int R = s.R;
   const S* Fm = B.F + off;
-  const bool upd = !raw && j >= 1;
-  const bool writer = upd && rbk == 0;
+  // the r-block-0 workgroups store their elements: at j = 0 the start r_0 = p_0 = M grad and
+  // y = 0 (formed here from grad), at j >= 1 the update of iteration j - 1
+  const bool writer = !raw && rbk == 0;
   const int t = threadIdx.x;
   // the 16 x 16 block on FP64 MFMA, wave w taking steps 4 w .. 4 w + 3 of each 64-row chunk:
   // lane (lg, li) supplies q[4 k + lg][c0 + li] and F[4 k + lg][r0 + li] at step k; register jj
@@ -324,7 +328,12 @@ __global__ __launch_bounds__(NTH) void k_cgm_A(CgPlan P, CgBufs<S> B, int raw, i
           } else {
             S pv;
             if (j == 0) {
-              pv = a1;
+              pv = (S)(vm[k] * (double)a1);   // r_0 = p_0 = M grad
+              if constexpr (EL == 1) {
+                ry = S(0), rr_ = pv, rp = pv;
+              } else {
+                ry[h] = S(0), rr_[h] = pv, rp[h] = pv;
+              }
             } else {
               const S rv = a1 - aS * lane_at<S, EL>(vz[k], h);
               pv = (S)((double)rv + beta * (double)(j == 1 ? rv : ao));
@@ -683,8 +692,10 @@ int cg_multi_t(tfx_ctx* ctx, const Shape& s, const S* F, const S* P1, const S* P
   TFX_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(kZ, dim3(k2), dim3(NTH), 0, st, P, B, 1, 0, jtjg);
   TFX_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_cgm_start<S>, dim3(nbv), dim3(NTH), 0, st, P, B);
-  TFX_CHECK_LAUNCH(ctx);
+  if (maxiter <= 0) {   // otherwise K1(0) forms r_0, p_0 and y = 0 itself
+    hipLaunchKernelGGL(k_cgm_start<S>, dim3(nbv), dim3(NTH), 0, st, P, B);
+    TFX_CHECK_LAUNCH(ctx);
+  }
   for (int jj = 0; jj < maxiter; ++jj) {
     hipLaunchKernelGGL(kA, dim3(k1), dim3(NTH), 0, st, P, B, 0, jj);
     TFX_CHECK_LAUNCH(ctx);
