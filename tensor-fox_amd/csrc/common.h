@@ -24,9 +24,12 @@ struct tfx_ctx {
 
 // Context scratch: 8 doubles of counters, then the per-rank partials of tfx_dgn_update (4 doubles
 // per rank).  Counters (each the first 4 bytes of its double slot, reset by the kernel that uses
-// it): slot 0 the rank-update kernel's arrivals, slot TFX_DSCAL_RES_COUNTER the residual's.
+// it): slot 0 the rank-update kernel's arrivals, slot TFX_DSCAL_RES_COUNTER the residual's,
+// slot TFX_DSCAL_GRAM_COUNTER the Gram products'.  The space after the counters holds one
+// kernel's transient partials at a time (stream order).
 constexpr size_t TFX_DSCAL_BYTES = (size_t)64 << 10;
 constexpr int TFX_DSCAL_RES_COUNTER = 1;
+constexpr int TFX_DSCAL_GRAM_COUNTER = 2;
 constexpr int TFX_UPD_MAX_RANKS = (int)((TFX_DSCAL_BYTES / 8 - 8) / 4);
 
 int tfx_set_error(tfx_ctx* ctx, int code, const char* fmt, ...);

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(256) void k_gram_factors(Shape s, const S* __restri
 template <typename S>
 __global__ __launch_bounds__(256) void k_gram_hadamard(Shape s, const S* __restrict__ Gr,
                                                         S* __restrict__ P1, S* __restrict__ P2,
-                                                        S* __restrict__ gamma) {
+                                                        S* __restrict__ gamma, S* __restrict__ pmax,
+                                                        unsigned* __restrict__ counter) {
   // one thread per (l, entry): the L mode Grams of the entry are loaded first, then
   // P1[l] = prod_{k != l} Gr_k and P2[l][m] = prod_{k != l, m} Gr_k (mode order, as the reference
   // forms them); workgroup 0 also forms gamma from max |P1_{L-1}| (GaussNewton.py:584-606),
@@ -91,6 +92,7 @@ __global__ __launch_bounds__(256) void k_gram_hadamard(Shape s, const S* __restr
   const int L = s.L, R = s.R;
   const int64_t RR = (int64_t)R * R;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  S lmx = S(0);   // max |P1_{L-1}| over this workgroup's entries
   if (e < (int64_t)L * RR) {
     const int l = (int)(e / RR);
     const int64_t rc = e - (int64_t)l * RR;
@@ -102,6 +104,7 @@ __global__ __launch_bounds__(256) void k_gram_hadamard(Shape s, const S* __restr
     for (int k = 0; k < TFX_MAXL; ++k)
       if (k < L && k != l) p *= g[k];
     P1[e] = p;
+    if (l == L - 1) lmx = fabs(p);
     for (int m = 0; m < L; ++m) {
       S q = S(1);
       if (m != l)
@@ -111,20 +114,40 @@ __global__ __launch_bounds__(256) void k_gram_hadamard(Shape s, const S* __restr
       P2[((int64_t)l * L + m) * RR + rc] = q;
     }
   }
-  if (blockIdx.x != 0) return;   // uniform: the rest is workgroup 0's
   S mx = S(0);
-  for (int64_t rc = threadIdx.x; rc < RR; rc += blockDim.x) {
-    S g[TFX_MAXL];
+  if (counter) {
+    // every workgroup stores the max of the P1_{L-1} entries it formed; the last one to arrive
+    // takes the max of those (a max is exact, so the value is the one max |P1_{L-1}| has)
+    lmx = block_max(lmx, red);
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+      pmax[blockIdx.x] = lmx;
+      __threadfence();
+      is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+      const S v = __builtin_nontemporal_load(pmax + b);
+      mx = v > mx ? v : mx;
+    }
+    mx = block_max(mx, red);
+  } else {
+    if (blockIdx.x != 0) return;   // uniform: the rest is workgroup 0's
+    for (int64_t rc = threadIdx.x; rc < RR; rc += blockDim.x) {
+      S g[TFX_MAXL];
 #pragma unroll
-    for (int k = 0; k < TFX_MAXL; ++k) g[k] = Gr[(int64_t)(k < L - 1 ? k : 0) * RR + rc];
-    S p = S(1);
+      for (int k = 0; k < TFX_MAXL; ++k) g[k] = Gr[(int64_t)(k < L - 1 ? k : 0) * RR + rc];
+      S p = S(1);
 #pragma unroll
-    for (int k = 0; k < TFX_MAXL; ++k)
-      if (k < L - 1) p *= g[k];
-    const S v = fabs(p);
-    mx = v > mx ? v : mx;
+      for (int k = 0; k < TFX_MAXL; ++k)
+        if (k < L - 1) p *= g[k];
+      const S v = fabs(p);
+      mx = v > mx ? v : mx;
+    }
+    mx = block_max(mx, red);
   }
-  mx = block_max(mx, red);
   for (int e2 = threadIdx.x; e2 < L * R; e2 += blockDim.x) {
     const int l = e2 / R, r = e2 % R;
     S p = S(1);
@@ -132,6 +155,7 @@ __global__ __launch_bounds__(256) void k_gram_hadamard(Shape s, const S* __restr
       if (k != l) p *= Gr[k * RR + (int64_t)r * R + r];
     gamma[e2] = mx * sqrt(fabs(p));
   }
+  if (counter && threadIdx.x == 0) *counter = 0u;
 }
 
 // grad = N - F_l P1_l (minus the gradient), flat layout.
@@ -852,8 +876,14 @@ int gramians_t(tfx_ctx* ctx, const Shape& s, const S* F, S* Gr, S* P1, S* P2, S*
                hipStream_t st) {
   hipLaunchKernelGGL(k_gram_factors<S>, dim3(s.L * s.R), dim3(256), 0, st, s, F, Gr);
   TFX_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_gram_hadamard<S>, dim3((unsigned)(((int64_t)s.L * s.R * s.R + 255) / 256)),
-                     dim3(256), 0, st, s, Gr, P1, P2, gamma);
+  // the per-workgroup maxima go to the context scratch after its counters (transient, like the
+  // update's partials there); very large products fall back to workgroup 0 forming the max
+  const int64_t nwg = ((int64_t)s.L * s.R * s.R + 255) / 256;
+  const bool part = nwg <= (int64_t)(TFX_DSCAL_BYTES / 8 - 8);
+  S* pmax = part ? (S*)(ctx->dscal + 8) : nullptr;
+  unsigned* counter = part ? (unsigned*)(ctx->dscal + TFX_DSCAL_GRAM_COUNTER) : nullptr;
+  hipLaunchKernelGGL(k_gram_hadamard<S>, dim3((unsigned)nwg), dim3(256), 0, st, s, Gr, P1, P2, gamma,
+                     pmax, counter);
   TFX_CHECK_LAUNCH(ctx);
   return TFX_OK;
 }
