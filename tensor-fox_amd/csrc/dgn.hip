@@ -174,6 +174,7 @@ __global__ void k_neg_grad(Shape s, const S* __restrict__ F, const S* __restrict
   // four partial sums (independent loads and FMA chains), combined in a fixed order
   S a0 = S(0), a1 = S(0), a2 = S(0), a3 = S(0);
   int c = 0;
+#pragma unroll 4   // four steps' loads in flight together; each accumulator's order unchanged
   for (; c + 4 <= s.R; c += 4) {
     a0 += Fl[(int64_t)c * I + i] * P[(int64_t)c * s.R + r];
     a1 += Fl[(int64_t)(c + 1) * I + i] * P[(int64_t)(c + 1) * s.R + r];

@@ -187,6 +187,7 @@ __global__ __launch_bounds__(256) void k_m64_sum(M64Plan P, const double* __rest
     const double* P1l = P.P1 + (int64_t)l * s.R * s.R;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int c = 0;
+#pragma unroll 4   // four steps' loads in flight together; each accumulator's order unchanged
     for (; c + 4 <= s.R; c += 4) {
       a0 += Fl[(int64_t)c * I + i] * P1l[(int64_t)c * s.R + r];
       a1 += Fl[(int64_t)(c + 1) * I + i] * P1l[(int64_t)(c + 1) * s.R + r];
