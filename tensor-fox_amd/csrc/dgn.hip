@@ -633,7 +633,9 @@ __global__ __launch_bounds__(UPD_NT) void k_dgn_update_rk(
   __shared__ double cn[TFX_MAXL], sc[TFX_MAXL];
   __shared__ int is_last;
   int64_t maxd = 0;
-  for (int l = 0; l < L; ++l) maxd = s.d[l] > maxd ? s.d[l] : maxd;
+#pragma unroll
+  for (int l = 0; l < TFX_MAXL; ++l)   // constant indices: the shape is read at kernel entry
+    if (l < L) maxd = s.d[l] > maxd ? s.d[l] : maxd;
   const bool cached = maxd <= (int64_t)UPD_NT * UPD_VC;
   double gain = 0.0, dy2 = 0.0, x2 = 0.0, gmax = 0.0;
   double ss[TFX_MAXL];
