@@ -528,42 +528,72 @@ struct SumJobs {
 
 constexpr int SUM_WAVES = 4;
 
+constexpr int SUM_ROWS = 4;   // rows per workgroup: 16 lanes x 4 ranks per row, 16-byte loads
+
+// Output (x, q) of a job = the sum of its np partials part[p][x][q], p ascending: wave w sums
+// the partials of quarter w, and wave 0 adds the quarters in wave order.  A workgroup takes rows
+// x0 .. x0 + 3 of one job, lane (x0 + lane / 16, ranks 4 (lane % 16) .. + 3).
 __global__ __launch_bounds__(64 * SUM_WAVES) void k_sum_parts(SumJobs J, int rc0, int R,
                                                              double* __restrict__ out,
                                                              double* __restrict__ Ps) {
-  __shared__ double red[SUM_WAVES - 1][64];
+  __shared__ double red[SUM_WAVES - 1][4][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int64_t e = (int64_t)blockIdx.x * 64 + lane;
+  // job b and its first tile, with constant indices into the job table
+  const int64_t blk = blockIdx.x;
   int b = 0;
-  while (b < J.n && e >= J.job[b].rows * RC) e -= J.job[b++].rows * RC;
+  int64_t base = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int64_t t = (J.job[k].rows + SUM_ROWS - 1) / SUM_ROWS;
+    if (k < J.n && b == k && blk >= base + t) {
+      base += t;
+      b = k + 1;
+    }
+  }
   if (b >= J.n) return;     // uniform over the workgroup
-  const SumJob& jb = J.job[b];
-  const int64_t x = e / RC;
-  const int q = (int)(e % RC);
-  const bool live = jb.to_ps || rc0 + q < R;
-  const int64_t stride = jb.rows * RC;
-  const int p0 = (int)((int64_t)jb.np * w / SUM_WAVES);
-  const int p1 = live ? (int)((int64_t)jb.np * (w + 1) / SUM_WAVES) : p0;
-  const float* src = jb.part + e;
-  double acc = 0.0;
+  const float* part = b == 0 ? J.job[0].part : (b == 1 ? J.job[1].part : J.job[2].part);
+  const int np = b == 0 ? J.job[0].np : (b == 1 ? J.job[1].np : J.job[2].np);
+  const int to_ps = b == 0 ? J.job[0].to_ps : (b == 1 ? J.job[1].to_ps : J.job[2].to_ps);
+  const int64_t rows = b == 0 ? J.job[0].rows : (b == 1 ? J.job[1].rows : J.job[2].rows);
+  const int64_t off = b == 0 ? J.job[0].off : (b == 1 ? J.job[1].off : J.job[2].off);
+  const int64_t x = (blk - base) * SUM_ROWS + (lane >> 4);
+  const int qd = lane & 15;
+  const bool xok = x < rows;
+  const int64_t stride = rows * RC;
+  const int p0 = (int)((int64_t)np * w / SUM_WAVES);
+  const int p1 = (int)((int64_t)np * (w + 1) / SUM_WAVES);
+  const float* src = part + (xok ? x : 0) * RC + 4 * qd;
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   int p = p0;
   for (; p + 8 <= p1; p += 8) {
-    float v[8];
+    f32x4 v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)(p + u) * stride];
+    for (int u = 0; u < 8; ++u) v[u] = *(const f32x4*)(src + (int64_t)(p + u) * stride);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc += (double)v[u];
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int h = 0; h < 4; ++h) acc[h] += (double)v[u][h];
   }
-  for (; p < p1; ++p) acc += (double)src[(int64_t)p * stride];
-  if (w > 0) red[w - 1][lane] = acc;
-  __syncthreads();
-  if (w == 0 && live) {
+  for (; p < p1; ++p) {
+    const f32x4 v = *(const f32x4*)(src + (int64_t)p * stride);
 #pragma unroll
-    for (int u = 0; u < SUM_WAVES - 1; ++u) acc += red[u][lane];
-    if (jb.to_ps)
-      Ps[e] = acc;
-    else
-      out[jb.off + (int64_t)(rc0 + q) * jb.rows + x] = acc;
+    for (int h = 0; h < 4; ++h) acc[h] += (double)v[h];
+  }
+  if (w > 0)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) red[w - 1][h][lane] = acc[h];
+  __syncthreads();
+  if (w != 0 || !xok) return;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+#pragma unroll
+    for (int u = 0; u < SUM_WAVES - 1; ++u) acc[h] += red[u][h][lane];
+    const int q = 4 * qd + h;
+    if (to_ps)
+      Ps[x * RC + q] = acc[h];
+    else if (rc0 + q < R)
+      out[off + (int64_t)(rc0 + q) * rows + x] = acc[h];
   }
 }
 
@@ -700,9 +730,9 @@ int split16_run(tfx_ctx* ctx, const float* T, MPlan P, const SplitScratch& W, co
     const bool outer = L > 2 && (mask & ((1u << (L - 2)) - 1u));
     if (outer) J.job[J.n++] = SumJob{Po, P.ngrp, L > 3 ? 1 : 0, P.NO, s.off[0]};
     int64_t tot = 0;
-    for (int b = 0; b < J.n; ++b) tot += J.job[b].rows * RC;
+    for (int b = 0; b < J.n; ++b) tot += (J.job[b].rows + SUM_ROWS - 1) / SUM_ROWS;
     if (tot > 0) {
-      hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)(tot / 64)), dim3(64 * SUM_WAVES), 0, stream,
+      hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)tot), dim3(64 * SUM_WAVES), 0, stream,
                          J, rc0, s.R, out, Ps);
       TFX_CHECK_LAUNCH(ctx);
     }
